@@ -1,0 +1,226 @@
+/*
+  librsfg -- MI355X (gfx950) implementation of the rsf_solve dense
+  quasi-dynamic cycle hot path, behind a plain C ABI.
+
+  The reference (thwbecker/interact) runs this path on the CPU through PETSc:
+  operator assembly in calc_petsc_Isn_matrices / GenKEntries_petsc, the
+  stress product MatMult(Is, v), the RHS callback rsf_ODE_RHSFunction with
+  rsf_domain_check, and the TSRK/TSAdapt/TSEvent loop with the rsf monitor.
+  Each entry point below names the reference interface it replaces.
+
+  Conventions
+   - every function returns 0 (RSFG_OK) or a negative error code; nothing
+     calls exit(); rsfg_last_error() gives a message;
+   - host pointers unless a name ends in _dev (device pointers on the
+     handle's GPU);
+   - a handle owns one device, one HIP stream and all device memory it
+     allocates; handles are independent (several may be driven from
+     separate host threads); calls on one handle must not overlap;
+   - state vectors use the reference's patch-major layout
+     (src/rsf_solve.c:364-372): scalar slip mode (psi, tau, sigma, slip)
+     per patch; two-component mode (psi, tau_s, tau_d, sigma, slip_s,
+     slip_d) per patch.
+*/
+#ifndef RSFG_H
+#define RSFG_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RSFG_OK 0
+#define RSFG_EINVAL -1   /* bad argument */
+#define RSFG_ENOMEM -2   /* device or host allocation failed */
+#define RSFG_EHIP -3     /* HIP runtime error */
+#define RSFG_ESTATE -4   /* call out of order (e.g. solve before assemble) */
+#define RSFG_ESTORM -5   /* domain-check rejection storm (src/rsf_engine.c:336-358) */
+#define RSFG_EDOMAIN -6  /* non-finite or out-of-domain accepted state */
+
+/* slip modes, as the reference's MODE_TYPE STRIKE/DIP/RAKE
+   (-rsf_slip_mode, src/rsf_init.c:459-468) plus the two-component mode */
+#define RSFG_SLIP_STRIKE 0
+#define RSFG_SLIP_DIP 1
+#define RSFG_SLIP_RAKE 3
+#define RSFG_SLIP_VECTOR 8   /* strike and dip slip both free: 2 DOF/patch */
+
+/* storage precisions of the operator K (bit mask for rsfg_assemble) */
+#define RSFG_PREC_FP64 1     /* master copy, always kept */
+#define RSFG_PREC_FP32 2     /* FP32 entries, FP32 accumulation */
+#define RSFG_PREC_BF16X2 4   /* bf16 hi + bf16 lo pair (hi+lo = FP32 value
+                                rounded to 16 significant bits), FP32 acc. */
+#define RSFG_PREC_BF16 8     /* bf16 hi only */
+
+/* time integrators (PETSc -ts_rk_type 5dp / 3bs) */
+#define RSFG_RK5DP 0
+#define RSFG_RK3BS 1
+
+/* one fault patch: a rectangle as in the reference's patch file
+   "x y z strike dip L W group" (src/read_geometry.c:115-117, strike and
+   dip kept as float like struct flt, src/includes/structures.h:69) */
+typedef struct {
+  double x, y, z;       /* centroid [m], z <= 0 in a half space */
+  float strike, dip;    /* degrees */
+  double l, w;          /* half length, half width [m] */
+  double rake[2];       /* (strike, dip) fractions for RSFG_SLIP_RAKE */
+  int group;
+} rsfg_patch;
+
+typedef struct rsfg_ctx rsfg_handle;
+
+const char *rsfg_last_error(void);
+/* library version string */
+const char *rsfg_version(void);
+
+/* number of visible devices (0 without a GPU) */
+int rsfg_device_count(int *count);
+
+/* Binds patches to a device and uploads the geometry (replaces the patch
+   part of read_geometry, src/read_geometry.c:23-700, whose per-patch
+   frames are recomputed here in FP64 from the float angles). */
+int rsfg_create(int device, const rsfg_patch *patches, int n,
+                rsfg_handle **h);
+int rsfg_destroy(rsfg_handle *h);
+
+/* Reads a reference patch file (rectangles; format above). On entry *n is
+   the capacity of patches; on return the number read. */
+int rsfg_read_patch_file(const char *fname, rsfg_patch *patches, int *n);
+
+/* ---- operator assembly ----------------------------------------------
+   Replaces the dense branch of calc_petsc_Isn_matrices
+   (src/petsc_matrix_operations.c:379-503, MatScale at :735) and
+   GenKEntries_petsc (src/interact_matrix_assembly.c:102-151): one GPU
+   thread per (receiver, source) pair evaluates the Okada (1992) stress at
+   the receiver centroid for unit strike and unit dip slip on the source,
+   resolves tractions on the receiver plane, scales, and stores.
+   Rows are receivers, columns sources.  Scalar slip modes give the N x N
+   "Is" operator; RSFG_SLIP_VECTOR gives 2N x 2N with dof 2p+c, c = 0
+   strike, 1 dip.  with_normal adds the "In" rows (normal traction,
+   compression positive, i.e. scaled by -scale, src/rsf_solve.c:245-247).
+   scale is G_SI / 1e4 (src/rsf_solve.c:235-237).  Singular pairs store 0. */
+typedef struct {
+  int slip_mode;        /* RSFG_SLIP_* */
+  int with_normal;      /* 0/1 */
+  int full_space;       /* 0 half space (default), 1 real-source term only */
+  int store;            /* RSFG_PREC_* mask (FP64 always kept) */
+  double scale;
+} rsfg_assembly;
+
+int rsfg_assemble(rsfg_handle *h, const rsfg_assembly *a, double *gpu_ms);
+
+/* operator dimensions after assembly: rows (shear), cols, leading dim */
+int rsfg_operator_dims(rsfg_handle *h, int *rows, int *cols, int *ld);
+
+/* GenKEntries_petsc semantics (src/interact_matrix_assembly.c:102): the
+   m x k block of the assembled FP64 shear operator with rows[] and cols[]
+   (0-based), written column-major into out (out[r + m*c]). */
+int rsfg_entries(rsfg_handle *h, int m, const int *rows, int k,
+                 const int *cols, double *out);
+
+/* Unscaled traction blocks for arbitrary (receiver, source) pairs computed
+   directly by the device kernel, internal units (shear modulus 1e4):
+   out[6*p + 3*s + t] for slip s (strike, dip) and traction t (strike,
+   dip, normal).  A bulk ckernel_func (src/eval_green.c:368-381) for
+   CPU-side H-matrix builders. */
+int rsfg_traction_pairs(rsfg_handle *h, long npairs, const int *recv,
+                        const int *src, int full_space, double *out);
+
+/* ---- stress product ---------------------------------------------------
+   MatMult(Is, x, y) (src/rsf_engine.c:168-170): y = K x with the stored
+   operator of precision prec (one RSFG_PREC_* bit).  FP64 accumulates in
+   FP64, the others in FP32.  x has cols entries, y rows entries. */
+int rsfg_matvec(rsfg_handle *h, int prec, const double *x, double *y);
+int rsfg_matvec_dev(rsfg_handle *h, int prec, const void *x_dev,
+                    void *y_dev);
+/* transposed product K^T x (adjoint) */
+int rsfg_matvec_t(rsfg_handle *h, int prec, const double *x, double *y);
+/* time nrep back-to-back products on device-resident data; returns the
+   mean kernel time [ms] measured with HIP events on the handle's stream */
+int rsfg_matvec_bench(rsfg_handle *h, int prec, int nrep, double *ms);
+
+/* ---- rate-and-state problem -------------------------------------------
+   Parameters of rsf_get_settings (src/rsf_init.c:273-503) that the RHS
+   uses; per-patch a, b (read_rsf, src/interact/input.c:83) and D_c
+   (-rsf_dc_file). */
+typedef struct {
+  int law;              /* 1 aging (default), 2 slip, 3 PRZ, 4 Sato, 5 KT */
+  double v0, f0;        /* reference speed [m/s], friction */
+  double eta;           /* radiation damping rd_fac G/(2 c_s) [Pa s/m] */
+  double vmin_state;    /* |v| floor of laws 2,4,5 (1e-16) */
+  double sato_beta;     /* 1e-2 */
+  double kt_vc;         /* Kato-Tullis Vc, default 1e-2 v0 (<= 0: default) */
+  int calc_sigma_dot;   /* evolve sigma with the In rows (-calc_sigma_dot) */
+  int limit_sigma;      /* HBI limitsigma */
+  double min_sigma, max_sigma;
+} rsfg_friction;
+
+int rsfg_set_friction(rsfg_handle *h, const rsfg_friction *f,
+                      const double *a, const double *b, const double *dc);
+
+/* backslip loading (src/rsf_solve.c:268-363): sinc = K (-vpl) with the
+   FP64 operator, per-patch plate rate vpl_patch (NULL: uniform vpl)
+   along the slip direction (strike component in the two-component
+   mode), plus optional additive per-patch rates taudot_add,
+   sigdot_add (-rsf_stress_rate_file; may be NULL). */
+int rsfg_set_loading(rsfg_handle *h, double vpl, const double *vpl_patch,
+                     const double *taudot_add, const double *sigdot_add);
+
+/* rsf_ODE_RHSFunction (src/rsf_engine.c:186-302) with the operator of
+   precision prec, and rsf_domain_check (:309-362) of X into *domain_ok.
+   X, F are patch-major AoS (see conventions). */
+int rsfg_rhs(rsfg_handle *h, int prec, const double *X, double *F,
+             int *domain_ok);
+
+/* ---- adaptive cycle integration ---------------------------------------
+   PETSc TSRK (5dp/3bs) + TSAdapt basic + TSEvent + rsf_TS_Monitor as set
+   up in src/rsf_solve.c:440-643, run entirely on the device: stage
+   kernels stream K with the RHS fused into their epilogue, and the
+   last workgroup of each step applies the step-size controller, locates
+   log10 max|v| threshold crossings on the step's dense output, writes
+   monitor and event rows, and advances the state. */
+typedef struct {
+  int method;           /* RSFG_RK5DP / RSFG_RK3BS */
+  int prec;             /* operator precision used in the stages */
+  double rtol;          /* scalar relative tolerance (-rtol) */
+  double atol[6];       /* per state component (psi, tau, ...) */
+  double dt_init, dt_max;           /* [s] */
+  double dt_monitor;    /* [s] monitor cadence, also caps the step */
+  double rdx_monitor, adx_monitor;
+  double vel_event, vel_event_hyst, rupture_vth;
+  long max_steps;
+  int max_domain_reject;            /* storm guard (<= 0 off) */
+  int steps_per_launch;             /* accepted-step attempts per graph */
+} rsfg_solver_opts;
+
+/* fills o with the reference defaults (src/rsf_init.c:331-350) */
+int rsfg_solver_defaults(rsfg_solver_opts *o);
+
+/* sets the state (AoS, reference layout) at time t */
+int rsfg_set_state(rsfg_handle *h, double t, const double *X);
+int rsfg_get_state(rsfg_handle *h, double *t, double *X);
+
+typedef struct {
+  long steps, rejects, domain_rejects, rhs_evals, launches;
+  double t, dt;         /* time reached and next step size [s] */
+  int status;           /* 0 ok, RSFG_ESTORM, RSFG_EDOMAIN */
+  int nevent, nmonitor; /* rows available */
+  double wall_s;        /* host wall time of the solve call */
+} rsfg_stats;
+
+/* integrates from the current state to t_end (STEPOVER semantics) */
+int rsfg_solve(rsfg_handle *h, const rsfg_solver_opts *o, double t_end,
+               rsfg_stats *st);
+
+/* event rows (rsf_events.dat + catalog, src/rsf_output.c:1000-1150):
+   8 doubles each: t [s], +1 onset / -1 arrest, log10 max|v|, mean slip,
+   mean mu, peak |v| since onset, mean stress drop over ruptured cells
+   [Pa], ruptured cell count.  *n in: capacity, out: rows copied. */
+int rsfg_events(rsfg_handle *h, double *rows, int *n);
+/* monitor rows (rsf_monitor.dat, src/rsf_output.c:832-842): 10 doubles
+   each: step, t [s], t [yr], dt, log10 max|v|, mean slip, mean mu,
+   max sigma, min sigma, log10 mean|v| */
+int rsfg_monitor(rsfg_handle *h, double *rows, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,844 @@
+// C ABI of librsfg (declarations and reference mapping in include/rsfg.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gemv_dev.h"
+#include "rsf_dev.h"
+#include "rsfg_common.h"
+
+namespace rsfg {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "HIP error %d (%s) in %s at %s:%d", (int)e,
+           hipGetErrorString(e), what, file, line);
+  g_err = buf;
+  return RSFG_EHIP;
+}
+
+int launch_assemble(rsfg_ctx* h, double* gpu_ms);
+int launch_pairs(rsfg_ctx* h, long np, const int* d_recv, const int* d_src, int full,
+                 double* d_out);
+int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st);
+
+constexpr double kDeg2Rad = 0.0174532925199432957692369076848861271344287188854172545609719144;
+constexpr double kOkEps = 1.0e-8;
+constexpr size_t kChunkMax = 512;   // largest GEMV chunk (bf16: 64 lanes x 8)
+
+int ensure_tmp(rsfg_ctx* h, size_t bytes) {
+  if (bytes <= h->tmp_bytes) return RSFG_OK;
+  if (h->d_tmp) hipFree(h->d_tmp);
+  h->d_tmp = nullptr;
+  h->tmp_bytes = 0;
+  RSFG_HIP(hipMalloc(&h->d_tmp, bytes));
+  h->tmp_bytes = bytes;
+  return RSFG_OK;
+}
+
+const void* operator_ptr(rsfg_ctx* h, int prec) {
+  if (!h->assembled) return nullptr;
+  switch (prec) {
+    case RSFG_PREC_FP64: return h->K64;
+    case RSFG_PREC_FP32: return h->K32;
+    case RSFG_PREC_BF16X2: return h->Kbx2;
+    case RSFG_PREC_BF16: return h->Kb16;
+    default: return nullptr;
+  }
+}
+
+static size_t vin_elems_for(int cols) {
+  return ((size_t)cols + 2 * kChunkMax + kChunkMax - 1) / kChunkMax * kChunkMax;
+}
+
+static void free_operator(rsfg_ctx* h) {
+  hipFree(h->K64);
+  hipFree(h->K32);
+  hipFree(h->Kbx2);
+  hipFree(h->Kb16);
+  h->K64 = nullptr;
+  h->K32 = nullptr;
+  h->Kbx2 = nullptr;
+  h->Kb16 = nullptr;
+  h->assembled = false;
+}
+
+static void free_state(rsfg_ctx* h) {
+  hipFree(h->d_state);
+  hipFree(h->partials);
+  hipFree(h->d_ev);
+  hipFree(h->d_mon);
+  hipFree(h->d_cat);
+  hipFree(h->d_rupt);
+  h->d_state = nullptr;
+  h->partials = nullptr;
+  h->d_ev = h->d_mon = h->d_cat = nullptr;
+  h->d_rupt = nullptr;
+}
+
+int alloc_state(rsfg_ctx* h) {
+  free_state(h);
+  const size_t n = h->n, dim = h->dim;
+  const size_t vec = dim * n;
+  h->vin_elems = vin_elems_for(h->cols);
+  const size_t total = 13 * vec + 2 * h->vin_elems;
+  RSFG_HIP(hipMalloc(&h->d_state, total * sizeof(double)));
+  RSFG_HIP(hipMemsetAsync(h->d_state, 0, total * sizeof(double), h->stream));
+  double* p = h->d_state;
+  for (int i = 0; i < 2; ++i) { h->Xb[i] = p; p += vec; }
+  for (int i = 0; i < 2; ++i) { h->Yb[i] = p; p += vec; }
+  for (int i = 0; i < 8; ++i) { h->Stg[i] = p; p += vec; }
+  h->Xmon = p; p += vec;
+  for (int i = 0; i < 2; ++i) { h->vin[i] = p; p += h->vin_elems; }
+  h->partial_count = (size_t)(n + 1) * kNRed;
+  RSFG_HIP(hipMalloc(&h->partials, h->partial_count * sizeof(double)));
+  RSFG_HIP(hipMalloc(&h->d_ev, (size_t)kMaxEv * 8 * sizeof(double)));
+  RSFG_HIP(hipMalloc(&h->d_mon, (size_t)kMaxMon * 10 * sizeof(double)));
+  RSFG_HIP(hipMalloc(&h->d_cat, 2 * n * sizeof(double)));
+  RSFG_HIP(hipMalloc(&h->d_rupt, n * sizeof(int)));
+  h->xcur = 0;
+  return RSFG_OK;
+}
+
+// ---------------------------------------------------------------- kernels
+template <typename VT>
+__global__ void k_to_vt(const double* __restrict__ x, VT* __restrict__ y, int n, int npad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < npad) y[i] = (i < n) ? static_cast<VT>(x[i]) : VT(0);
+}
+
+template <typename TK, int R>
+__global__ __launch_bounds__(256) void k_matvec(const TK* __restrict__ K, long ld, int rows,
+                                                int cols,
+                                                const typename KTraits<TK>::VT* __restrict__ x,
+                                                double* __restrict__ y) {
+  __shared__ double rowsum[R];
+  __shared__ double red[4 * R];
+  const int row0 = blockIdx.x * R;
+  gemv_rows<TK, R>(K, ld, row0, rows, cols, x, rowsum, red);
+  if (threadIdx.x < R && row0 + (int)threadIdx.x < rows) y[row0 + threadIdx.x] = rowsum[threadIdx.x];
+}
+
+// y_part[split][j] = sum over rows of this split of K[i][j] x[i]
+template <typename TK>
+__global__ __launch_bounds__(256) void k_matvec_t(const TK* __restrict__ K, long ld, int rows,
+                                                  int cols, const double* __restrict__ x,
+                                                  int rows_per_split, double* __restrict__ part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i0 = blockIdx.y * rows_per_split;
+  const int i1 = min(rows, i0 + rows_per_split);
+  if (j >= cols) return;
+  double acc = 0.0;
+  for (int i = i0; i < i1; ++i) {
+    double kv;
+    const TK e = K[(size_t)i * ld + j];
+    if constexpr (sizeof(TK) == 8) kv = static_cast<double>(e);
+    else if constexpr (sizeof(TK) == 4 && std::is_same<TK, float>::value) kv = e;
+    else if constexpr (sizeof(TK) == 4) kv = (double)bf_hi(e) + (double)bf_lo(e);
+    else kv = (double)__uint_as_float(((uint32_t)e) << 16);
+    acc = fma(kv, x[i], acc);
+  }
+  part[(size_t)blockIdx.y * cols + j] = acc;
+}
+
+__global__ void k_sum_splits(const double* __restrict__ part, int nsplit, int cols,
+                             double* __restrict__ y) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= cols) return;
+  double s = 0.0;
+  for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * cols + j];
+  y[j] = s;
+}
+
+__global__ void k_pick_shear(const double* __restrict__ yall, int n, int nc, int G,
+                             double* __restrict__ y) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n * nc) return;
+  y[r] = yall[(size_t)(r / nc) * G + (r % nc)];
+}
+
+__global__ void k_gather(const double* __restrict__ K, long ld, int m, const int* rows, int k,
+                         const int* cols, double* out) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)m * k) return;
+  const int r = (int)(t % m), c = (int)(t / m);
+  out[t] = K[(size_t)rows[r] * ld + cols[c]];
+}
+
+// velocities of a SoA state (for rsfg_rhs)
+template <typename VT, int NC>
+__global__ void k_vel(const double* __restrict__ Y, const double* __restrict__ coef,
+                      FricParams fp, int n, VT* __restrict__ v, int* __restrict__ ok) {
+  constexpr int D = 2 * NC + 2;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double y[D], vv[2];
+  for (int f = 0; f < D; ++f) y[f] = Y[f * n + p];
+  PatchCoef c{coef[p], coef[n + p], coef[2 * n + p]};
+  rsf_patch_vel<NC>(fp, c.a, y, vv);
+  for (int k = 0; k < NC; ++k) v[NC * p + k] = static_cast<VT>(vv[k]);
+  ok[p] = rsf_patch_ok<NC>(fp, c, y) ? 1 : 0;
+}
+
+template <typename TK, int NC, int G>
+__global__ __launch_bounds__(256) void k_rhs(const TK* __restrict__ K, long ld, int rows, int cols,
+                                             const typename KTraits<TK>::VT* __restrict__ v,
+                                             const double* __restrict__ Y,
+                                             const double* __restrict__ coef,
+                                             const double* __restrict__ sinc, FricParams fp, int n,
+                                             double* __restrict__ F) {
+  constexpr int PPW = (G == 3) ? 2 : (G == 1 ? 8 : 4);
+  constexpr int R = G * PPW, D = 2 * NC + 2;
+  __shared__ double rowsum[R];
+  __shared__ double red[4 * R];
+  gemv_rows<TK, R>(K, ld, blockIdx.x * R, rows, cols, v, rowsum, red);
+  const int t = threadIdx.x;
+  const int p = blockIdx.x * PPW + t;
+  if (t >= PPW || p >= n) return;
+  double y[D], f[D], td[NC], sn[NC];
+  for (int k = 0; k < D; ++k) y[k] = Y[k * n + p];
+  for (int k = 0; k < NC; ++k) {
+    td[k] = rowsum[t * G + k];
+    sn[k] = sinc[NC * p + k];
+  }
+  const double sdn = (G > NC) ? rowsum[t * G + NC] : 0.0;
+  PatchCoef c{coef[p], coef[n + p], coef[2 * n + p]};
+  rsf_patch_rhs<NC>(fp, c, y, td, sdn, sn, sinc[NC * n + p], f);
+  for (int k = 0; k < D; ++k) F[k * n + p] = f[k];
+}
+
+template <typename TK>
+static int matvec_t_launch(rsfg_ctx* h, const TK* K, const double* d_x, double* d_y,
+                           double* d_part, int nsplit) {
+  const int rows = h->rows_all, cols = h->cols;
+  const int rps = (rows + nsplit - 1) / nsplit;
+  dim3 grid((cols + 255) / 256, nsplit);
+  hipLaunchKernelGGL(k_matvec_t<TK>, grid, dim3(256), 0, h->stream, K, (long)h->ld, rows, cols,
+                     d_x, rps, d_part);
+  hipLaunchKernelGGL(k_sum_splits, dim3((cols + 255) / 256), dim3(256), 0, h->stream, d_part,
+                     nsplit, cols, d_y);
+  RSFG_HIP(hipGetLastError());
+  return RSFG_OK;
+}
+
+template <typename TK>
+static int matvec_launch(rsfg_ctx* h, const TK* K, const void* d_xvt, double* d_y) {
+  constexpr int R = 8;
+  using VT = typename KTraits<TK>::VT;
+  const int nb = (h->rows_all + R - 1) / R;
+  hipLaunchKernelGGL((k_matvec<TK, R>), dim3(nb), dim3(256), 0, h->stream, K, (long)h->ld,
+                     h->rows_all, h->cols, static_cast<const VT*>(d_xvt), d_y);
+  RSFG_HIP(hipGetLastError());
+  return RSFG_OK;
+}
+
+static int matvec_any(rsfg_ctx* h, int prec, const void* d_xvt, double* d_y) {
+  switch (prec) {
+    case RSFG_PREC_FP64: return matvec_launch<double>(h, h->K64, d_xvt, d_y);
+    case RSFG_PREC_FP32: return matvec_launch<float>(h, h->K32, d_xvt, d_y);
+    case RSFG_PREC_BF16X2: return matvec_launch<bf16x2_t>(h, h->Kbx2, d_xvt, d_y);
+    case RSFG_PREC_BF16: return matvec_launch<bf16_t>(h, h->Kb16, d_xvt, d_y);
+  }
+  set_error("matvec: bad precision");
+  return RSFG_EINVAL;
+}
+
+// x (FP64, device) -> the operator's input format in vin[slot], zero padded
+static int stage_input(rsfg_ctx* h, int prec, const double* d_x, void* dst) {
+  const int npad = (int)h->vin_elems;
+  if (prec == RSFG_PREC_FP64)
+    hipLaunchKernelGGL(k_to_vt<double>, dim3((npad + 255) / 256), dim3(256), 0, h->stream, d_x,
+                       static_cast<double*>(dst), h->cols, npad);
+  else
+    hipLaunchKernelGGL(k_to_vt<float>, dim3((npad + 255) / 256), dim3(256), 0, h->stream, d_x,
+                       static_cast<float*>(dst), h->cols, npad);
+  RSFG_HIP(hipGetLastError());
+  return RSFG_OK;
+}
+
+static int prec_ok(rsfg_ctx* h, int prec) {
+  return operator_ptr(h, prec) != nullptr;
+}
+
+}  // namespace rsfg
+
+using namespace rsfg;
+
+extern "C" {
+
+const char* rsfg_last_error(void) { return g_err.c_str(); }
+
+const char* rsfg_version(void) { return "librsfg 0.1 (gfx950)"; }
+
+int rsfg_device_count(int* count) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  if (count) *count = c;
+  return RSFG_OK;
+}
+
+int rsfg_read_patch_file(const char* fname, rsfg_patch* patches, int* n) {
+  RSFG_CHECK(fname && patches && n, RSFG_EINVAL, "rsfg_read_patch_file: null argument");
+  FILE* in = fopen(fname, "r");
+  RSFG_CHECK(in != nullptr, RSFG_EINVAL, std::string("cannot open ") + fname);
+  int k = 0;
+  rsfg_patch p;
+  memset(&p, 0, sizeof p);
+  while (k < *n && fscanf(in, "%lf %lf %lf %f %f %lf %lf %i", &p.x, &p.y, &p.z, &p.strike,
+                          &p.dip, &p.l, &p.w, &p.group) == 8) {
+    p.rake[0] = 1.0;
+    p.rake[1] = 0.0;
+    patches[k++] = p;
+  }
+  fclose(in);
+  *n = k;
+  return RSFG_OK;
+}
+
+int rsfg_create(int device, const rsfg_patch* pt, int n, rsfg_handle** out) {
+  RSFG_CHECK(out && pt && n > 0, RSFG_EINVAL, "rsfg_create: bad arguments");
+  *out = nullptr;
+  int ndev = 0;
+  RSFG_HIP(hipGetDeviceCount(&ndev));
+  RSFG_CHECK(device >= 0 && device < ndev, RSFG_EINVAL, "rsfg_create: no such device");
+  for (int i = 0; i < n; ++i) {
+    RSFG_CHECK(pt[i].l > 0 && pt[i].w > 0, RSFG_EINVAL,
+               "rsfg_create: rectangles need positive half length and width");
+  }
+  RSFG_HIP(hipSetDevice(device));
+  rsfg_ctx* h = new rsfg_ctx();
+  h->device = device;
+  h->n = n;
+  h->h_geo = new double[(size_t)kGeoFields * n];
+  double* g = h->h_geo;
+  for (int i = 0; i < n; ++i) {
+    const rsfg_patch& q = pt[i];
+    // angle range fix-up of check_fault_angles (src/geometry/geometry.c:985-1015)
+    double dip = (double)q.dip, strike = (double)q.strike;
+    if (dip > 90.0) { dip = 180.0 - dip; strike += 180.0; }
+    if (dip < 0.0) { dip = -dip; strike += 180.0; }
+    if (strike >= 360.0) strike -= 360.0;
+    if (strike < 0.0) strike = 360.0 + strike;
+    const float fdip = (float)dip, fstrike = (float)strike;
+    double sa, ca, sdp, cdp;
+    sincos((90.0 - (double)fstrike) * kDeg2Rad, &sa, &ca);
+    sincos((double)fdip * kDeg2Rad, &sdp, &cdp);
+    g[kGeoX * n + i] = q.x;
+    g[kGeoY * n + i] = q.y;
+    g[kGeoZ * n + i] = q.z;
+    g[kGeoCa * n + i] = ca;
+    g[kGeoSa * n + i] = sa;
+    // DC3D's own dip constants (DCCON0, src/green/dc3d.F:1161-1196)
+    double sd = sdp, cd = cdp;
+    if (fabs(cd) < kOkEps) {
+      cd = 0.0;
+      if (sd > 0.0) sd = 1.0;
+      if (sd < 0.0) sd = -1.0;
+    }
+    g[kGeoSd * n + i] = sd;
+    g[kGeoCd * n + i] = cd;
+    g[kGeoL * n + i] = q.l;
+    g[kGeoW * n + i] = q.w;
+    // calc_quad_base_vecs (src/geometry/geometry.c:128-148)
+    g[kGeoS0 * n + i] = ca;
+    g[kGeoS1 * n + i] = sa;
+    g[kGeoS2 * n + i] = 0.0;
+    g[kGeoN0 * n + i] = sa * sdp;
+    g[kGeoN1 * n + i] = -ca * sdp;
+    g[kGeoN2 * n + i] = cdp;
+    g[kGeoD0 * n + i] = -sa * cdp;
+    g[kGeoD1 * n + i] = ca * cdp;
+    g[kGeoD2 * n + i] = sdp;
+    g[kGeoR0 * n + i] = q.rake[0];
+    g[kGeoR1 * n + i] = q.rake[1];
+  }
+  hipError_t e = hipStreamCreate(&h->stream);
+  if (e == hipSuccess) e = hipMalloc(&h->d_geo, sizeof(double) * kGeoFields * n);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->d_geo, g, sizeof(double) * kGeoFields * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_ctl, sizeof(DevCtl), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&h->d_ctl, sizeof(DevCtl));
+  if (e != hipSuccess) {
+    int rc = hip_fail(e, "rsfg_create", __FILE__, __LINE__);
+    rsfg_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return RSFG_OK;
+}
+
+int rsfg_destroy(rsfg_handle* h) {
+  if (!h) return RSFG_OK;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  free_operator(h);
+  free_state(h);
+  hipFree(h->d_geo);
+  hipFree(h->d_coef);
+  hipFree(h->d_sinc);
+  hipFree(h->d_ctl);
+  hipFree(h->d_tmp);
+  if (h->h_ctl) hipHostFree(h->h_ctl);
+  if (h->ev0) hipEventDestroy(h->ev0);
+  if (h->ev1) hipEventDestroy(h->ev1);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete[] h->h_geo;
+  delete h;
+  return RSFG_OK;
+}
+
+int rsfg_assemble(rsfg_handle* h, const rsfg_assembly* a, double* gpu_ms) {
+  RSFG_CHECK(h && a, RSFG_EINVAL, "rsfg_assemble: null argument");
+  const int m = a->slip_mode;
+  RSFG_CHECK(m == RSFG_SLIP_STRIKE || m == RSFG_SLIP_DIP || m == RSFG_SLIP_RAKE ||
+                 m == RSFG_SLIP_VECTOR,
+             RSFG_EINVAL, "rsfg_assemble: unknown slip mode");
+  RSFG_HIP(hipSetDevice(h->device));
+  free_operator(h);
+  h->slip_mode = m;
+  h->ncomp = (m == RSFG_SLIP_VECTOR) ? 2 : 1;
+  h->with_normal = a->with_normal ? 1 : 0;
+  h->full_space = a->full_space ? 1 : 0;
+  h->store = a->store | RSFG_PREC_FP64;
+  h->scale = a->scale;
+  h->G = h->ncomp + h->with_normal;
+  h->rows_all = h->G * h->n;
+  h->cols = h->ncomp * h->n;
+  h->ld = (h->cols + 63) / 64 * 64;
+  h->kslack = 2 * kChunkMax;
+  const size_t ne = (size_t)h->rows_all * h->ld + h->kslack;
+  RSFG_HIP(hipMalloc(&h->K64, ne * sizeof(double)));
+  RSFG_HIP(hipMemsetAsync(h->K64, 0, ne * sizeof(double), h->stream));
+  if (h->store & RSFG_PREC_FP32) {
+    RSFG_HIP(hipMalloc(&h->K32, ne * sizeof(float)));
+    RSFG_HIP(hipMemsetAsync(h->K32, 0, ne * sizeof(float), h->stream));
+  }
+  if (h->store & RSFG_PREC_BF16X2) {
+    RSFG_HIP(hipMalloc(&h->Kbx2, ne * sizeof(bf16x2_t)));
+    RSFG_HIP(hipMemsetAsync(h->Kbx2, 0, ne * sizeof(bf16x2_t), h->stream));
+  }
+  if (h->store & RSFG_PREC_BF16) {
+    RSFG_HIP(hipMalloc(&h->Kb16, ne * sizeof(bf16_t)));
+    RSFG_HIP(hipMemsetAsync(h->Kb16, 0, ne * sizeof(bf16_t), h->stream));
+  }
+  int rc = launch_assemble(h, gpu_ms);
+  if (rc != RSFG_OK) return rc;
+  h->assembled = true;
+  h->have_loading = false;
+  h->have_state = false;
+  h->dim = 2 * h->ncomp + 2;
+  return RSFG_OK;
+}
+
+int rsfg_operator_dims(rsfg_handle* h, int* rows, int* cols, int* ld) {
+  RSFG_CHECK(h && h->assembled, RSFG_ESTATE, "rsfg_operator_dims: not assembled");
+  if (rows) *rows = h->ncomp * h->n;
+  if (cols) *cols = h->cols;
+  if (ld) *ld = h->ld;
+  return RSFG_OK;
+}
+
+// shear row r of the operator -> row of the combined (shear + normal) storage
+static int shear_row(rsfg_ctx* h, int r) { return (r / h->ncomp) * h->G + (r % h->ncomp); }
+
+int rsfg_entries(rsfg_handle* h, int m, const int* rows, int k, const int* cols, double* out) {
+  RSFG_CHECK(h && h->assembled, RSFG_ESTATE, "rsfg_entries: not assembled");
+  RSFG_CHECK(m >= 0 && k >= 0 && rows && cols && out, RSFG_EINVAL, "rsfg_entries: bad args");
+  if (m == 0 || k == 0) return RSFG_OK;
+  const int nshear = h->ncomp * h->n;
+  std::vector<int> rr(m), cc(k);
+  for (int i = 0; i < m; ++i) {
+    RSFG_CHECK(rows[i] >= 0 && rows[i] < nshear, RSFG_EINVAL, "rsfg_entries: row out of range");
+    rr[i] = shear_row(h, rows[i]);
+  }
+  for (int j = 0; j < k; ++j) {
+    RSFG_CHECK(cols[j] >= 0 && cols[j] < h->cols, RSFG_EINVAL, "rsfg_entries: col out of range");
+    cc[j] = cols[j];
+  }
+  RSFG_HIP(hipSetDevice(h->device));
+  const size_t ib = ((sizeof(int) * (m + k) + 7) / 8) * 8;
+  int rc = ensure_tmp(h, ib + sizeof(double) * (size_t)m * k);
+  if (rc) return rc;
+  int* d_r = reinterpret_cast<int*>(h->d_tmp);
+  int* d_c = d_r + m;
+  double* d_o = reinterpret_cast<double*>(reinterpret_cast<char*>(h->d_tmp) + ib);
+  RSFG_HIP(hipMemcpyAsync(d_r, rr.data(), sizeof(int) * m, hipMemcpyHostToDevice, h->stream));
+  RSFG_HIP(hipMemcpyAsync(d_c, cc.data(), sizeof(int) * k, hipMemcpyHostToDevice, h->stream));
+  const long tot = (long)m * k;
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream,
+                     h->K64, (long)h->ld, m, d_r, k, d_c, d_o);
+  RSFG_HIP(hipGetLastError());
+  RSFG_HIP(hipMemcpyAsync(out, d_o, sizeof(double) * tot, hipMemcpyDeviceToHost, h->stream));
+  RSFG_HIP(hipStreamSynchronize(h->stream));
+  return RSFG_OK;
+}
+
+int rsfg_traction_pairs(rsfg_handle* h, long np, const int* recv, const int* src, int full,
+                        double* out) {
+  RSFG_CHECK(h && np >= 0 && recv && src && out, RSFG_EINVAL, "rsfg_traction_pairs: bad args");
+  if (np == 0) return RSFG_OK;
+  for (long p = 0; p < np; ++p)
+    RSFG_CHECK(recv[p] >= 0 && recv[p] < h->n && src[p] >= 0 && src[p] < h->n, RSFG_EINVAL,
+               "rsfg_traction_pairs: index out of range");
+  RSFG_HIP(hipSetDevice(h->device));
+  const size_t ib = ((sizeof(int) * 2 * np + 7) / 8) * 8;
+  int rc = ensure_tmp(h, ib + sizeof(double) * 6 * np);
+  if (rc) return rc;
+  int* d_r = reinterpret_cast<int*>(h->d_tmp);
+  int* d_s = d_r + np;
+  double* d_o = reinterpret_cast<double*>(reinterpret_cast<char*>(h->d_tmp) + ib);
+  RSFG_HIP(hipMemcpyAsync(d_r, recv, sizeof(int) * np, hipMemcpyHostToDevice, h->stream));
+  RSFG_HIP(hipMemcpyAsync(d_s, src, sizeof(int) * np, hipMemcpyHostToDevice, h->stream));
+  rc = launch_pairs(h, np, d_r, d_s, full, d_o);
+  if (rc) return rc;
+  RSFG_HIP(hipMemcpyAsync(out, d_o, sizeof(double) * 6 * np, hipMemcpyDeviceToHost, h->stream));
+  RSFG_HIP(hipStreamSynchronize(h->stream));
+  return RSFG_OK;
+}
+
+int rsfg_matvec_dev(rsfg_handle* h, int prec, const void* x_dev, void* y_dev) {
+  RSFG_CHECK(h && x_dev && y_dev, RSFG_EINVAL, "rsfg_matvec_dev: null argument");
+  RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_matvec_dev: precision not assembled");
+  RSFG_HIP(hipSetDevice(h->device));
+  // input staging needs the padded buffer: use the scratch
+  const size_t need = h->vin_elems ? h->vin_elems : vin_elems_for(h->cols);
+  const size_t ybytes = sizeof(double) * h->rows_all;
+  int rc = ensure_tmp(h, need * sizeof(double) + ybytes);
+  if (rc) return rc;
+  void* xin = h->d_tmp;
+  double* yall = reinterpret_cast<double*>(reinterpret_cast<char*>(h->d_tmp) + need * sizeof(double));
+  const size_t saved = h->vin_elems;
+  h->vin_elems = need;
+  rc = stage_input(h, prec, static_cast<const double*>(x_dev), xin);
+  h->vin_elems = saved;
+  if (rc) return rc;
+  rc = matvec_any(h, prec, xin, yall);
+  if (rc) return rc;
+  if (h->G == h->ncomp) {
+    RSFG_HIP(hipMemcpyAsync(y_dev, yall, ybytes, hipMemcpyDeviceToDevice, h->stream));
+  } else {
+    const int nr = h->n * h->ncomp;  // drop the normal rows
+    hipLaunchKernelGGL(k_pick_shear, dim3((nr + 255) / 256), dim3(256), 0, h->stream, yall, h->n,
+                       h->ncomp, h->G, static_cast<double*>(y_dev));
+    RSFG_HIP(hipGetLastError());
+  }
+  return RSFG_OK;
+}
+
+int rsfg_matvec(rsfg_handle* h, int prec, const double* x, double* y) {
+  RSFG_CHECK(h && x && y, RSFG_EINVAL, "rsfg_matvec: null argument");
+  RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_matvec: precision not assembled");
+  RSFG_HIP(hipSetDevice(h->device));
+  double *dx = nullptr, *dy = nullptr;
+  const size_t nr = (size_t)h->ncomp * h->n;
+  RSFG_HIP(hipMalloc(&dx, sizeof(double) * h->cols));
+  RSFG_HIP(hipMalloc(&dy, sizeof(double) * nr));
+  int rc = RSFG_OK;
+  hipError_t e = hipMemcpyAsync(dx, x, sizeof(double) * h->cols, hipMemcpyHostToDevice, h->stream);
+  if (e != hipSuccess) rc = hip_fail(e, "matvec upload", __FILE__, __LINE__);
+  if (!rc) rc = rsfg_matvec_dev(h, prec, dx, dy);
+  if (!rc) {
+    e = hipMemcpyAsync(y, dy, sizeof(double) * nr, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "matvec download", __FILE__, __LINE__);
+  }
+  hipFree(dx);
+  hipFree(dy);
+  return rc;
+}
+
+int rsfg_matvec_t(rsfg_handle* h, int prec, const double* x, double* y) {
+  RSFG_CHECK(h && x && y, RSFG_EINVAL, "rsfg_matvec_t: null argument");
+  RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_matvec_t: precision not assembled");
+  RSFG_CHECK(h->G == h->ncomp, RSFG_EINVAL, "rsfg_matvec_t: operator holds normal rows");
+  RSFG_HIP(hipSetDevice(h->device));
+  const int rows = h->rows_all, cols = h->cols;
+  const int nsplit = std::max(1, std::min(64, rows / 64));
+  double *dx = nullptr, *dy = nullptr, *dp = nullptr;
+  RSFG_HIP(hipMalloc(&dx, sizeof(double) * rows));
+  RSFG_HIP(hipMalloc(&dy, sizeof(double) * cols));
+  RSFG_HIP(hipMalloc(&dp, sizeof(double) * cols * (size_t)nsplit));
+  int rc = RSFG_OK;
+  hipError_t e = hipMemcpyAsync(dx, x, sizeof(double) * rows, hipMemcpyHostToDevice, h->stream);
+  if (e != hipSuccess) rc = hip_fail(e, "matvec_t upload", __FILE__, __LINE__);
+  if (!rc) {
+    switch (prec) {
+      case RSFG_PREC_FP64: rc = matvec_t_launch<double>(h, h->K64, dx, dy, dp, nsplit); break;
+      case RSFG_PREC_FP32: rc = matvec_t_launch<float>(h, h->K32, dx, dy, dp, nsplit); break;
+      case RSFG_PREC_BF16X2: rc = matvec_t_launch<bf16x2_t>(h, h->Kbx2, dx, dy, dp, nsplit); break;
+      default: rc = matvec_t_launch<bf16_t>(h, h->Kb16, dx, dy, dp, nsplit); break;
+    }
+  }
+  if (!rc) {
+    e = hipMemcpyAsync(y, dy, sizeof(double) * cols, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) rc = hip_fail(e, "matvec_t download", __FILE__, __LINE__);
+  }
+  hipFree(dx);
+  hipFree(dy);
+  hipFree(dp);
+  return rc;
+}
+
+int rsfg_matvec_bench(rsfg_handle* h, int prec, int nrep, double* ms) {
+  RSFG_CHECK(h && nrep > 0 && ms, RSFG_EINVAL, "rsfg_matvec_bench: bad args");
+  RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_matvec_bench: precision not assembled");
+  RSFG_HIP(hipSetDevice(h->device));
+  const size_t need = vin_elems_for(h->cols);
+  int rc = ensure_tmp(h, need * sizeof(double) + sizeof(double) * h->rows_all);
+  if (rc) return rc;
+  void* xin = h->d_tmp;
+  double* y = reinterpret_cast<double*>(reinterpret_cast<char*>(h->d_tmp) + need * sizeof(double));
+  RSFG_HIP(hipMemsetAsync(xin, 0, need * sizeof(double), h->stream));
+  rc = matvec_any(h, prec, xin, y);  // warm
+  if (rc) return rc;
+  RSFG_HIP(hipEventRecord(h->ev0, h->stream));
+  for (int i = 0; i < nrep; ++i) {
+    rc = matvec_any(h, prec, xin, y);
+    if (rc) return rc;
+  }
+  RSFG_HIP(hipEventRecord(h->ev1, h->stream));
+  RSFG_HIP(hipEventSynchronize(h->ev1));
+  float t = 0.f;
+  RSFG_HIP(hipEventElapsedTime(&t, h->ev0, h->ev1));
+  *ms = t / nrep;
+  return RSFG_OK;
+}
+
+int rsfg_set_friction(rsfg_handle* h, const rsfg_friction* f, const double* a, const double* b,
+                      const double* dc) {
+  RSFG_CHECK(h && f && a && b && dc, RSFG_EINVAL, "rsfg_set_friction: null argument");
+  RSFG_CHECK(h->assembled, RSFG_ESTATE, "rsfg_set_friction: assemble first");
+  RSFG_CHECK(f->law >= 1 && f->law <= 5, RSFG_EINVAL, "rsfg_set_friction: law must be 1..5");
+  RSFG_CHECK(!f->calc_sigma_dot || h->with_normal, RSFG_EINVAL,
+             "rsfg_set_friction: calc_sigma_dot needs the normal rows (with_normal)");
+  for (int i = 0; i < h->n; ++i)
+    RSFG_CHECK(a[i] > 0 && dc[i] > 0, RSFG_EINVAL, "rsfg_set_friction: a and D_c must be > 0");
+  RSFG_HIP(hipSetDevice(h->device));
+  FricParams& p = h->fp;
+  p.v0 = f->v0;
+  p.f0 = f->f0;
+  p.eta = f->eta;
+  p.vmin_state = f->vmin_state > 0 ? f->vmin_state : 1e-16;
+  p.sato_beta = f->sato_beta > 0 ? f->sato_beta : 1e-2;
+  p.kt_vc = f->kt_vc > 0 ? f->kt_vc : 1e-2 * f->v0;
+  p.min_sigma = f->min_sigma;
+  p.max_sigma = f->max_sigma;
+  p.law = f->law;
+  p.calc_sigma_dot = f->calc_sigma_dot;
+  p.limit_sigma = f->limit_sigma;
+  p.ncomp = h->ncomp;
+  const int n = h->n;
+  std::vector<double> c(3 * (size_t)n);
+  memcpy(c.data(), a, sizeof(double) * n);
+  memcpy(c.data() + n, b, sizeof(double) * n);
+  memcpy(c.data() + 2 * n, dc, sizeof(double) * n);
+  if (!h->d_coef) RSFG_HIP(hipMalloc(&h->d_coef, sizeof(double) * 3 * n));
+  RSFG_HIP(hipMemcpy(h->d_coef, c.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  if (!h->d_sinc) RSFG_HIP(hipMalloc(&h->d_sinc, sizeof(double) * 3 * n));
+  h->have_friction = true;
+  return alloc_state(h);
+}
+
+int rsfg_set_loading(rsfg_handle* h, double vpl, const double* vpl_patch, const double* taudot_add,
+                     const double* sigdot_add) {
+  RSFG_CHECK(h && h->have_friction, RSFG_ESTATE, "rsfg_set_loading: set friction first");
+  RSFG_HIP(hipSetDevice(h->device));
+  const int n = h->n, nc = h->ncomp;
+  // backslip: slip rate -vpl along the slip direction (strike component in
+  // the two-component mode), src/rsf_solve.c:268-331
+  std::vector<double> x((size_t)h->cols, 0.0), y((size_t)h->rows_all, 0.0);
+  for (int p = 0; p < n; ++p) x[(size_t)nc * p] = -(vpl_patch ? vpl_patch[p] : vpl);
+  double *dx = nullptr, *dy = nullptr;
+  RSFG_HIP(hipMalloc(&dx, sizeof(double) * h->cols));
+  RSFG_HIP(hipMalloc(&dy, sizeof(double) * h->rows_all));
+  RSFG_HIP(hipMemcpy(dx, x.data(), sizeof(double) * h->cols, hipMemcpyHostToDevice));
+  const size_t need = vin_elems_for(h->cols);
+  int rc = ensure_tmp(h, need * sizeof(double));
+  if (!rc) rc = stage_input(h, RSFG_PREC_FP64, dx, h->d_tmp);
+  if (!rc) rc = matvec_any(h, RSFG_PREC_FP64, h->d_tmp, dy);
+  hipError_t e = hipSuccess;
+  if (!rc) e = hipMemcpyAsync(y.data(), dy, sizeof(double) * h->rows_all, hipMemcpyDeviceToHost,
+                              h->stream);
+  if (!rc && e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  hipFree(dx);
+  hipFree(dy);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "rsfg_set_loading", __FILE__, __LINE__);
+  std::vector<double> s(3 * (size_t)n, 0.0);
+  for (int p = 0; p < n; ++p) {
+    for (int k = 0; k < nc; ++k) s[(size_t)nc * p + k] = y[(size_t)p * h->G + k];
+    if (taudot_add) s[(size_t)nc * p] += taudot_add[p];
+    const double sn = (h->G > nc) ? y[(size_t)p * h->G + nc] : 0.0;
+    s[(size_t)nc * n + p] = sn + ((sigdot_add && h->fp.calc_sigma_dot) ? sigdot_add[p] : 0.0);
+  }
+  RSFG_HIP(hipMemcpy(h->d_sinc, s.data(), sizeof(double) * (nc + 1) * (size_t)n,
+                     hipMemcpyHostToDevice));
+  h->have_loading = true;
+  return RSFG_OK;
+}
+
+int rsfg_set_state(rsfg_handle* h, double t, const double* X) {
+  RSFG_CHECK(h && X && h->have_friction, RSFG_ESTATE, "rsfg_set_state: set friction first");
+  RSFG_HIP(hipSetDevice(h->device));
+  const int n = h->n, D = h->dim;
+  std::vector<double> soa((size_t)D * n);
+  for (int p = 0; p < n; ++p)
+    for (int f = 0; f < D; ++f) soa[(size_t)f * n + p] = X[(size_t)p * D + f];
+  h->xcur = 0;
+  RSFG_HIP(hipMemcpy(h->Xb[0], soa.data(), sizeof(double) * D * n, hipMemcpyHostToDevice));
+  h->t = t;
+  h->have_state = true;
+  return RSFG_OK;
+}
+
+int rsfg_get_state(rsfg_handle* h, double* t, double* X) {
+  RSFG_CHECK(h && X && h->have_state, RSFG_ESTATE, "rsfg_get_state: no state");
+  RSFG_HIP(hipSetDevice(h->device));
+  const int n = h->n, D = h->dim;
+  std::vector<double> soa((size_t)D * n);
+  RSFG_HIP(hipMemcpy(soa.data(), h->Xb[h->xcur], sizeof(double) * D * n, hipMemcpyDeviceToHost));
+  for (int p = 0; p < n; ++p)
+    for (int f = 0; f < D; ++f) X[(size_t)p * D + f] = soa[(size_t)f * n + p];
+  if (t) *t = h->t;
+  return RSFG_OK;
+}
+
+int rsfg_rhs(rsfg_handle* h, int prec, const double* X, double* F, int* domain_ok) {
+  RSFG_CHECK(h && X && F, RSFG_EINVAL, "rsfg_rhs: null argument");
+  RSFG_CHECK(h->have_friction && h->have_loading, RSFG_ESTATE,
+             "rsfg_rhs: set friction and loading first");
+  RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_rhs: precision not assembled");
+  RSFG_HIP(hipSetDevice(h->device));
+  const int n = h->n, D = h->dim, nc = h->ncomp;
+  std::vector<double> soa((size_t)D * n);
+  for (int p = 0; p < n; ++p)
+    for (int f = 0; f < D; ++f) soa[(size_t)f * n + p] = X[(size_t)p * D + f];
+  double* Y = h->Yb[0];
+  double* Fd = h->Yb[1];
+  int rc = ensure_tmp(h, sizeof(int) * n);
+  if (rc) return rc;
+  int* okd = reinterpret_cast<int*>(h->d_tmp);
+  RSFG_HIP(hipMemcpyAsync(Y, soa.data(), sizeof(double) * D * n, hipMemcpyHostToDevice, h->stream));
+  RSFG_HIP(hipMemsetAsync(h->vin[0], 0, sizeof(double) * h->vin_elems, h->stream));
+  const dim3 gv((n + 255) / 256);
+  const bool f64 = (prec == RSFG_PREC_FP64);
+#define RSFG_VEL(VT)                                                                          \
+  if (nc == 1)                                                                                \
+    hipLaunchKernelGGL((k_vel<VT, 1>), gv, dim3(256), 0, h->stream, Y, h->d_coef, h->fp, n,  \
+                       reinterpret_cast<VT*>(h->vin[0]), okd);                                \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_vel<VT, 2>), gv, dim3(256), 0, h->stream, Y, h->d_coef, h->fp, n,  \
+                       reinterpret_cast<VT*>(h->vin[0]), okd);
+  if (f64) {
+    RSFG_VEL(double)
+  } else {
+    RSFG_VEL(float)
+  }
+#undef RSFG_VEL
+  const int G = h->G;
+  const long ld = h->ld;
+  const int rows = h->rows_all, cols = h->cols;
+#define RSFG_RHS(TK, NC, GG)                                                                  \
+  {                                                                                           \
+    constexpr int PPW = (GG == 3) ? 2 : (GG == 1 ? 8 : 4);                                    \
+    hipLaunchKernelGGL((k_rhs<TK, NC, GG>), dim3((n + PPW - 1) / PPW), dim3(256), 0,          \
+                       h->stream, static_cast<const TK*>(operator_ptr(h, prec)), ld, rows,    \
+                       cols, reinterpret_cast<const typename KTraits<TK>::VT*>(h->vin[0]), Y, \
+                       h->d_coef, h->d_sinc, h->fp, n, Fd);                                   \
+  }
+#define RSFG_RHS_T(TK)                              \
+  if (nc == 1 && G == 1) RSFG_RHS(TK, 1, 1)         \
+  else if (nc == 1) RSFG_RHS(TK, 1, 2)              \
+  else if (G == 2) RSFG_RHS(TK, 2, 2)               \
+  else RSFG_RHS(TK, 2, 3)
+  switch (prec) {
+    case RSFG_PREC_FP64: RSFG_RHS_T(double) break;
+    case RSFG_PREC_FP32: RSFG_RHS_T(float) break;
+    case RSFG_PREC_BF16X2: RSFG_RHS_T(bf16x2_t) break;
+    default: RSFG_RHS_T(bf16_t) break;
+  }
+#undef RSFG_RHS_T
+#undef RSFG_RHS
+  RSFG_HIP(hipGetLastError());
+  std::vector<int> okh(n);
+  RSFG_HIP(hipMemcpyAsync(soa.data(), Fd, sizeof(double) * D * n, hipMemcpyDeviceToHost, h->stream));
+  RSFG_HIP(hipMemcpyAsync(okh.data(), okd, sizeof(int) * n, hipMemcpyDeviceToHost, h->stream));
+  RSFG_HIP(hipStreamSynchronize(h->stream));
+  for (int p = 0; p < n; ++p)
+    for (int f = 0; f < D; ++f) F[(size_t)p * D + f] = soa[(size_t)f * n + p];
+  if (domain_ok) {
+    int ok = 1;
+    for (int p = 0; p < n; ++p) ok &= okh[p];
+    *domain_ok = ok;
+  }
+  return RSFG_OK;
+}
+
+int rsfg_solver_defaults(rsfg_solver_opts* o) {
+  RSFG_CHECK(o, RSFG_EINVAL, "rsfg_solver_defaults: null");
+  memset(o, 0, sizeof(*o));
+  o->method = RSFG_RK5DP;
+  o->prec = RSFG_PREC_FP64;
+  o->rtol = 1e-4;
+  o->atol[0] = 1e-12;  // psi (RSF_AERROR_PSI)
+  for (int f = 1; f < 6; ++f) o->atol[f] = 1e-3;  // tau, sigma [Pa]; slip [m]
+  o->dt_init = 1.0;
+  o->dt_max = 1e10;
+  o->dt_monitor = 5.0 * 31557600.0;
+  o->rdx_monitor = 1e-4;
+  o->adx_monitor = 0.0;
+  o->vel_event = 1e-3;
+  o->vel_event_hyst = 0.5;
+  o->rupture_vth = 1e-3;
+  o->max_steps = 1000000000L;
+  o->max_domain_reject = 5000;
+  o->steps_per_launch = 32;
+  return RSFG_OK;
+}
+
+int rsfg_solve(rsfg_handle* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st) {
+  RSFG_CHECK(h && o, RSFG_EINVAL, "rsfg_solve: null argument");
+  RSFG_CHECK(h->have_state && h->have_loading, RSFG_ESTATE,
+             "rsfg_solve: set friction, loading and state first");
+  RSFG_HIP(hipSetDevice(h->device));
+  if (h->xcur != 0) {  // the solver starts from buffer 0
+    RSFG_HIP(hipMemcpyAsync(h->Xb[0], h->Xb[h->xcur], sizeof(double) * h->dim * h->n,
+                            hipMemcpyDeviceToDevice, h->stream));
+    h->xcur = 0;
+  }
+  return solve_device(h, o, t_end, st);
+}
+
+int rsfg_events(rsfg_handle* h, double* rows, int* n) {
+  RSFG_CHECK(h && rows && n, RSFG_EINVAL, "rsfg_events: null argument");
+  const int k = std::min(*n, h->nevent);
+  if (k > 0)
+    RSFG_HIP(hipMemcpy(rows, h->d_ev, sizeof(double) * 8 * k, hipMemcpyDeviceToHost));
+  *n = k;
+  return RSFG_OK;
+}
+
+int rsfg_monitor(rsfg_handle* h, double* rows, int* n) {
+  RSFG_CHECK(h && rows && n, RSFG_EINVAL, "rsfg_monitor: null argument");
+  const int k = std::min(*n, h->nmon);
+  if (k > 0)
+    RSFG_HIP(hipMemcpy(rows, h->d_mon, sizeof(double) * 10 * k, hipMemcpyDeviceToHost));
+  *n = k;
+  return RSFG_OK;
+}
+
+}  // extern "C"

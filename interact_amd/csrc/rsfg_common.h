@@ -1,0 +1,152 @@
+// Internal types shared by the librsfg translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/rsfg.h"
+#include "rsf_dev.h"
+
+namespace rsfg {
+
+// ----------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define RSFG_HIP(call)                                                   \
+  do {                                                                   \
+    hipError_t _e = (call);                                              \
+    if (_e != hipSuccess) return ::rsfg::hip_fail(_e, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define RSFG_CHECK(cond, code, msg) \
+  do {                              \
+    if (!(cond)) {                  \
+      ::rsfg::set_error(msg);       \
+      return (code);                \
+    }                               \
+  } while (0)
+
+// storage element of the bf16 hi+lo pair: hi in the upper 16 bits
+typedef uint32_t bf16x2_t;
+typedef uint16_t bf16_t;
+
+// capacity of the device event and monitor logs (rows)
+constexpr int kMaxEv = 4096;
+constexpr int kMaxMon = 1 << 18;
+// maximum number of Runge-Kutta stages (rk5dp)
+constexpr int kMaxStages = 7;
+// reductions per workgroup in the final stage
+constexpr int kNRed = 9;
+enum { RED_ERR = 0, RED_VMAX, RED_VSUM, RED_SLIP, RED_MU, RED_SMAX, RED_SMIN, RED_XN2, RED_DX2 };
+
+// Controller state, resident in device memory and updated by the last
+// workgroup of every final-stage kernel.
+struct DevCtl {
+  double t, h, t_end, hmax;
+  double t_mon;
+  double fprev;         // event function at the accepted state
+  double peakv;         // running peak |v| since the last onset
+  double last_vmax;
+  long long steps, rejects, domain_rejects, rhs_evals, max_steps;
+  int xcur;             // accepted-state buffer (0/1)
+  int kslot[8];         // logical stage -> physical derivative slot
+  int attempt_failed;   // previous attempt of this step was rejected
+  int stage_fail;       // a stage domain check failed in this attempt
+  int halted;
+  int status;
+  int slipping;
+  int mon_copy;         // next prep copies X into Xmon
+  int rupt_update;      // next prep marks ruptured cells
+  int storm;
+  unsigned int counter; // last-workgroup ticket
+  int nevent, nmon;
+  int max_event, max_mon;
+};
+
+// Butcher tableau and integrator settings (kernel argument, by value)
+struct Tableau {
+  double A[kMaxStages][kMaxStages];
+  double b[kMaxStages], bh[kMaxStages];
+  int s, order;
+};
+
+struct SolveOpts {
+  double rtol, atol[6];
+  double dt_monitor, rdx, adx;
+  double vel_event, hyst, rupture_vth;
+  int max_domain_reject;
+};
+
+}  // namespace rsfg
+
+// the opaque handle of the C ABI
+struct rsfg_ctx {
+  int device = 0;
+  int n = 0;                  // patches
+  hipStream_t stream = nullptr;
+  // geometry (device, SoA)
+  double* d_geo = nullptr;    // kGeoFields x n, see kGeo*
+  double* h_geo = nullptr;    // host copy
+  // operator
+  bool assembled = false;
+  int slip_mode = 0, ncomp = 1, with_normal = 0, full_space = 0, store = 0;
+  int G = 1;                  // rows per patch (ncomp shear + normal)
+  int rows_all = 0;           // G * n
+  int cols = 0;               // ncomp * n
+  int ld = 0;                 // leading dimension (elements)
+  double scale = 1.0;
+  double* K64 = nullptr;
+  float* K32 = nullptr;
+  rsfg::bf16x2_t* Kbx2 = nullptr;
+  rsfg::bf16_t* Kb16 = nullptr;
+  size_t kslack = 0;          // extra elements allocated after the last row
+  // friction and loading
+  bool have_friction = false, have_loading = false;
+  rsfg::FricParams fp{};
+  double* d_coef = nullptr;   // a, b, dc (3 n)
+  double* d_sinc = nullptr;   // ncomp n shear loading + n normal loading
+  // solver state
+  int dim = 4;
+  int xcur = 0;               // Xb buffer holding the accepted state
+  double t = 0.0;
+  bool have_state = false;
+  double* d_state = nullptr;  // one block holding all solver vectors
+  double *Xb[2] = {nullptr, nullptr}, *Yb[2] = {nullptr, nullptr};
+  double* Stg[8] = {nullptr};
+  double* Xmon = nullptr;
+  double* vin[2] = {nullptr, nullptr};  // D padded, used as double or float
+  size_t vin_elems = 0;
+  double* partials = nullptr;
+  size_t partial_count = 0;
+  double* d_ev = nullptr;
+  double* d_mon = nullptr;
+  double* d_cat = nullptr;    // tau0, slip0 per patch
+  int* d_rupt = nullptr;
+  rsfg::DevCtl* d_ctl = nullptr;
+  rsfg::DevCtl* h_ctl = nullptr;  // pinned
+  int nevent = 0, nmon = 0;
+  // scratch for API calls
+  double* d_tmp = nullptr;
+  size_t tmp_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace rsfg {
+// geometry, structure of arrays: field f of patch p at d_geo[f * n + p]
+enum {
+  kGeoX = 0, kGeoY, kGeoZ,      // centroid
+  kGeoCa, kGeoSa,               // cos, sin of alpha = 90 - strike
+  kGeoSd, kGeoCd,               // sin, cos of dip after the 1e-8 snap of DC3D
+  kGeoL, kGeoW,                 // half length, half width
+  kGeoN0, kGeoN1, kGeoN2,       // normal
+  kGeoS0, kGeoS1, kGeoS2,       // strike vector
+  kGeoD0, kGeoD1, kGeoD2,       // dip vector
+  kGeoR0, kGeoR1,               // rake fractions
+  kGeoFields
+};
+
+int ensure_tmp(rsfg_ctx* h, size_t bytes);
+int alloc_state(rsfg_ctx* h);
+}  // namespace rsfg

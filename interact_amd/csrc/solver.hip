@@ -1,0 +1,819 @@
+// Device-resident adaptive Runge-Kutta cycle integration.
+//
+// One attempted step of an s-stage FSAL pair (rk5dp: s = 7, rk3bs: s = 4)
+// is s launches:
+//   k_prep            Y_1 = X + h a10 K_0, v(Y_1), stage domain check
+//   k_stage(i), i=1..s-1
+//                     y = K v(Y_i) streamed from HBM; epilogue per patch:
+//                     K_i = F(Y_i, y) (rsf_ODE_RHSFunction), then either
+//                     Y_{i+1} and v(Y_{i+1}) for the next launch, or (last
+//                     stage) the weighted error and the reductions of the
+//                     new state.  The last workgroup to finish the final
+//                     stage runs the controller below.
+// The controller restates the PETSc behaviour the reference relies on
+// (src/rsf_solve.c:440-546): TSAdapt basic (safety 0.9, x0.5 after a
+// failed attempt, exponent -1/order, clip [0.5, 2], h <= min(dt_max,
+// dt_monitor)), stage domain failure -> h * 0.25 (rsf_domain_check,
+// src/rsf_engine.c:309-362), threshold events on log10 max|v| with
+// hysteresis (src/rsf_solve.c:677-703, src/rsf_output.c:1000-1150)
+// located on the step's dense output, and the change-triggered monitor
+// (src/rsf_output.c:718-864).  The host only launches captured graphs of
+// step attempts and polls the halted flag.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gemv_dev.h"
+#include "rsf_dev.h"
+#include "rsfg_common.h"
+
+namespace rsfg {
+
+constexpr double kSecPerYear = 31557600.0;
+
+struct StageArgs {
+  const void* K;
+  long ld;
+  int rows_all, cols, n, G;
+  const double* coef;  // a, b, dc
+  const double* sinc;  // ncomp n shear rows, then n normal
+  FricParams fp;
+  double* Xb[2];
+  double* Yb[2];
+  double* Stg[8];
+  double* Xmon;
+  const void* vin;
+  void* vout;
+  double* partials;
+  double* ev;
+  double* mon;
+  double* tau0;
+  double* slip0;
+  int* rupt;
+  DevCtl* ctl;
+  Tableau tb;
+  SolveOpts so;
+  int s;       // stage index; 0 = evaluation of K_0 at the initial state
+};
+
+template <int NC>
+struct Dim {
+  static constexpr int v = 2 * NC + 2;
+};
+
+__device__ __forceinline__ PatchCoef load_coef(const double* coef, int n, int p) {
+  PatchCoef c;
+  c.a = coef[p];
+  c.b = coef[n + p];
+  c.dc = coef[2 * n + p];
+  return c;
+}
+
+template <int NC>
+__device__ __forceinline__ double vel_mag(const FricParams& fp, double a, const double* s) {
+  double v[2];
+  rsf_patch_vel<NC>(fp, a, s, v);
+  return (NC == 1) ? fabs(v[0]) : sqrt(v[0] * v[0] + v[1] * v[1]);
+}
+
+template <typename VT, int NC>
+__device__ __forceinline__ void write_vel(const FricParams& fp, double a,
+                                          const double* s, VT* vout, int p) {
+  double v[2];
+  rsf_patch_vel<NC>(fp, a, s, v);
+#pragma unroll
+  for (int c = 0; c < NC; ++c) vout[NC * p + c] = static_cast<VT>(v[c]);
+}
+
+// ------------------------------------------------------------ block helpers
+__device__ __forceinline__ double block_reduce(double v, int op, double* sh) {
+  // op 0 sum, 1 max, 2 min; 256 threads; deterministic order
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off, 64);
+    v = (op == 0) ? v + o : (op == 1 ? fmax(v, o) : fmin(v, o));
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r;
+  if (op == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  else if (op == 1) r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+  else r = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double evfun(double vmax, double thr) {
+  return log10(vmax > 1e-300 ? vmax : 1e-300) - log10(thr);
+}
+
+// dense output of patch p at fraction th of the step [t, t + hs]
+template <int NC>
+__device__ void dense_patch(const StageArgs& a, const double* X0, const double* X1,
+                            const double* const* Kp, double hs, double th, int p,
+                            double* out) {
+  constexpr int D = Dim<NC>::v;
+  const int n = a.n;
+  if (a.tb.s == 7) {
+    const double d1 = -12715105075.0 / 11282082432.0, d3 = 87487479700.0 / 32700410799.0,
+                 d4 = -10690763975.0 / 1880347072.0, d5 = 701980252875.0 / 199316789632.0,
+                 d6 = -1453857185.0 / 822651844.0, d7 = 69997945.0 / 29380423.0;
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+      const int i = f * n + p;
+      const double r2 = X1[i] - X0[i];
+      const double r3 = hs * Kp[0][i] - r2;
+      const double r4 = r2 - hs * Kp[6][i] - r3;
+      const double r5 = hs * (d1 * Kp[0][i] + d3 * Kp[2][i] + d4 * Kp[3][i] +
+                              d5 * Kp[4][i] + d6 * Kp[5][i] + d7 * Kp[6][i]);
+      out[f] = X0[i] + th * (r2 + (1.0 - th) * (r3 + th * (r4 + (1.0 - th) * r5)));
+    }
+  } else {
+    const double t2 = th * th, t3 = t2 * th;
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+      const int i = f * n + p;
+      out[f] = (2 * t3 - 3 * t2 + 1) * X0[i] + (t3 - 2 * t2 + th) * hs * Kp[0][i] +
+               (-2 * t3 + 3 * t2) * X1[i] + (t3 - t2) * hs * Kp[3][i];
+    }
+  }
+}
+
+// block-wide max |v| at dense-output fraction th
+template <int NC>
+__device__ double dense_vmax(const StageArgs& a, const double* X0, const double* X1,
+                             const double* const* Kp, double hs, double th, double* sh) {
+  constexpr int D = Dim<NC>::v;
+  double m = 0.0, y[D];
+  for (int p = threadIdx.x; p < a.n; p += blockDim.x) {
+    dense_patch<NC>(a, X0, X1, Kp, hs, th, p, y);
+    m = fmax(m, vel_mag<NC>(a.fp, a.coef[p], y));
+  }
+  return block_reduce(m, 1, sh);
+}
+
+// ------------------------------------------------------------ controller
+// Runs in the last workgroup of the final stage (or of the initial K_0
+// evaluation when a.s == 0).  red[] holds the grid reductions.
+template <int NC>
+__device__ void controller(const StageArgs& a, const double* red) {
+  constexpr int D = Dim<NC>::v;
+  __shared__ double sh[8];
+  __shared__ double s_par[8];
+  __shared__ int s_int[4];
+  DevCtl* c = a.ctl;
+  const int n = a.n, S = a.tb.s, tid = threadIdx.x;
+  const SolveOpts& o = a.so;
+  if (a.s == 0) {
+    if (tid == 0) {
+      if (c->stage_fail) {
+        c->status = RSFG_EDOMAIN;
+        c->halted = 1;
+      }
+      c->rhs_evals += 1;
+      c->slipping = red[RED_VMAX] > o.vel_event;
+      const double thr = o.vel_event * (c->slipping ? o.hyst : 1.0);
+      c->fprev = evfun(red[RED_VMAX], thr);
+      c->peakv = 0.0;
+      c->last_vmax = red[RED_VMAX];
+      // first monitor row (old time forced to t0 - 2 dt_monitor)
+      if (c->nmon < c->max_mon) {
+        double* row = a.mon + 10 * c->nmon;
+        row[0] = (double)c->steps; row[1] = c->t; row[2] = c->t / kSecPerYear;
+        row[3] = c->h; row[4] = log10(red[RED_VMAX]); row[5] = red[RED_SLIP] / n;
+        row[6] = red[RED_MU] / n; row[7] = red[RED_SMAX]; row[8] = red[RED_SMIN];
+        const double mv = red[RED_VSUM] / n;
+        row[9] = log10(mv > 1e-300 ? mv : 1e-300);
+        c->nmon++;
+      }
+      c->t_mon = c->t;
+      c->mon_copy = 1;
+      c->rupt_update = 0;
+      c->stage_fail = 0;
+      c->attempt_failed = 0;
+      if (!(c->t < c->t_end)) c->halted = 1;
+      c->counter = 0;
+    }
+    return;
+  }
+  // ---- accept / reject (thread 0), PETSc TSAdapt basic
+  if (tid == 0) {
+    c->rhs_evals += S - 1;
+    int acc;
+    double hnew;
+    if (c->stage_fail) {
+      c->domain_rejects++;
+      c->rejects++;
+      hnew = c->h * 0.25;
+      c->attempt_failed = 1;
+      acc = 0;
+    } else {
+      double en = red[RED_ERR];
+      if (!(en == en) || isinf(en)) en = 1e300;
+      double safety = 0.9;
+      acc = 1;
+      if (en > 1.0) {
+        if (c->attempt_failed) safety *= 0.5;
+        acc = 0;
+      }
+      double hfac = (en > 0.0) ? safety * pow(en, -1.0 / a.tb.order) : 2.0;
+      hfac = fmin(fmax(hfac, 0.5), 2.0);
+      hnew = fmin(c->h * hfac, c->hmax);
+      if (!acc) {
+        c->rejects++;
+        c->attempt_failed = 1;
+      }
+    }
+    s_int[0] = acc;
+    s_par[0] = hnew;
+    c->mon_copy = 0;
+    c->rupt_update = 0;
+    c->stage_fail = 0;
+    if (!acc) c->h = hnew;
+  }
+  __syncthreads();
+  if (!s_int[0]) {
+    if (tid == 0) c->counter = 0;
+    return;
+  }
+  // ---- accepted: events on the dense output of [t, t + h]
+  const int xo = c->xcur;
+  const double* X0 = a.Xb[xo];
+  const double* X1 = a.Xb[xo ^ 1];
+  const double* Kp[kMaxStages];
+  for (int j = 0; j < S; ++j) Kp[j] = a.Stg[c->kslot[j]];
+  const double hs = c->h;
+  const double vnew = red[RED_VMAX];
+  double th_lo = 0.0;
+  for (int guard = 0; guard < 8; ++guard) {
+    // thread 0 decides whether [th_lo, 1] holds a crossing
+    if (tid == 0) {
+      const double thr = o.vel_event * (c->slipping ? o.hyst : 1.0);
+      const double fnew = evfun(vnew, thr);
+      const double fp = c->fprev;
+      const bool cross = (fp < 0.0 && fnew > 0.0) || (fp > 0.0 && fnew < 0.0) || fnew == 0.0;
+      s_int[1] = cross;
+      s_par[1] = th_lo; s_par[2] = fp; s_par[3] = 1.0; s_par[4] = fnew; s_par[5] = thr;
+      if (!cross) c->fprev = fnew;
+    }
+    __syncthreads();
+    if (!s_int[1]) break;
+    // Illinois regula falsi on the fraction
+    int side = 0;
+    for (int it = 0; it < 200; ++it) {
+      if (tid == 0) {
+        const double lo = s_par[1], flo = s_par[2], hi = s_par[3], fhi = s_par[4];
+        double cth = (fhi - flo != 0.0) ? hi - fhi * (hi - lo) / (fhi - flo) : 0.5 * (lo + hi);
+        if (!(cth > lo && cth < hi)) cth = 0.5 * (lo + hi);
+        s_par[6] = cth;
+      }
+      __syncthreads();
+      const double cth = s_par[6];
+      const double vm = dense_vmax<NC>(a, X0, X1, Kp, hs, cth, sh);
+      if (tid == 0) {
+        const double fc = evfun(vm, s_par[5]);
+        int done = 0;
+        if (fabs(fc) < 1e-10 || (s_par[3] - s_par[1]) * hs < 1e-9) {
+          s_par[1] = s_par[3] = cth;
+          done = 1;
+        } else if ((fc > 0.0) == (s_par[4] > 0.0)) {
+          s_par[3] = cth; s_par[4] = fc;
+          if (side == -1) s_par[2] *= 0.5;
+          side = -1;
+        } else {
+          s_par[1] = cth; s_par[2] = fc;
+          if (side == 1) s_par[4] *= 0.5;
+          side = 1;
+        }
+        s_int[2] = done;
+      }
+      __syncthreads();
+      if (s_int[2]) break;
+    }
+    const double th = s_par[3];
+    // state at the crossing: statistics, onset snapshots, arrest drop
+    const int onset = !c->slipping;  // toggles to slipping
+    double vm = 0.0, ss = 0.0, sm = 0.0, drop = 0.0, cnt = 0.0, y[D];
+    for (int p = tid; p < n; p += blockDim.x) {
+      dense_patch<NC>(a, X0, X1, Kp, hs, th, p, y);
+      vm = fmax(vm, vel_mag<NC>(a.fp, a.coef[p], y));
+      ss += y[NC + 2];
+      sm += y[1] / y[NC + 1];
+      if (onset) {
+        a.tau0[p] = y[1];
+        a.slip0[p] = y[NC + 2];
+        a.rupt[p] = 0;
+      } else if (a.rupt[p]) {
+        drop += fabs(a.tau0[p]) - fabs(y[1]);
+        cnt += 1.0;
+      }
+    }
+    vm = block_reduce(vm, 1, sh);
+    ss = block_reduce(ss, 0, sh);
+    sm = block_reduce(sm, 0, sh);
+    drop = block_reduce(drop, 0, sh);
+    cnt = block_reduce(cnt, 0, sh);
+    if (tid == 0) {
+      c->slipping = onset;
+      if (onset) c->peakv = vm;
+      if (c->nevent < c->max_event) {
+        double* row = a.ev + 8 * c->nevent;
+        row[0] = c->t + th * hs;
+        row[1] = onset ? 1.0 : -1.0;
+        row[2] = log10(vm > 1e-300 ? vm : 1e-300);
+        row[3] = ss / n;
+        row[4] = sm / n;
+        row[5] = onset ? vm : fmax(c->peakv, vm);
+        row[6] = cnt > 0.0 ? drop / cnt : 0.0;
+        row[7] = cnt;
+        c->nevent++;
+      }
+      const double thr = o.vel_event * (c->slipping ? o.hyst : 1.0);
+      c->fprev = evfun(vm, thr);
+    }
+    __syncthreads();
+    th_lo = th;
+    if (th_lo >= 1.0) {
+      if (tid == 0) {
+        const double thr = o.vel_event * (c->slipping ? o.hyst : 1.0);
+        c->fprev = evfun(vnew, thr);
+      }
+      break;
+    }
+  }
+  __syncthreads();
+  // ---- advance
+  if (tid == 0) {
+    c->t += c->h;
+    c->h = s_par[0];
+    c->xcur = xo ^ 1;
+    const int k0 = c->kslot[0];
+    c->kslot[0] = c->kslot[S - 1];
+    c->kslot[S - 1] = k0;
+    c->steps++;
+    c->attempt_failed = 0;
+    c->last_vmax = vnew;
+    if (c->slipping) {
+      c->peakv = fmax(c->peakv, vnew);
+      c->rupt_update = 1;
+    }
+    // monitor (rsf_TS_Monitor): change since the last logged state
+    const double xn = sqrt(red[RED_XN2]), dx = sqrt(red[RED_DX2]);
+    int bail = 0;
+    if (xn > 1e-15 && dx / xn > o.rdx) bail = 1;
+    if (o.adx > 0.0 && dx > o.adx) bail = 1;
+    if (fabs(c->t - c->t_mon) > o.dt_monitor) bail = 1;
+    if (bail) {
+      if (c->nmon < c->max_mon) {
+        double* row = a.mon + 10 * c->nmon;
+        row[0] = (double)c->steps; row[1] = c->t; row[2] = c->t / kSecPerYear;
+        row[3] = c->h; row[4] = log10(vnew); row[5] = red[RED_SLIP] / n;
+        row[6] = red[RED_MU] / n; row[7] = red[RED_SMAX]; row[8] = red[RED_SMIN];
+        const double mv = red[RED_VSUM] / n;
+        row[9] = log10(mv > 1e-300 ? mv : 1e-300);
+        c->nmon++;
+      }
+      c->t_mon = c->t;
+      c->mon_copy = 1;
+    }
+    if (!(c->t < c->t_end)) c->halted = 1;
+    if (c->steps >= c->max_steps) {
+      c->halted = 1;
+      c->status = 2;
+    }
+    c->counter = 0;
+  }
+}
+
+// ------------------------------------------------------------ kernels
+// prep: first stage of an attempt (also Xmon copy and rupture marking)
+template <typename VT, int NC>
+__global__ __launch_bounds__(256) void k_prep(StageArgs a) {
+  constexpr int D = Dim<NC>::v;
+  const DevCtl* c = a.ctl;
+  if (c->halted) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = a.n;
+  if (p >= n) return;
+  const int xc = c->xcur;
+  const double* X = a.Xb[xc];
+  const double* K0 = a.Stg[c->kslot[0]];
+  const double h = c->h;
+  const PatchCoef cf = load_coef(a.coef, n, p);
+  double x[D], y[D];
+#pragma unroll
+  for (int f = 0; f < D; ++f) x[f] = X[f * n + p];
+  if (c->mon_copy) {
+#pragma unroll
+    for (int f = 0; f < D; ++f) a.Xmon[f * n + p] = x[f];
+  }
+  if (c->rupt_update && vel_mag<NC>(a.fp, cf.a, x) >= a.so.rupture_vth) a.rupt[p] = 1;
+  const double w = a.tb.A[1][0];
+  // stage 1 is never the last stage of the supported pairs (s >= 4)
+  double* Y = a.Yb[1];
+#pragma unroll
+  for (int f = 0; f < D; ++f) {
+    y[f] = x[f] + h * (w * K0[f * n + p]);
+    Y[f * n + p] = y[f];
+  }
+  write_vel<VT, NC>(a.fp, cf.a, y, static_cast<VT*>(a.vout), p);
+  if (!rsf_patch_ok<NC>(a.fp, cf, y)) atomicOr(&a.ctl->stage_fail, 1);
+}
+
+// initial velocity field (before K_0 = F(X_0))
+template <typename VT, int NC>
+__global__ __launch_bounds__(256) void k_init_vel(StageArgs a) {
+  constexpr int D = Dim<NC>::v;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = a.n;
+  if (p >= n) return;
+  const double* X = a.Xb[a.ctl->xcur];
+  const PatchCoef cf = load_coef(a.coef, n, p);
+  double x[D];
+#pragma unroll
+  for (int f = 0; f < D; ++f) x[f] = X[f * n + p];
+  write_vel<VT, NC>(a.fp, cf.a, x, static_cast<VT*>(a.vout), p);
+  if (!rsf_patch_ok<NC>(a.fp, cf, x)) atomicOr(&a.ctl->stage_fail, 1);
+}
+
+template <typename TK, int NC, int G, int PPW>
+__global__ __launch_bounds__(256) void k_stage(StageArgs a) {
+  using VT = typename KTraits<TK>::VT;
+  constexpr int D = Dim<NC>::v;
+  constexpr int R = G * PPW;
+  __shared__ double rowsum[R];
+  __shared__ double red[4 * R];
+  __shared__ double pred[PPW][kNRed];
+  __shared__ double gred[kNRed];
+  __shared__ int s_last;
+  DevCtl* c = a.ctl;
+  if (c->halted) return;
+  const int n = a.n, S = a.tb.s, s = a.s;
+  gemv_rows<TK, R>(static_cast<const TK*>(a.K), a.ld, blockIdx.x * R, a.rows_all,
+                   a.cols, static_cast<const VT*>(a.vin), rowsum, red);
+  const int xc = c->xcur;
+  const double* X = a.Xb[xc];
+  double* Xn = a.Xb[xc ^ 1];
+  const bool reduce = (s == 0) || (s == S - 1);
+  const int t = threadIdx.x;
+  if (t < PPW) {
+    const int p = blockIdx.x * PPW + t;
+    double r9[kNRed];
+    r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
+    r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
+    r9[RED_DX2] = 0.0;
+    if (p < n) {
+      const double* Yc = (s == 0) ? X : (s == S - 1 ? Xn : a.Yb[s & 1]);
+      const PatchCoef cf = load_coef(a.coef, n, p);
+      double y[D], kf[D], td[NC], sn[NC];
+#pragma unroll
+      for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
+#pragma unroll
+      for (int k = 0; k < NC; ++k) {
+        td[k] = rowsum[t * G + k];
+        sn[k] = a.sinc[NC * p + k];
+      }
+      const double sdn = (G > NC) ? rowsum[t * G + NC] : 0.0;
+      rsf_patch_rhs<NC>(a.fp, cf, y, td, sdn, sn, a.sinc[NC * n + p], kf);
+      double* Ks = a.Stg[c->kslot[s]];
+#pragma unroll
+      for (int f = 0; f < D; ++f) Ks[f * n + p] = kf[f];
+      if (s > 0 && s < S - 1) {
+        // next stage state
+        const double h = c->h;
+        double acc[D];
+#pragma unroll
+        for (int f = 0; f < D; ++f) acc[f] = 0.0;
+        for (int j = 0; j < s; ++j) {
+          const double w = a.tb.A[s + 1][j];
+          if (w == 0.0) continue;
+          const double* Kj = a.Stg[c->kslot[j]];
+#pragma unroll
+          for (int f = 0; f < D; ++f) acc[f] += w * Kj[f * n + p];
+        }
+        {
+          const double w = a.tb.A[s + 1][s];
+#pragma unroll
+          for (int f = 0; f < D; ++f) acc[f] += w * kf[f];
+        }
+        double* Yn = (s + 1 == S - 1) ? Xn : a.Yb[(s + 1) & 1];
+        double yn[D];
+#pragma unroll
+        for (int f = 0; f < D; ++f) {
+          yn[f] = X[f * n + p] + h * acc[f];
+          Yn[f * n + p] = yn[f];
+        }
+        write_vel<VT, NC>(a.fp, cf.a, yn, static_cast<VT*>(a.vout), p);
+        if (!rsf_patch_ok<NC>(a.fp, cf, yn)) atomicOr(&c->stage_fail, 1);
+      } else {
+        // reductions of the state y (initial state or the new solution)
+        if (s == S - 1) {
+          const double h = c->h;
+          double en = 0.0;
+#pragma unroll
+          for (int f = 0; f < D; ++f) {
+            double e = 0.0;
+            for (int j = 0; j < S - 1; ++j) {
+              const double w = a.tb.bh[j];
+              if (w != 0.0) e += w * a.Stg[c->kslot[j]][f * n + p];
+            }
+            e += a.tb.bh[S - 1] * kf[f];
+            const double ye = X[f * n + p] + h * e;
+            const double tol = a.so.atol[f] + a.so.rtol * fmax(fabs(y[f]), fabs(ye));
+            if (tol > 0.0) en = fmax(en, fabs(y[f] - ye) / tol);
+            const double dm = a.Xmon[f * n + p] - y[f];
+            r9[RED_DX2] += dm * dm;
+          }
+          r9[RED_ERR] = en;
+        }
+        const double vm = vel_mag<NC>(a.fp, cf.a, y);
+        r9[RED_VMAX] = vm;
+        r9[RED_VSUM] = vm;
+        r9[RED_SLIP] = y[NC + 2];
+        r9[RED_MU] = y[1] / y[NC + 1];
+        r9[RED_SMAX] = y[NC + 1];
+        r9[RED_SMIN] = y[NC + 1];
+#pragma unroll
+        for (int f = 0; f < D; ++f) r9[RED_XN2] += y[f] * y[f];
+      }
+    }
+    if (reduce) {
+#pragma unroll
+      for (int k = 0; k < kNRed; ++k) pred[t][k] = r9[k];
+    }
+  }
+  if (!reduce) return;
+  __syncthreads();
+  if (t == 0) {
+    double q[kNRed];
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) q[k] = pred[0][k];
+    for (int w = 1; w < PPW; ++w) {
+      q[RED_ERR] = fmax(q[RED_ERR], pred[w][RED_ERR]);
+      q[RED_VMAX] = fmax(q[RED_VMAX], pred[w][RED_VMAX]);
+      q[RED_SMAX] = fmax(q[RED_SMAX], pred[w][RED_SMAX]);
+      q[RED_SMIN] = fmin(q[RED_SMIN], pred[w][RED_SMIN]);
+      q[RED_VSUM] += pred[w][RED_VSUM];
+      q[RED_SLIP] += pred[w][RED_SLIP];
+      q[RED_MU] += pred[w][RED_MU];
+      q[RED_XN2] += pred[w][RED_XN2];
+      q[RED_DX2] += pred[w][RED_DX2];
+    }
+    double* out = a.partials + (size_t)blockIdx.x * kNRed;
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) out[k] = q[k];
+    __threadfence();
+    const unsigned tk = atomicAdd(&c->counter, 1u);
+    s_last = (tk == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  // grid reduction over the partials in fixed order
+  {
+    __shared__ double sh[8];
+    const int nb = gridDim.x;
+    double q[kNRed];
+    q[RED_ERR] = 0.0; q[RED_VMAX] = 0.0; q[RED_VSUM] = 0.0; q[RED_SLIP] = 0.0;
+    q[RED_MU] = 0.0; q[RED_SMAX] = -1e30; q[RED_SMIN] = 1e30; q[RED_XN2] = 0.0;
+    q[RED_DX2] = 0.0;
+    for (int b = t; b < nb; b += blockDim.x) {
+      const double* pb = a.partials + (size_t)b * kNRed;
+      q[RED_ERR] = fmax(q[RED_ERR], pb[RED_ERR]);
+      q[RED_VMAX] = fmax(q[RED_VMAX], pb[RED_VMAX]);
+      q[RED_SMAX] = fmax(q[RED_SMAX], pb[RED_SMAX]);
+      q[RED_SMIN] = fmin(q[RED_SMIN], pb[RED_SMIN]);
+      q[RED_VSUM] += pb[RED_VSUM];
+      q[RED_SLIP] += pb[RED_SLIP];
+      q[RED_MU] += pb[RED_MU];
+      q[RED_XN2] += pb[RED_XN2];
+      q[RED_DX2] += pb[RED_DX2];
+    }
+    const int ops[kNRed] = {1, 1, 0, 0, 0, 1, 2, 0, 0};
+    for (int k = 0; k < kNRed; ++k) {
+      const double r = block_reduce(q[k], ops[k], sh);
+      if (t == 0) gred[k] = r;
+    }
+    __syncthreads();
+  }
+  controller<NC>(a, gred);
+}
+
+// ------------------------------------------------------------ host side
+template <typename TK, int NC, int G>
+struct StageCfg {
+  static constexpr int PPW = (G == 3) ? 2 : (G == 1 ? 8 : 4);
+};
+
+static void fill_tableau(Tableau& tb, int method) {
+  memset(&tb, 0, sizeof(tb));
+  if (method == RSFG_RK3BS) {
+    tb.s = 4;
+    tb.order = 3;
+    tb.A[1][0] = 0.5;
+    tb.A[2][1] = 0.75;
+    tb.A[3][0] = 2.0 / 9.0; tb.A[3][1] = 1.0 / 3.0; tb.A[3][2] = 4.0 / 9.0;
+    tb.b[0] = 2.0 / 9.0; tb.b[1] = 1.0 / 3.0; tb.b[2] = 4.0 / 9.0;
+    tb.bh[0] = 7.0 / 24.0; tb.bh[1] = 0.25; tb.bh[2] = 1.0 / 3.0; tb.bh[3] = 0.125;
+  } else {
+    tb.s = 7;
+    tb.order = 5;
+    tb.A[1][0] = 0.2;
+    tb.A[2][0] = 3.0 / 40.0; tb.A[2][1] = 9.0 / 40.0;
+    tb.A[3][0] = 44.0 / 45.0; tb.A[3][1] = -56.0 / 15.0; tb.A[3][2] = 32.0 / 9.0;
+    tb.A[4][0] = 19372.0 / 6561.0; tb.A[4][1] = -25360.0 / 2187.0;
+    tb.A[4][2] = 64448.0 / 6561.0; tb.A[4][3] = -212.0 / 729.0;
+    tb.A[5][0] = 9017.0 / 3168.0; tb.A[5][1] = -355.0 / 33.0; tb.A[5][2] = 46732.0 / 5247.0;
+    tb.A[5][3] = 49.0 / 176.0; tb.A[5][4] = -5103.0 / 18656.0;
+    tb.A[6][0] = 35.0 / 384.0; tb.A[6][2] = 500.0 / 1113.0; tb.A[6][3] = 125.0 / 192.0;
+    tb.A[6][4] = -2187.0 / 6784.0; tb.A[6][5] = 11.0 / 84.0;
+    for (int j = 0; j < 7; ++j) tb.b[j] = tb.A[6][j];
+    tb.bh[0] = 5179.0 / 57600.0; tb.bh[2] = 7571.0 / 16695.0; tb.bh[3] = 393.0 / 640.0;
+    tb.bh[4] = -92097.0 / 339200.0; tb.bh[5] = 187.0 / 2100.0; tb.bh[6] = 1.0 / 40.0;
+  }
+}
+
+const void* operator_ptr(rsfg_ctx* h, int prec);
+
+struct Launcher {
+  virtual ~Launcher() {}
+  virtual void init_vel(const StageArgs& a, hipStream_t st) = 0;
+  virtual void prep(const StageArgs& a, hipStream_t st) = 0;
+  virtual void stage(const StageArgs& a, hipStream_t st) = 0;
+  virtual int wg_count(int n) const = 0;
+};
+
+template <typename TK, int NC, int G>
+struct LauncherT : Launcher {
+  using VT = typename KTraits<TK>::VT;
+  static constexpr int PPW = StageCfg<TK, NC, G>::PPW;
+  void init_vel(const StageArgs& a, hipStream_t st) override {
+    hipLaunchKernelGGL((k_init_vel<VT, NC>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);
+  }
+  void prep(const StageArgs& a, hipStream_t st) override {
+    hipLaunchKernelGGL((k_prep<VT, NC>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);
+  }
+  void stage(const StageArgs& a, hipStream_t st) override {
+    hipLaunchKernelGGL((k_stage<TK, NC, G, PPW>), dim3(wg_count(a.n)), dim3(256), 0, st, a);
+  }
+  int wg_count(int n) const override { return (n + PPW - 1) / PPW; }
+};
+
+template <typename TK>
+static Launcher* make_launcher_t(int ncomp, int G) {
+  if (ncomp == 1) {
+    if (G == 1) return new LauncherT<TK, 1, 1>();
+    return new LauncherT<TK, 1, 2>();
+  }
+  if (G == 2) return new LauncherT<TK, 2, 2>();
+  return new LauncherT<TK, 2, 3>();
+}
+
+static Launcher* make_launcher(int prec, int ncomp, int G) {
+  switch (prec) {
+    case RSFG_PREC_FP32: return make_launcher_t<float>(ncomp, G);
+    case RSFG_PREC_BF16X2: return make_launcher_t<bf16x2_t>(ncomp, G);
+    case RSFG_PREC_BF16: return make_launcher_t<bf16_t>(ncomp, G);
+    default: return make_launcher_t<double>(ncomp, G);
+  }
+}
+
+int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st) {
+  const auto wall0 = std::chrono::steady_clock::now();
+  const void* K = operator_ptr(h, o->prec);
+  RSFG_CHECK(K != nullptr, RSFG_ESTATE, "rsfg_solve: operator precision not assembled");
+  RSFG_CHECK(o->method == RSFG_RK5DP || o->method == RSFG_RK3BS, RSFG_EINVAL,
+             "rsfg_solve: unknown method");
+  const int n = h->n;
+  Launcher* L = make_launcher(o->prec, h->ncomp, h->G);
+  const int nwg = L->wg_count(n);
+  if ((size_t)nwg * kNRed > h->partial_count) {
+    delete L;
+    set_error("rsfg_solve: partial buffer too small");
+    return RSFG_ESTATE;
+  }
+  StageArgs a;
+  memset(&a, 0, sizeof(a));
+  a.K = K;
+  a.ld = h->ld;
+  a.rows_all = h->rows_all;
+  a.cols = h->cols;
+  a.n = n;
+  a.G = h->G;
+  a.coef = h->d_coef;
+  a.sinc = h->d_sinc;
+  a.fp = h->fp;
+  for (int i = 0; i < 2; ++i) {
+    a.Xb[i] = h->Xb[i];
+    a.Yb[i] = h->Yb[i];
+  }
+  for (int i = 0; i < 8; ++i) a.Stg[i] = h->Stg[i];
+  a.Xmon = h->Xmon;
+  a.partials = h->partials;
+  a.ev = h->d_ev;
+  a.mon = h->d_mon;
+  a.tau0 = h->d_cat;
+  a.slip0 = h->d_cat + n;
+  a.rupt = h->d_rupt;
+  a.ctl = h->d_ctl;
+  fill_tableau(a.tb, o->method);
+  a.so.rtol = o->rtol;
+  for (int f = 0; f < 6; ++f) a.so.atol[f] = o->atol[f];
+  a.so.dt_monitor = o->dt_monitor;
+  a.so.rdx = o->rdx_monitor;
+  a.so.adx = o->adx_monitor;
+  a.so.vel_event = o->vel_event;
+  a.so.hyst = o->vel_event_hyst;
+  a.so.rupture_vth = o->rupture_vth;
+  a.so.max_domain_reject = o->max_domain_reject;
+  // controller state for this solve
+  DevCtl* c = h->h_ctl;
+  memset(c, 0, sizeof(*c));
+  c->t = h->t;
+  c->h = o->dt_init;
+  c->t_end = t_end;
+  c->hmax = (o->dt_max < o->dt_monitor) ? o->dt_max : o->dt_monitor;
+  c->max_steps = o->max_steps;
+  for (int i = 0; i < 8; ++i) c->kslot[i] = i;
+  c->max_event = kMaxEv;
+  c->max_mon = kMaxMon;
+  c->xcur = h->xcur;
+  RSFG_HIP(hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, h->stream));
+  RSFG_HIP(hipMemsetAsync(h->d_rupt, 0, sizeof(int) * n, h->stream));
+  // K_0 at the initial state
+  a.s = 0;
+  a.vout = h->vin[0];
+  L->init_vel(a, h->stream);
+  a.vin = h->vin[0];
+  a.vout = nullptr;
+  L->stage(a, h->stream);
+  RSFG_HIP(hipGetLastError());
+  RSFG_HIP(hipMemcpyAsync(c, h->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->stream));
+  RSFG_HIP(hipStreamSynchronize(h->stream));
+  if (c->status == RSFG_EDOMAIN) {
+    delete L;
+    set_error("rsfg_solve: initial state fails the domain check");
+    return RSFG_EDOMAIN;
+  }
+  // capture one graph of step attempts
+  const int S = a.tb.s;
+  const int per = o->steps_per_launch > 0 ? o->steps_per_launch : 32;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  RSFG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < per; ++k) {
+    StageArgs b = a;
+    b.s = 0;
+    b.vout = h->vin[1];
+    L->prep(b, h->stream);
+    for (int s = 1; s < S; ++s) {
+      b.s = s;
+      b.vin = h->vin[s & 1];
+      b.vout = (s < S - 1) ? h->vin[(s + 1) & 1] : nullptr;
+      L->stage(b, h->stream);
+    }
+  }
+  RSFG_HIP(hipStreamEndCapture(h->stream, &graph));
+  RSFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  long launches = 0;
+  int rc = RSFG_OK;
+  while (!c->halted) {
+    hipError_t e = hipGraphLaunch(exec, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c, h->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "rsfg_solve graph launch", __FILE__, __LINE__);
+      break;
+    }
+    ++launches;
+  }
+  hipGraphExecDestroy(exec);
+  hipGraphDestroy(graph);
+  delete L;
+  if (rc != RSFG_OK) return rc;
+  h->xcur = c->xcur;
+  h->t = c->t;
+  h->nevent = c->nevent;
+  h->nmon = c->nmon;
+  if (st) {
+    st->steps = c->steps;
+    st->rejects = c->rejects;
+    st->domain_rejects = c->domain_rejects;
+    st->rhs_evals = c->rhs_evals;
+    st->launches = launches;
+    st->t = c->t;
+    st->dt = c->h;
+    st->status = c->status == 2 ? RSFG_OK : c->status;
+    st->nevent = c->nevent;
+    st->nmonitor = c->nmon;
+    st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+  }
+  return RSFG_OK;
+}
+
+}  // namespace rsfg

@@ -1,0 +1,87 @@
+"""Seeded fault meshes and friction fields of the benchmark configurations.
+
+Planar meshes follow the layout of the reference's BP5 generator
+(seas_bp_tests/bp5/make_bp5.py): a vertical strike-slip fault in the x = 0
+plane, strike 0 (strike direction +y), dip 90, square cells of side ds,
+patch k = i * nz + j with i along strike and j down dip, half lengths
+l = w = ds / 2 [m].
+"""
+import numpy as np
+
+SEC_PER_YEAR = 31557600.0
+
+
+def planar_grid(ds_km, length_km=100.0, depth_km=40.0):
+    nx, nz = int(round(length_km / ds_km)), int(round(depth_km / ds_km))
+    i, j = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    y = (i + 0.5 - nx / 2.0) * ds_km * 1e3
+    z = -(j + 0.5) * ds_km * 1e3
+    n = nx * nz
+    half = ds_km * 1e3 / 2.0
+    xyz = np.stack([np.zeros(n), y, z], axis=1)
+    return dict(xyz=xyz, strike=np.zeros(n, np.float32),
+                dip=np.full(n, 90.0, np.float32), l=np.full(n, half),
+                w=np.full(n, half), nx=nx, nz=nz, ds_km=ds_km,
+                along=(i + 0.5 - nx / 2.0) * ds_km, depth=(j + 0.5) * ds_km)
+
+
+def write_patch_file(fname, g):
+    """Reference patch-file format "x y z strike dip L W group"."""
+    grp = g.get("group", np.zeros(len(g["l"]), int))
+    with open(fname, "w") as f:
+        for k in range(len(g["l"])):
+            x, y, z = g["xyz"][k]
+            f.write(f"{x:.6e} {y:.6e} {z:.6e} {float(g['strike'][k]):.6f} "
+                    f"{float(g['dip'][k]):.6f} {g['l'][k]:.6e} {g['w'][k]:.6e} "
+                    f"{int(grp[k])}\n")
+
+
+def bp5_fields(g, a0=0.004, a_max=0.04, b0=0.03, dc0=0.14, dc_nuc=0.13,
+               f0=0.6, v0=1e-6, vpl=1e-9, sigma=25e6, shear_modulus=3.204e10,
+               cs=3464.0, vnuc=3e-2):
+    """SEAS BP5-QD fields as written by make_bp5.py (ic convention 1)."""
+    x, dep = g["along"], g["depth"]
+    r = np.maximum(np.abs(dep - 10) - 6, np.abs(x) - 30) / 2.0
+    a = np.clip(a0 + r * (a_max - a0), a0, a_max)
+    b = np.full_like(a, b0)
+    nuc = (np.abs(x + 24) < 6) & (np.abs(dep - 10) < 6)
+    v = np.where(nuc, vnuc, vpl)
+    dc = np.where(nuc, dc_nuc, dc0)
+    eta = shear_modulus / (2.0 * cs)
+    tau = sigma * a * np.arcsinh(0.5 * v / v0 * np.exp((f0 + b0 * np.log(v0 / vpl)) / a)) + eta * v
+    return dict(a=a, b=b, dc=dc, tau=tau, vel=v, sigma=np.full_like(a, sigma),
+                f0=f0, v0=v0, vpl=vpl, eta=eta, shear_modulus=shear_modulus, cs=cs)
+
+
+def cycle_fields(g, seed=20261202, a_vw=0.004, b0=0.03, amb_excess=0.04,
+                 taper_km=2.0, dc0=0.14, dc_nuc=0.13, f0=0.6, v0=1e-6,
+                 vpl=1e-9, sigma=25e6, shear_modulus=32e9, cs=3464.0,
+                 perturb=1e-3, box=(30.0, 10.0, 6.0), nuc=(-24.0, 10.0, 6.0)):
+    """Friction fields of the planar benchmark cycles.
+
+    a = a_vw and b = b0 inside the central seismogenic box (|along| <= 30 km,
+    |depth - 10| <= 6 km, i.e. 60 x 12 km); outside, a - b ramps linearly over
+    taper_km to +amb_excess with b held at b0 (so a -> b0 + amb_excess);
+    D_c = dc0 with dc_nuc in a 12 x 12 km nucleation square (the BP5 one);
+    initial slip rate vpl (1 + perturb N(0,1)) from a seeded PCG64 stream,
+    state at steady state for that rate, and the quasi-dynamic prestress
+    tau = sigma a asinh(v/(2 v0) e^(psi_ss/a)) + eta v.
+    """
+    x, dep = g["along"], g["depth"]
+    half_len, cdep, half_dep = box
+    r = np.maximum(np.abs(dep - cdep) - half_dep, np.abs(x) - half_len) / taper_km
+    amb_in = a_vw - b0
+    amb = np.clip(amb_in + r * (amb_excess - amb_in), amb_in, amb_excess)
+    b = np.full_like(x, b0)
+    a = amb + b
+    nx0, nd0, nh = nuc
+    nucm = (np.abs(x - nx0) < nh) & (np.abs(dep - nd0) < nh)
+    dc = np.where(nucm, dc_nuc, dc0)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    v = vpl * (1.0 + perturb * rng.standard_normal(len(x)))
+    eta = shear_modulus / (2.0 * cs)
+    psi_ss = f0 + b * np.log(v0 / v)
+    tau = sigma * a * np.arcsinh(0.5 * v / v0 * np.exp(psi_ss / a)) + eta * v
+    return dict(a=a, b=b, dc=dc, tau=tau, vel=v, sigma=np.full_like(x, sigma),
+                f0=f0, v0=v0, vpl=vpl, eta=eta, shear_modulus=shear_modulus, cs=cs)

@@ -1,0 +1,216 @@
+"""Host-side mirror of the rsf_solve driver for the dense GPU path.
+
+``FaultSystem`` owns one librsfg handle and follows the call sequence of
+rsf_solve_run (src/rsf_solve.c:85-668): geometry -> operator assembly
+(calc_petsc_Isn_matrices) -> per-patch friction -> backslip loading ->
+initial state -> adaptive integration with monitor and event output.
+All compute runs in librsfg on the GPU; this module only marshals arrays.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+
+SEC_PER_YEAR = 31557600.0
+G_INTERNAL = 1.0e4   # interact's internal shear modulus (src/includes/properties.h:23)
+
+
+class FaultSystem:
+    def __init__(self, patches, device=0):
+        """patches: ctypes array of lib.Patch or a dict of arrays
+        (xyz, strike, dip, l, w[, rake, group])."""
+        if isinstance(patches, dict):
+            patches = L.patches_from_arrays(
+                patches["xyz"], patches["strike"], patches["dip"],
+                patches["l"], patches["w"], patches.get("rake"),
+                patches.get("group"))
+        self._lib = L.load()
+        self.n = len(patches)
+        self._patches = patches
+        hh = C.c_void_p()
+        L.check(self._lib.rsfg_create(device, patches, self.n, C.byref(hh)),
+                "rsfg_create")
+        self.h = hh
+        self.ncomp = 1
+        self.G = 1
+        self.precisions = 0
+        self.assembly_ms = None
+
+    def close(self):
+        if self.h:
+            self._lib.rsfg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ------------------------------------------------------------ operator
+    def assemble(self, slip_mode=L.SLIP_STRIKE, *, shear_modulus=3.204e10,
+                 full_space=False, with_normal=False, store=L.PREC_FP64,
+                 scale=None):
+        a = L.Assembly(slip_mode, int(with_normal), int(full_space), store,
+                       shear_modulus / G_INTERNAL if scale is None else scale)
+        ms = C.c_double()
+        L.check(self._lib.rsfg_assemble(self.h, C.byref(a), C.byref(ms)),
+                "rsfg_assemble")
+        self.ncomp = 2 if slip_mode == L.SLIP_VECTOR else 1
+        self.G = self.ncomp + int(bool(with_normal))
+        self.precisions = store | L.PREC_FP64
+        self.assembly_ms = ms.value
+        self.dim = 2 * self.ncomp + 2
+        return ms.value
+
+    def dims(self):
+        r, c, ld = C.c_int(), C.c_int(), C.c_int()
+        L.check(self._lib.rsfg_operator_dims(self.h, C.byref(r), C.byref(c),
+                                             C.byref(ld)), "dims")
+        return r.value, c.value, ld.value
+
+    def entries(self, rows, cols):
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        out = np.zeros(len(rows) * len(cols))
+        L.check(self._lib.rsfg_entries(self.h, len(rows), rows, len(cols),
+                                       cols, out), "rsfg_entries")
+        return out.reshape(len(cols), len(rows)).T.copy()
+
+    def dense(self):
+        r, c, _ = self.dims()
+        return self.entries(np.arange(r), np.arange(c))
+
+    def traction_pairs(self, recv, src, full_space=False):
+        recv = np.ascontiguousarray(recv, np.int32)
+        src = np.ascontiguousarray(src, np.int32)
+        out = np.zeros(6 * len(recv))
+        L.check(self._lib.rsfg_traction_pairs(self.h, len(recv), recv, src,
+                                              int(full_space), out),
+                "rsfg_traction_pairs")
+        return out.reshape(-1, 2, 3)
+
+    def matvec(self, x, prec=L.PREC_FP64):
+        _, c, _ = self.dims()
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.ncomp * self.n)
+        L.check(self._lib.rsfg_matvec(self.h, prec, x, y), "rsfg_matvec")
+        return y
+
+    def matvec_t(self, x, prec=L.PREC_FP64):
+        _, c, _ = self.dims()
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(c)
+        L.check(self._lib.rsfg_matvec_t(self.h, prec, x, y), "rsfg_matvec_t")
+        return y
+
+    def matvec_bench(self, prec=L.PREC_FP64, nrep=50):
+        ms = C.c_double()
+        L.check(self._lib.rsfg_matvec_bench(self.h, prec, nrep, C.byref(ms)),
+                "rsfg_matvec_bench")
+        return ms.value
+
+    # ------------------------------------------------------------ physics
+    def set_friction(self, a, b, dc, *, law=1, v0=1e-6, f0=0.6, eta=None,
+                     shear_modulus=3.204e10, s_wave_speed=3464.0, rd_fac=1.0,
+                     vmin_state=1e-16, sato_beta=1e-2, kt_vc=-1.0,
+                     calc_sigma_dot=False, limit_sigma=False, min_sigma=1e6,
+                     max_sigma=300e6):
+        if eta is None:   # src/rsf_init.c:427
+            eta = rd_fac * shear_modulus / (2.0 * s_wave_speed)
+        self.eta = eta
+        f = L.Friction(law, v0, f0, eta, vmin_state, sato_beta, kt_vc,
+                       int(calc_sigma_dot), int(limit_sigma), min_sigma,
+                       max_sigma)
+        n = self.n
+        bc = lambda v: np.ascontiguousarray(np.broadcast_to(v, (n,)), np.float64)
+        L.check(self._lib.rsfg_set_friction(self.h, C.byref(f), bc(a), bc(b),
+                                            bc(dc)), "rsfg_set_friction")
+
+    def set_loading(self, vpl=1e-9, vpl_patch=None, taudot_add=None,
+                    sigdot_add=None):
+        keep = []
+
+        def ptr(v):
+            if v is None:
+                return None
+            v = np.ascontiguousarray(np.broadcast_to(v, (self.n,)), np.float64)
+            keep.append(v)
+            return v.ctypes.data_as(C.c_void_p)
+
+        L.check(self._lib.rsfg_set_loading(self.h, vpl, ptr(vpl_patch),
+                                           ptr(taudot_add), ptr(sigdot_add)),
+                "rsfg_set_loading")
+
+    def rhs(self, X, prec=L.PREC_FP64):
+        X = np.ascontiguousarray(X, np.float64)
+        F = np.zeros_like(X)
+        ok = C.c_int()
+        L.check(self._lib.rsfg_rhs(self.h, prec, X, F, C.byref(ok)),
+                "rsfg_rhs")
+        return F, bool(ok.value)
+
+    def set_state(self, X, t=0.0):
+        L.check(self._lib.rsfg_set_state(self.h, t,
+                                         np.ascontiguousarray(X, np.float64)),
+                "rsfg_set_state")
+
+    def get_state(self):
+        X = np.zeros(self.n * self.dim)
+        t = C.c_double()
+        L.check(self._lib.rsfg_get_state(self.h, C.byref(t), X),
+                "rsfg_get_state")
+        return X, t.value
+
+    @staticmethod
+    def solver_options(**kw):
+        o = L.SolverOpts()
+        L.load().rsfg_solver_defaults(C.byref(o))
+        for k, v in kw.items():
+            if k == "atol":
+                for i, a in enumerate(v):
+                    o.atol[i] = a
+            else:
+                setattr(o, k, v)
+        return o
+
+    def solve(self, t_end, **kw):
+        o = self.solver_options(**kw)
+        st = L.Stats()
+        L.check(self._lib.rsfg_solve(self.h, C.byref(o), t_end, C.byref(st)),
+                "rsfg_solve")
+        return {k: getattr(st, k) for k, _ in L.Stats._fields_}
+
+    def events(self, cap=4096):
+        rows = np.zeros(8 * cap)
+        n = C.c_int(cap)
+        L.check(self._lib.rsfg_events(self.h, rows, C.byref(n)), "events")
+        return rows[:8 * n.value].reshape(-1, 8).copy()
+
+    def monitor(self, cap=1 << 18):
+        rows = np.zeros(10 * cap)
+        n = C.c_int(cap)
+        L.check(self._lib.rsfg_monitor(self.h, rows, C.byref(n)), "monitor")
+        return rows[:10 * n.value].reshape(-1, 10).copy()
+
+
+def initial_state(tau, sigma, vel, a, v0, ncomp=1):
+    """State from per-cell (tau, sigma, v), src/rsf_solve.c:378-423:
+    psi = a ln(2 v0 / v sinh(tau / (sigma a))), slip = 0."""
+    tau, sigma, vel, a = (np.asarray(v, np.float64) for v in (tau, sigma, vel, a))
+    n = len(a)
+    psi = a * np.log(2.0 * v0 / vel * np.sinh(tau / sigma / a))
+    if ncomp == 1:
+        X = np.zeros((n, 4))
+        X[:, 0], X[:, 1], X[:, 2] = psi, tau, sigma
+    else:
+        X = np.zeros((n, 6))
+        X[:, 0], X[:, 1], X[:, 3] = psi, tau, sigma
+    return X.ravel()

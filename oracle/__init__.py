@@ -1,0 +1,249 @@
+"""CPU oracle for the rsf_solve dense hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package, and only as the checker / the timed
+CPU baseline.  The product (``interact_amd``) never imports it.
+
+Python bindings (ctypes) for ``librsfo.so`` built from ``rsfo_okada.c`` and
+``rsfo_rsf.c`` in this directory (``make -C oracle``).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+SEC_PER_YEAR = 31557600.0
+
+
+def build(force=False):
+    """Compile librsfo.so (gcc, OpenMP)."""
+    so = os.path.join(_HERE, "librsfo.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-C", _HERE],
+                       check=True)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "librsfo.so")
+        if not os.path.exists(so):
+            build()
+        _LIB = C.CDLL(so)
+        _setup(_LIB)
+    return _LIB
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class Problem(C.Structure):
+    _fields_ = [("n", C.c_int), ("ncomp", C.c_int),
+                ("K", C.c_void_p), ("Kn", C.c_void_p),
+                ("a", C.c_void_p), ("b", C.c_void_p), ("dc", C.c_void_p),
+                ("sinc", C.c_void_p), ("sinc_n", C.c_void_p),
+                ("v0", C.c_double), ("f0", C.c_double), ("eta", C.c_double),
+                ("vmin_state", C.c_double), ("sato_beta", C.c_double),
+                ("kt_vc", C.c_double), ("law", C.c_int),
+                ("calc_sigma_dot", C.c_int), ("limit_sigma", C.c_int),
+                ("min_sigma", C.c_double), ("max_sigma", C.c_double),
+                ("nthreads", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("method", C.c_int), ("rtol", C.c_double),
+                ("atol", C.c_double * 6), ("t0", C.c_double),
+                ("t_end", C.c_double), ("dt_init", C.c_double),
+                ("dt_max", C.c_double), ("dt_monitor", C.c_double),
+                ("rdx_monitor", C.c_double), ("adx_monitor", C.c_double),
+                ("vel_event", C.c_double), ("vel_event_hyst", C.c_double),
+                ("rupture_vth", C.c_double), ("max_steps", C.c_long),
+                ("max_domain_reject", C.c_int), ("max_events", C.c_int),
+                ("max_monitor", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("steps", C.c_long), ("rejects", C.c_long),
+                ("domain_rejects", C.c_long), ("rhs_evals", C.c_long),
+                ("t_final", C.c_double), ("dt_final", C.c_double),
+                ("status", C.c_int), ("nevent", C.c_int),
+                ("ev", C.c_void_p), ("nmon", C.c_int), ("mon", C.c_void_p)]
+
+
+def _setup(L):
+    L.rsfo_dc3d.restype = C.c_int
+    L.rsfo_dc3d.argtypes = [C.c_double] * 12 + [C.c_int, _dp]
+    L.rsfo_patch_frame.restype = None
+    L.rsfo_patch_frame.argtypes = [C.c_float, C.c_float, _dp]
+    L.rsfo_traction_tensor.restype = None
+    L.rsfo_traction_tensor.argtypes = [C.c_int, _dp, _fp, _fp, _dp, _dp,
+                                       C.c_int, C.c_int, _dp, C.c_void_p]
+    L.rsfo_read_patches.restype = C.c_int
+    L.rsfo_read_patches.argtypes = [C.c_char_p, C.c_int, _dp, _fp, _fp, _dp,
+                                    _dp, _ip]
+    L.rsfo_vel.restype = C.c_double
+    L.rsfo_vel.argtypes = [C.c_double] * 5
+    L.rsfo_state_rate.restype = C.c_double
+    L.rsfo_state_rate.argtypes = [C.POINTER(Problem), C.c_int, C.c_double,
+                                  C.c_double, C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
+    L.rsfo_rhs.restype = None
+    L.rsfo_rhs.argtypes = [C.POINTER(Problem), _dp, _dp, C.c_void_p,
+                           C.c_void_p]
+    L.rsfo_domain_ok.restype = C.c_int
+    L.rsfo_domain_ok.argtypes = [C.POINTER(Problem), _dp]
+    L.rsfo_solve.restype = C.c_int
+    L.rsfo_solve.argtypes = [C.POINTER(Problem), C.POINTER(Options), _dp,
+                             C.POINTER(Result)]
+    L.rsfo_rk_attempt.restype = C.c_int
+    L.rsfo_rk_attempt.argtypes = [C.POINTER(Problem), C.c_int, C.c_double,
+                                  _dp, C.c_double, _dp, _dp, _dp, _dp,
+                                  C.POINTER(C.c_double)]
+
+
+# ------------------------------------------------------------------ geometry
+def read_patches(fname, nmax=10_000_000):
+    """Parse a reference patch file; returns dict of arrays."""
+    n_lines = sum(1 for _ in open(fname))
+    nmax = min(nmax, n_lines)
+    xyz = np.zeros(3 * nmax)
+    st = np.zeros(nmax, np.float32)
+    dp = np.zeros(nmax, np.float32)
+    ll = np.zeros(nmax)
+    ww = np.zeros(nmax)
+    gr = np.zeros(nmax, np.int32)
+    n = lib().rsfo_read_patches(fname.encode(), nmax, xyz, st, dp, ll, ww, gr)
+    if n < 0:
+        raise OSError(fname)
+    return dict(xyz=xyz[:3 * n].reshape(n, 3).copy(), strike=st[:n].copy(),
+                dip=dp[:n].copy(), l=ll[:n].copy(), w=ww[:n].copy(),
+                group=gr[:n].copy())
+
+
+def dc3d(alpha, x, y, z, depth, dip, al1, al2, aw1, aw2, d1, d2, full=0):
+    u = np.zeros(12)
+    iret = lib().rsfo_dc3d(alpha, x, y, z, depth, dip, al1, al2, aw1, aw2,
+                           d1, d2, int(full), u)
+    return u, iret
+
+
+def patch_frame(strike, dip):
+    fr = np.zeros(13)
+    lib().rsfo_patch_frame(float(strike), float(dip), fr)
+    return fr
+
+
+def traction_tensor(g, full_space=0, nthreads=8, want_iret=False):
+    """T[i, j, slip(strike,dip), trac(strike,dip,normal)], internal units."""
+    n = len(g["l"])
+    T = np.zeros(n * n * 6)
+    ir = np.zeros(2 * n * n, np.int32) if want_iret else None
+    lib().rsfo_traction_tensor(
+        n, np.ascontiguousarray(g["xyz"], np.float64).ravel(),
+        np.ascontiguousarray(g["strike"], np.float32),
+        np.ascontiguousarray(g["dip"], np.float32),
+        np.ascontiguousarray(g["l"], np.float64),
+        np.ascontiguousarray(g["w"], np.float64), int(full_space),
+        int(nthreads), T,
+        ir.ctypes.data_as(C.c_void_p) if want_iret else None)
+    T = T.reshape(n, n, 2, 3)
+    if want_iret:
+        return T, ir.reshape(n, n, 2)
+    return T
+
+
+# ------------------------------------------------------------------- physics
+class RSFProblem:
+    """Keeps numpy arrays alive behind the ctypes Problem struct."""
+
+    def __init__(self, K, a, b, dc, sinc, *, ncomp=1, Kn=None, sinc_n=None,
+                 v0=1e-6, f0=0.6, eta=0.0, law=1, vmin_state=1e-16,
+                 sato_beta=1e-2, kt_vc=None, calc_sigma_dot=False,
+                 limit_sigma=False, min_sigma=1e6, max_sigma=300e6,
+                 nthreads=8):
+        self._keep = []
+
+        def arr(x):
+            if x is None:
+                return None
+            x = np.ascontiguousarray(x, np.float64)
+            self._keep.append(x)
+            return x.ctypes.data
+
+        n = len(a)
+        self.n, self.ncomp = n, ncomp
+        self.p = Problem(n, ncomp, arr(K), arr(Kn), arr(a), arr(b), arr(dc),
+                         arr(sinc), arr(sinc_n), v0, f0, eta, vmin_state,
+                         sato_beta, 1e-2 * v0 if kt_vc is None else kt_vc,
+                         law, int(calc_sigma_dot), int(limit_sigma),
+                         min_sigma, max_sigma, nthreads)
+
+    @property
+    def dim(self):
+        return 6 if self.ncomp == 2 else 4
+
+    def rhs(self, X):
+        X = np.ascontiguousarray(X, np.float64)
+        F = np.zeros_like(X)
+        lib().rsfo_rhs(C.byref(self.p), X, F, None, None)
+        return F
+
+    def domain_ok(self, X):
+        return bool(lib().rsfo_domain_ok(C.byref(self.p),
+                                         np.ascontiguousarray(X, np.float64)))
+
+    def state_rate(self, i, psi, v):
+        d1, d2 = C.c_double(), C.c_double()
+        S = lib().rsfo_state_rate(C.byref(self.p), i, psi, v, C.byref(d1),
+                                  C.byref(d2))
+        return S, d1.value, d2.value
+
+    def rk_attempt(self, X, F0, h, method=0, rtol=1e-8, atol=None):
+        atol = np.asarray(atol if atol is not None else
+                          [1e-12, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3][:self.dim],
+                          np.float64)
+        Xn = np.zeros_like(X)
+        Fn = np.zeros_like(X)
+        en = C.c_double()
+        ok = lib().rsfo_rk_attempt(C.byref(self.p), method, rtol,
+                                   np.ascontiguousarray(atol), h,
+                                   np.ascontiguousarray(X, np.float64),
+                                   np.ascontiguousarray(F0, np.float64), Xn,
+                                   Fn, C.byref(en))
+        return Xn, Fn, en.value, bool(ok)
+
+    def solve(self, X0, t_end, *, method=0, rtol=1e-8, atol=None, t0=0.0,
+              dt_init=1.0, dt_max=1e10, dt_monitor=5 * SEC_PER_YEAR,
+              rdx_monitor=1e-4, adx_monitor=0.0, vel_event=1e-3,
+              vel_event_hyst=0.5, rupture_vth=None, max_steps=10_000_000,
+              max_domain_reject=5000, max_events=4096, max_monitor=200000):
+        X = np.array(X0, np.float64, copy=True)
+        at = (C.c_double * 6)(*(list(atol) if atol is not None else
+                                [1e-12, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3][:self.dim]
+                                + [0.0] * (6 - self.dim)))
+        o = Options(method, rtol, at, t0, t_end, dt_init, dt_max, dt_monitor,
+                    rdx_monitor, adx_monitor, vel_event, vel_event_hyst,
+                    vel_event if rupture_vth is None else rupture_vth,
+                    max_steps, max_domain_reject, max_events, max_monitor)
+        ev = np.zeros(max_events * 8)
+        mon = np.zeros(max_monitor * 10)
+        r = Result()
+        r.ev = ev.ctypes.data
+        r.mon = mon.ctypes.data
+        lib().rsfo_solve(C.byref(self.p), C.byref(o), X, C.byref(r))
+        return dict(X=X, steps=r.steps, rejects=r.rejects,
+                    domain_rejects=r.domain_rejects, rhs_evals=r.rhs_evals,
+                    t_final=r.t_final, dt_final=r.dt_final, status=r.status,
+                    events=ev[:8 * r.nevent].reshape(-1, 8).copy(),
+                    monitor=mon[:10 * r.nmon].reshape(-1, 10).copy())
+
+
+def vel(tau, sigma, psi, a, v0):
+    return lib().rsfo_vel(tau, sigma, psi, a, v0)
