@@ -39,8 +39,16 @@ def write_patch_file(fname, g):
 
 def bp5_fields(g, a0=0.004, a_max=0.04, b0=0.03, dc0=0.14, dc_nuc=0.13,
                f0=0.6, v0=1e-6, vpl=1e-9, sigma=25e6, shear_modulus=3.204e10,
-               cs=3464.0, vnuc=3e-2):
-    """SEAS BP5-QD fields as written by make_bp5.py (ic convention 1)."""
+               cs=3464.0, vnuc=3e-2, prestress="make_bp5"):
+    """SEAS BP5-QD fields.
+
+    prestress="make_bp5": tau = sigma a asinh(.) + eta v, the committed
+    ic_bp5_*.in files (seas_bp_tests/bp5/make_bp5.py:95);
+    prestress="hbi": tau = sigma (a asinh(.) + eta_MPa v), the convention of
+    HBI's generator (hbi_tests/bp5r_param_gen.py:51-53), which the
+    reference's published BP5 recurrences (234.2937 yr 5dp and 234.2722 yr
+    3bs at 1 km, 236.81 yr at 2 km) were computed with.
+    """
     x, dep = g["along"], g["depth"]
     r = np.maximum(np.abs(dep - 10) - 6, np.abs(x) - 30) / 2.0
     a = np.clip(a0 + r * (a_max - a0), a0, a_max)
@@ -50,6 +58,9 @@ def bp5_fields(g, a0=0.004, a_max=0.04, b0=0.03, dc0=0.14, dc_nuc=0.13,
     dc = np.where(nuc, dc_nuc, dc0)
     eta = shear_modulus / (2.0 * cs)
     tau = sigma * a * np.arcsinh(0.5 * v / v0 * np.exp((f0 + b0 * np.log(v0 / vpl)) / a)) + eta * v
+    if prestress == "hbi":
+        # eta in MPa s/m multiplied by the normal stress in MPa
+        tau = tau + (sigma / 1e6 - 1.0) * eta * v
     return dict(a=a, b=b, dc=dc, tau=tau, vel=v, sigma=np.full_like(a, sigma),
                 f0=f0, v0=v0, vpl=vpl, eta=eta, shear_modulus=shear_modulus, cs=cs)
 

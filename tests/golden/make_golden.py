@@ -1,0 +1,166 @@
+"""Write the golden vectors of tests/golden from the reference's own code.
+
+Runs only in a container that has the reference sources (default
+/root/reference) and the tools of oracle/Makefile.ref built from them
+(oracle/_ref/ref_kgen, oracle/_ref/makefault).  Nothing here is needed at
+test time: the outputs are committed as data.
+
+    make -f oracle/Makefile.ref REF=/root/reference
+    python tests/golden/make_golden.py [--ref /root/reference]
+
+Outputs
+  geom_fvh.in.gz        makefault -n 40 -m 25 twice (second at x=1, group 1):
+                        the 2000-patch mesh of scaling_tests/full_vs_half_test.sh
+  ktensor_grid.npz      reference tractions T[i,j,slip,trac] (internal units)
+                        on a 12 x 5 planar grid with 1 km cells, half space
+  ktensor_random.npz    same on 40 random rectangles (strike, dip, size),
+                        half and full space
+  dc3d_cases.npz        direct DC3D calls: random points, points on edges,
+                        on negative edge extensions, at the surface, and
+                        dips 0 / 90 / 89.9999995 (cos snap) / 45
+  bp5_2km/              the reference's BP5 2 km input files (data)
+  ref_results/          the reference's published result files used as
+                        fixtures (events, slider physics, |K 1| answers)
+"""
+import argparse
+import gzip
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REFBIN = os.path.join(ROOT, "oracle", "_ref")
+
+
+def run(args, stdin=None):
+    return subprocess.run(args, input=stdin, capture_output=True, text=True,
+                          check=True).stdout
+
+
+def write_geom(fname, xyz, strike, dip, l, w, grp=None):
+    with open(fname, "w") as f:
+        for k in range(len(l)):
+            g = 0 if grp is None else grp[k]
+            f.write("%.12e %.12e %.12e %.6f %.6f %.12e %.12e %d\n" %
+                    (xyz[k][0], xyz[k][1], xyz[k][2], strike[k], dip[k], l[k],
+                     w[k], g))
+
+
+def kmat(geom_file, full):
+    n = sum(1 for line in open(geom_file) if line.strip())
+    with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as tf:
+        out = tf.name
+    try:
+        run([os.path.join(REFBIN, "ref_kgen"), "kmat", geom_file, str(full), out])
+        raw = np.fromfile(out, dtype=np.uint8)
+    finally:
+        os.unlink(out)
+    T = raw[:n * n * 48].view(np.float64).reshape(n, n, 2, 3)
+    ir = raw[n * n * 48:].view(np.int32).reshape(n, n, 2)
+    return T, ir
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    args = ap.parse_args()
+    ref = args.ref
+    if not os.path.exists(os.path.join(REFBIN, "ref_kgen")):
+        sys.exit("build oracle/_ref first: make -f oracle/Makefile.ref")
+    tmp = tempfile.mkdtemp()
+
+    # 1. the full_vs_half mesh (scaling_tests/full_vs_half_test.sh:85-86)
+    mf = os.path.join(REFBIN, "makefault")
+    txt = run([mf, "-n", "40", "-m", "25"]) + run([mf, "-n", "40", "-m", "25",
+                                                  "-x", "1", "-grp", "1"])
+    with gzip.open(os.path.join(HERE, "geom_fvh.in.gz"), "wt") as f:
+        f.write(txt)
+
+    # 2. planar grid, 12 x 5 cells of 1 km (BP5 layout)
+    nx, nz, ds = 12, 5, 1000.0
+    ii, jj = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    ii, jj = ii.ravel(), jj.ravel()
+    n = nx * nz
+    xyz = np.stack([np.zeros(n), (ii + 0.5 - nx / 2) * ds, -(jj + 0.5) * ds], 1)
+    gfile = os.path.join(tmp, "grid.in")
+    write_geom(gfile, xyz, np.zeros(n), np.full(n, 90.0), np.full(n, ds / 2),
+               np.full(n, ds / 2))
+    T, ir = kmat(gfile, 0)
+    np.savez_compressed(os.path.join(HERE, "ktensor_grid.npz"),
+                        geom=open(gfile).read(), T=T, iret=ir)
+
+    # 3. random rectangles (strike, dip, size), half and full space
+    rng = np.random.default_rng(20261202)
+    m = 40
+    xyz = np.stack([rng.uniform(-4e3, 4e3, m), rng.uniform(-4e3, 4e3, m),
+                    rng.uniform(-7e3, -1.5e3, m)], 1)
+    strike = np.round(rng.uniform(0, 360, m), 3)
+    dip = np.round(rng.uniform(5, 90, m), 3)
+    l = rng.uniform(150, 700, m)
+    w = rng.uniform(150, 500, m)
+    gfile = os.path.join(tmp, "rand.in")
+    write_geom(gfile, xyz, strike, dip, l, w)
+    Th, irh = kmat(gfile, 0)
+    Tf, irf = kmat(gfile, 1)
+    np.savez_compressed(os.path.join(HERE, "ktensor_random.npz"),
+                        geom=open(gfile).read(), T_half=Th, iret_half=irh,
+                        T_full=Tf, iret_full=irf)
+
+    # 4. direct DC3D calls incl. special cases
+    cases = []
+    alpha = 2.0 / 3.0
+    for k in range(200):   # generic points
+        cases.append([alpha, rng.uniform(-3, 3), rng.uniform(-3, 3),
+                      -rng.uniform(0, 4), rng.uniform(0.5, 4),
+                      rng.choice([0.0, 30.0, 45.0, 70.0, 90.0, 89.9999995]),
+                      -1.0, 1.0, -0.7, 0.7, 1.0, 0.0, 0.0, int(k % 3 == 0)])
+    for dip in (0.0, 45.0, 90.0, 89.9999995):
+        for d1, d2 in ((1.0, 0.0), (0.0, 1.0), (0.7, -0.3)):
+            cd, sd = np.cos(np.radians(dip)), np.sin(np.radians(dip))
+            depth = 2.0
+            for (x, ep) in ((1.0, 0.0), (-1.0, 0.3), (1.0, 0.7), (0.2, -0.7),
+                            (-1.5, 0.7), (-1.5, 0.0), (0.3, 1.2), (3.0, -0.7)):
+                # point in the fault plane at along-strike x, along-dip ep
+                y = ep * cd
+                d = depth - ep * sd
+                cases.append([alpha, x, y, -d, depth, dip, -1.0, 1.0, -0.7,
+                              0.7, d1, d2, 0.0, 0])
+            cases.append([alpha, 0.5, -0.4, 0.0, depth, dip, -1.0, 1.0, -0.7,
+                          0.7, d1, d2, 0.0, 0])   # free surface
+            cases.append([alpha, 0.3, 0.2, -1.0, 0.6, dip, -1.0, 1.0, -0.6,
+                          0.6, d1, d2, 0.0, 0])   # surface-breaking source
+    cases = np.array(cases, dtype=np.float64)
+    lines = "\n".join(" ".join("%.17g" % v for v in c) for c in cases) + "\n"
+    out = run([os.path.join(REFBIN, "ref_kgen"), "dc3d"], stdin=lines)
+    res = np.array([[float(v) for v in line.split()] for line in out.strip().split("\n")])
+    np.savez_compressed(os.path.join(HERE, "dc3d_cases.npz"), args=cases,
+                        u=res[:, :12], iret=res[:, 12].astype(np.int32))
+
+    # 5. reference data files (inputs and published results)
+    bp = os.path.join(HERE, "bp5_2km")
+    os.makedirs(bp, exist_ok=True)
+    src = os.path.join(ref, "seas_bp_tests", "bp5")
+    for f in ("geom_bp5_2km.in", "rsf_bp5_2km.dat", "ic_bp5_2km.in",
+              "dc_bp5_2km.in"):
+        shutil.copyfile(os.path.join(src, f), os.path.join(bp, f))
+    rr = os.path.join(HERE, "ref_results")
+    os.makedirs(rr, exist_ok=True)
+    shutil.copyfile(os.path.join(src, "ntest.0", "rsf_events.1km.0.inf.dat"),
+                    os.path.join(rr, "bp5_1km_dense_3bs_rtol1e-4_events.dat"))
+    shutil.copyfile(os.path.join(ref, "slider", "slider_physics.dat"),
+                    os.path.join(rr, "slider_physics.dat"))
+    shutil.copyfile(os.path.join(ref, "slider", "geom_slider10.in"),
+                    os.path.join(rr, "geom_slider10.in"))
+    shutil.copyfile(os.path.join(ref, "scaling_tests", "full_vs_half.csv"),
+                    os.path.join(rr, "full_vs_half.csv"))
+    shutil.rmtree(tmp)
+    print("golden vectors written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,288 @@
+"""GPU parity tests: librsfg kernels through the C ABI vs the oracle, the
+reference-built golden vectors and the reference's published numbers."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from helpers import (SEC_PER_YEAR, bp5_2km, first_recurrence, fvh_geometry,
+                     geom_from_text, initial_state, row_rel, strike_operator,
+                     vector_operator)
+
+pytestmark = pytest.mark.gpu
+
+G_BP5 = 3.204e10
+
+
+def _fs(g):
+    from interact_amd.model import FaultSystem
+    return FaultSystem(g)
+
+
+def _L():
+    from interact_amd import lib
+    return lib
+
+
+# ------------------------------------------------------------------ assembly
+def test_assembly_grid_all_modes():
+    """Every slip mode and the normal rows vs the reference's tractions on
+    the golden planar grid (row-relative 1e-13)."""
+    L = _L()
+    z = np.load(os.path.join(GOLDEN, "ktensor_grid.npz"))
+    g = geom_from_text(str(z["geom"]))
+    T = z["T"]
+    n = T.shape[0]
+    sc = G_BP5 / 1e4
+    fs = _fs(g)
+    fs.assemble(L.SLIP_STRIKE, shear_modulus=G_BP5)
+    assert row_rel(fs.dense(), strike_operator(T, sc)) < 1e-13
+    fs.assemble(L.SLIP_DIP, shear_modulus=G_BP5)
+    assert row_rel(fs.dense(), T[:, :, 1, 1] * sc) < 1e-13
+    fs.assemble(L.SLIP_VECTOR, shear_modulus=G_BP5)
+    assert row_rel(fs.dense(), vector_operator(T, sc)) < 1e-13
+    # normal rows: compression positive (-scale), checked through a product
+    fs.assemble(L.SLIP_STRIKE, shear_modulus=G_BP5, with_normal=True)
+    x = np.sin(np.arange(n) + 1.0)
+    y = fs.matvec(x)
+    np.testing.assert_allclose(y, (T[:, :, 0, 0] * sc) @ x, rtol=1e-12,
+                               atol=1e-12 * np.abs(y).max())
+    fs.close()
+
+
+def test_assembly_rake_mode():
+    L = _L()
+    z = np.load(os.path.join(GOLDEN, "ktensor_grid.npz"))
+    g = geom_from_text(str(z["geom"]))
+    T = z["T"]
+    n = T.shape[0]
+    rng = np.random.default_rng(4)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    g["rake"] = np.stack([np.cos(ang), np.sin(ang)], 1)
+    fs = _fs(g)
+    fs.assemble(L.SLIP_RAKE, shear_modulus=G_BP5)
+    r = g["rake"]
+    ref = np.einsum("ia,jb,ijba->ij", r, r, T[:, :, :, :2]) * G_BP5 / 1e4
+    assert row_rel(fs.dense(), ref) < 1e-13
+    fs.close()
+
+
+@pytest.mark.parametrize("full", [0, 1])
+def test_traction_pairs_random_rectangles(full):
+    """Random strikes, dips and sizes (golden, reference-built).  Near-edge
+    pairs are ill-conditioned: the reference itself moves by ~4e-10 of the
+    row scale between FMA and non-FMA builds, so the bound is 1e-9."""
+    z = np.load(os.path.join(GOLDEN, "ktensor_random.npz"))
+    g = geom_from_text(str(z["geom"]))
+    T0 = z["T_full"] if full else z["T_half"]
+    m = T0.shape[0]
+    fs = _fs(g)
+    r, s = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    T = fs.traction_pairs(r.ravel(), s.ravel(), full_space=bool(full)).reshape(m, m, 2, 3)
+    assert row_rel(T, T0) < 1e-9
+    big = np.abs(T0) > 1e-3 * np.abs(T0).max(axis=(1, 2, 3))[:, None, None, None]
+    assert (np.abs(T - T0)[big] / np.abs(T0[big])).max() < 1e-9
+    fs.close()
+
+
+def test_unit_slip_known_answers_gpu():
+    """|K 1|_2 = 1.0867985710e+06 (half) / 1.1916906189e+06 (full) on the
+    2000-patch makefault mesh (scaling_tests/full_vs_half.csv:2,5)."""
+    L = _L()
+    g = fvh_geometry()
+    fs = _fs(g)
+    for full, ref in ((0, "1.0867985710e+06"), (1, "1.1916906189e+06")):
+        fs.assemble(L.SLIP_STRIKE, full_space=bool(full), scale=1.0)
+        y = fs.matvec(np.ones(2000))
+        assert "%.10e" % np.linalg.norm(y) == ref
+    fs.close()
+
+
+# ------------------------------------------------------------------ products
+def test_matvec_precisions_and_transpose():
+    """2 km planar mesh, two slip components (2000 DOF): each storage
+    precision vs the FP64 product on x_i = sin(i+1) + 0.3 cos(3i)
+    (src/compress_interaction_matrix.c:68-77) and on a random vector."""
+    L = _L()
+    from interact_amd import faults
+    g = faults.planar_grid(2.0)
+    fs = _fs(g)
+    fs.assemble(L.SLIP_VECTOR, shear_modulus=32e9, store=15)
+    K = fs.dense()
+    D = K.shape[0]
+    i = np.arange(D)
+    bound = {L.PREC_FP64: 1e-13, L.PREC_FP32: 1e-6, L.PREC_BF16X2: 5e-5,
+             L.PREC_BF16: 1e-2}
+    for x in (np.sin(i + 1.0) + 0.3 * np.cos(3.0 * i),
+              np.random.default_rng(0).normal(size=D)):
+        yref = K @ x
+        for prec, b in bound.items():
+            y = fs.matvec(x, prec)
+            assert np.linalg.norm(y - yref) / np.linalg.norm(yref) < b, L.PREC_NAMES[prec]
+        yt = fs.matvec_t(x)
+        np.testing.assert_allclose(yt, K.T @ x, rtol=1e-12, atol=1e-12 * np.abs(yt).max())
+    fs.close()
+
+
+# ------------------------------------------------------------------ physics
+def _bp5(ds, prestress="hbi", store=1):
+    from interact_amd import faults, lib
+    from interact_amd.model import FaultSystem
+    g = faults.planar_grid(ds)
+    f = faults.bp5_fields(g, prestress=prestress)
+    fs = FaultSystem(g)
+    fs.assemble(lib.SLIP_STRIKE, shear_modulus=G_BP5, store=store)
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=1e-6, f0=0.6, eta=f["eta"])
+    fs.set_loading(1e-9)
+    X0 = initial_state(f["tau"], 25e6, f["vel"], f["a"], 1e-6)
+    return g, f, fs, X0
+
+
+def _oracle_problem(g, f):
+    K = strike_operator(oracle.traction_tensor(g, 0, 16), G_BP5 / 1e4)
+    return oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ np.full(len(f["a"]), -1e-9),
+                             eta=f["eta"], nthreads=16), K
+
+
+def test_rhs_matches_oracle():
+    g, f, fs, X0 = _bp5(2.0)
+    P, K = _oracle_problem(g, f)
+    rng = np.random.default_rng(1)
+    X = X0.reshape(-1, 4).copy()
+    X[:, 0] += rng.normal(0, 0.01, len(X))      # off steady state
+    X = X.ravel()
+    Fg, ok = fs.rhs(X)
+    Fo = P.rhs(X)
+    assert ok
+    Fg, Fo = Fg.reshape(-1, 4), Fo.reshape(-1, 4)
+    for k in range(4):
+        assert np.abs(Fg[:, k] - Fo[:, k]).max() <= 1e-10 * np.abs(Fo[:, k]).max() + 1e-300
+    bad = X.reshape(-1, 4).copy()
+    bad[7, 2] = -1.0
+    _, ok = fs.rhs(bad.ravel())
+    assert not ok
+    fs.close()
+
+
+def test_solver_seed_event_matches_oracle():
+    """BP5 2 km, rk5dp rtol 1e-6 over the seeded event: the device-driven
+    integrator takes the same steps and rejections as the oracle."""
+    g, f, fs, X0 = _bp5(2.0, prestress="make_bp5")
+    P, K = _oracle_problem(g, f)
+    fs.set_state(X0)
+    st = fs.solve(0.05 * SEC_PER_YEAR, rtol=1e-6)
+    ro = P.solve(X0, 0.05 * SEC_PER_YEAR, rtol=1e-6)
+    assert st["steps"] == ro["steps"] and st["rejects"] == ro["rejects"]
+    ev, oev = fs.events(), ro["events"]
+    assert len(ev) == len(oev)
+    np.testing.assert_allclose(ev[:, 0], oev[:, 0], rtol=1e-9)
+    X, t = fs.get_state()
+    assert abs(t - ro["t_final"]) < 1e-6 * t
+    assert np.abs(X - ro["X"]).max() / np.abs(ro["X"]).max() < 1e-8
+    fs.close()
+
+
+def test_solver_bp5_2km_recurrence():
+    """First spontaneous event 236.81 yr (seas_bp_tests/bp5/README.md:129)."""
+    g, f, fs, X0 = _bp5(2.0)
+    fs.set_state(X0)
+    st = fs.solve(238 * SEC_PER_YEAR, rtol=1e-4)
+    assert st["status"] == 0
+    assert abs(first_recurrence(fs.events()) - 236.81) < 0.005
+    fs.close()
+
+
+def test_solver_bp5_1km_recurrence_rk5dp():
+    """Dense 1 km, rk5dp rtol 1e-4: 234.2937 yr (rsf_solve.md:237)."""
+    g, f, fs, X0 = _bp5(1.0)
+    fs.set_state(X0)
+    st = fs.solve(234.5 * SEC_PER_YEAR, rtol=1e-4)
+    assert st["status"] == 0
+    assert abs(first_recurrence(fs.events()) - 234.2937) < 5e-4
+    fs.close()
+
+
+def test_solver_bp5_1km_rk3bs_matches_reference_file():
+    g, f, fs, X0 = _bp5(1.0)
+    fs.set_state(X0)
+    fs.solve(234.5 * SEC_PER_YEAR, rtol=1e-4, method=1)
+    ref = np.loadtxt(os.path.join(GOLDEN, "ref_results", "bp5_1km_dense_3bs_rtol1e-4_events.dat"))
+    ev = fs.events()
+    on = ev[(ev[:, 1] > 0) & (ev[:, 0] > SEC_PER_YEAR)][0]
+    assert abs(on[0] / SEC_PER_YEAR - ref[2, 1]) < 1e-3
+    assert abs(on[3] - ref[2, 4]) < 1e-2
+    fs.close()
+
+
+def test_solver_slider_physics():
+    """slider/slider_physics.dat (aging): 7 events, recurrence 66.68 yr,
+    drop 6.63 MPa, peak slip rate 1.502 m/s, duration 15.0 s."""
+    from interact_amd import lib
+    from interact_amd.model import FaultSystem
+    g = oracle.read_patches(os.path.join(GOLDEN, "ref_results", "geom_slider10.in"))
+    fs = FaultSystem(g)
+    fs.assemble(lib.SLIP_STRIKE, shear_modulus=G_BP5)
+    eta = G_BP5 / (2 * 3464.0)
+    fs.set_friction([0.01], [0.015], [0.008], eta=eta)
+    fs.set_loading(1e-9)
+    sig = 50e6
+    fs.set_state(initial_state(0.6 * sig + eta * 1e-9, sig, 1e-9, np.array([0.01]), 1e-6))
+    fs.solve(450 * SEC_PER_YEAR, rtol=1e-9, rdx_monitor=1e-5, rupture_vth=0.1,
+             steps_per_launch=256)
+    ev = fs.events()
+    rows, onset = [], None
+    for e in ev:
+        if e[1] > 0:
+            onset = e[0]
+        elif onset is not None and e[7] > 0:
+            rows.append((onset, e[0], e[5], e[6]))
+            onset = None
+    cat = np.array(rows)
+    assert len(cat) == 7
+    assert abs(np.diff(cat[:, 0] / SEC_PER_YEAR)[1:].mean() - 66.68) < 0.005
+    assert abs(cat[1:, 3].mean() / 1e6 - 6.63) < 0.005
+    assert abs(cat[1:, 2].mean() - 1.502) < 0.0005
+    assert abs((cat[1:, 1] - cat[1:, 0]).mean() - 15.0) < 0.05
+    fs.close()
+
+
+def test_mixed_precision_cycle_bp5_2km():
+    """FP32 and bf16 hi+lo storage with FP64 state keep the recurrence
+    within 0.05 yr of the FP64 path."""
+    from interact_amd import lib
+    g, f, fs, X0 = _bp5(2.0, store=lib.PREC_FP32 | lib.PREC_BF16X2)
+    t = {}
+    for prec in (lib.PREC_FP64, lib.PREC_FP32, lib.PREC_BF16X2):
+        fs.set_state(X0)
+        fs.solve(238 * SEC_PER_YEAR, rtol=1e-4, prec=prec)
+        t[prec] = first_recurrence(fs.events())
+    assert abs(t[lib.PREC_FP32] - t[lib.PREC_FP64]) < 0.05
+    assert abs(t[lib.PREC_BF16X2] - t[lib.PREC_FP64]) < 0.05
+    fs.close()
+
+
+# ------------------------------------------------------------------ full size
+def test_full_size_operator_properties():
+    """8000-DOF operator (1 km mesh, two slip components): along-strike
+    translation invariance of the uniform mesh, and FP32 storage = rounded
+    FP64 storage entry by entry."""
+    from interact_amd import faults, lib
+    g = faults.planar_grid(1.0)
+    fs = _fs(g)
+    fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9, store=lib.PREC_FP32)
+    nx, nz = g["nx"], g["nz"]
+    # rows of receiver (ix, jz) vs (ix + s, jz): identical when shifted
+    rng = np.random.default_rng(0)
+    for _ in range(8):
+        ix, jz, s = rng.integers(0, nx - 10), rng.integers(0, nz), rng.integers(1, 10)
+        r0, r1 = 2 * (ix * nz + jz), 2 * ((ix + s) * nz + jz)
+        cols0 = np.arange(2 * nz * (nx - s))
+        a = fs.entries([r0, r0 + 1], cols0)
+        b = fs.entries([r1, r1 + 1], cols0 + 2 * nz * s)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    x = np.ones(2 * len(g["l"]))
+    y64, y32 = fs.matvec(x, lib.PREC_FP64), fs.matvec(x, lib.PREC_FP32)
+    assert np.linalg.norm(y32 - y64) / np.linalg.norm(y64) < 1e-6
+    fs.close()

@@ -1,0 +1,51 @@
+"""CPU checks of the product boundary: librsfg.so loads and exports every
+entry point include/rsfg.h declares; the Python mirror matches the header."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "rsfg.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rsfg_[a-z_]+)\s*\(", txt)))
+
+
+def test_header_declarations_are_exported():
+    from interact_amd import lib
+    L = lib.load()
+    names = _declared()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+    assert sorted(lib.EXPORTS) == names
+
+
+def test_version_string():
+    from interact_amd import lib
+    assert b"gfx950" in lib.load().rsfg_version()
+
+
+def test_struct_layouts_match_header():
+    """ctypes mirrors of the C structs have the C sizes (checked against a
+    tiny C translation unit compiled from the header)."""
+    import subprocess
+    import tempfile
+    from interact_amd import lib
+    src = r'''
+#include <stdio.h>
+#include "rsfg.h"
+int main(void){printf("%zu %zu %zu %zu %zu\n", sizeof(rsfg_patch),
+ sizeof(rsfg_assembly), sizeof(rsfg_friction), sizeof(rsfg_solver_opts),
+ sizeof(rsfg_stats));return 0;}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "s.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "s")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        sizes = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
+    py = [ctypes.sizeof(t) for t in (lib.Patch, lib.Assembly, lib.Friction, lib.SolverOpts, lib.Stats)]
+    assert sizes == py

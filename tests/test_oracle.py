@@ -1,0 +1,306 @@
+"""CPU tests that pin the oracle (oracle/, FP64 C restatement).
+
+Each check uses a source independent of the restatement itself: golden
+vectors written by the reference's own compiled code (tests/golden), the
+reference's published numbers, closed forms, or scipy's Runge-Kutta pairs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from helpers import (SEC_PER_YEAR, bp5_2km, first_recurrence, fvh_geometry,
+                     geom_from_text, initial_state)
+
+
+# ------------------------------------------------------------ Green's functions
+def test_dc3d_matches_reference_build():
+    """DC3D restatement vs the reference's dc3d.F (compiled, no FMA)."""
+    z = np.load(os.path.join(GOLDEN, "dc3d_cases.npz"))
+    args, uref, iref = z["args"], z["u"], z["iret"]
+    assert (iref == 1).any() and (iref == 0).any()   # edges are exercised
+    for a, u0, i0 in zip(args, uref, iref):
+        assert a[12] == 0.0   # tensile opening is not part of the operator
+        u, ir = oracle.dc3d(*a[:12], full=int(a[13]))
+        assert ir == i0
+        scale = max(np.abs(u0).max(), 1e-300)
+        assert np.abs(u - u0).max() <= 4e-16 * scale, (a, u, u0)
+
+
+@pytest.mark.parametrize("which", ["grid", "random_half", "random_full"])
+def test_tractions_match_reference_build(which):
+    """GenKEntries-path tractions (eval_okada -> rotate -> resolve) vs the
+    reference's own objects on a planar grid and on random rectangles."""
+    if which == "grid":
+        z = np.load(os.path.join(GOLDEN, "ktensor_grid.npz"))
+        T0, full = z["T"], 0
+    else:
+        z = np.load(os.path.join(GOLDEN, "ktensor_random.npz"))
+        full = int(which.endswith("full"))
+        T0 = z["T_full"] if full else z["T_half"]
+    g = geom_from_text(str(z["geom"]))
+    T, ir = oracle.traction_tensor(g, full, 4, want_iret=True)
+    assert T.shape == T0.shape
+    np.testing.assert_array_equal(T, T0)
+
+
+def test_unit_slip_known_answers():
+    """|K 1|_2 of scaling_tests/full_vs_half.csv:2,5 on the makefault mesh
+    (strike slip -> strike traction, internal units, scale 1)."""
+    g = fvh_geometry()
+    assert len(g["l"]) == 2000
+    ref = {}
+    for line in open(os.path.join(GOLDEN, "ref_results", "full_vs_half.csv")):
+        p = line.strip().split(",")
+        if p[0] in ("half", "full") and p[1] == "htool":
+            ref[p[0]] = float(p[-1])
+    for mode, full in (("half", 0), ("full", 1)):
+        T = oracle.traction_tensor(g, full, 8)
+        b = T[:, :, 0, 0].sum(axis=1)
+        assert "%.10e" % np.linalg.norm(b) == "%.10e" % ref[mode]
+
+
+def test_frame_conventions():
+    """calc_quad_base_vecs: strike 0 dip 90 -> normal +x, strike +y, dip +z."""
+    fr = oracle.patch_frame(0.0, 90.0)
+    np.testing.assert_allclose(fr[2:5], [1, 0, 0], atol=1e-15)
+    np.testing.assert_allclose(fr[5:8], [0, 1, 0], atol=1e-15)
+    np.testing.assert_allclose(fr[8:11], [0, 0, 1], atol=1e-15)
+
+
+def test_self_stiffness_sign_and_toeplitz():
+    """Positive slip unloads the slipping patch (negative diagonal,
+    src/rsf_solve.c:224-231); a uniform planar mesh is translation invariant
+    along strike (K depends on the along-strike offset only)."""
+    z = np.load(os.path.join(GOLDEN, "ktensor_grid.npz"))
+    T = z["T"]
+    K = T[:, :, 0, 0]
+    assert (np.diag(K) < 0).all()
+    nx, nz = 12, 5
+    K4 = K.reshape(nx, nz, nx, nz)
+    for s in range(1, 4):
+        a = np.array([K4[i, :, i + s, :] for i in range(nx - s)])
+        assert np.abs(a - a[0]).max() <= 1e-12 * np.abs(K).max()
+
+
+# ------------------------------------------------------------ physics closed forms
+def _toy(n=6, ncomp=1, seed=3, eta=4.6e6, law=1):
+    rng = np.random.default_rng(seed)
+    m = ncomp * n
+    K = -np.eye(m) * 3e6 + rng.normal(0, 2e5, (m, m))
+    a = rng.uniform(0.008, 0.02, n)
+    b = rng.uniform(0.01, 0.02, n)
+    dc = rng.uniform(0.01, 0.05, n)
+    vpl = 1e-9
+    x = np.zeros(m)
+    x[::ncomp] = -vpl
+    sinc = K @ x
+    P = oracle.RSFProblem(K, a, b, dc, sinc, ncomp=ncomp, eta=eta, law=law,
+                          v0=1e-6, f0=0.6, nthreads=1)
+    return P, K, a, b, dc, sinc, vpl
+
+
+def test_steady_state_invariant():
+    """Uniform plate-rate steady state: F = (0, 0, 0, vpl) to rounding."""
+    P, K, a, b, dc, sinc, vpl = _toy()
+    sig = 30e6
+    psi = 0.6 + b * np.log(1e-6 / vpl)
+    tau = sig * a * np.arcsinh(vpl / 2e-6 * np.exp(psi / a))
+    X = np.stack([psi, tau, np.full_like(a, sig), np.zeros_like(a)], 1).ravel()
+    F = P.rhs(X).reshape(-1, 4)
+    assert np.abs(F[:, 0]).max() < 1e-22
+    assert np.abs(F[:, 1]).max() < 1e-8 * np.abs(K).max() * vpl
+    assert np.abs(F[:, 2]).max() == 0.0
+    np.testing.assert_allclose(F[:, 3], vpl, rtol=1e-12)
+
+
+@pytest.mark.parametrize("law", [1, 2, 3, 4, 5])
+def test_quasi_dynamic_balance(law):
+    """tau_dot solves tau_dot = K v + sinc - eta dv/dt with dv/dt along the
+    flow computed by finite differences of v(state) -- an independent check
+    of the implicit damping formula (src/rsf_engine.c:280-290)."""
+    P, K, a, b, dc, sinc, vpl = _toy(law=law)
+    rng = np.random.default_rng(5)
+    n = len(a)
+    sig = rng.uniform(20e6, 40e6, n)
+    v = 10 ** rng.uniform(-9, -4, n)
+    psi = 0.6 + b * np.log(1e-6 / v) + rng.normal(0, 0.01, n)
+    tau = sig * a * np.arcsinh(v / 2e-6 * np.exp(psi / a))
+    X = np.stack([psi, tau, sig, np.zeros(n)], 1).ravel()
+    F = P.rhs(X).reshape(-1, 4)
+    vel = lambda Y: np.array([oracle.vel(Y[4 * i + 1], Y[4 * i + 2], Y[4 * i], a[i], 1e-6) for i in range(n)])
+    h = 1e-3 * np.min(np.abs(X.reshape(-1, 4)[:, :2]) / (np.abs(F[:, :2]) + 1e-300))
+    cd = lambda hh: (vel(X + hh * F.ravel()) - vel(X - hh * F.ravel())) / (2 * hh)
+    dvdt = (4.0 * cd(h / 2) - cd(h)) / 3.0   # Richardson: O(h^4)
+    lhs = F[:, 1]
+    rhs = K @ vel(X) + sinc - P.p.eta * dvdt
+    assert np.abs(lhs - rhs).max() <= 1e-6 * np.abs(lhs).max()
+
+
+def test_aging_law_closed_form():
+    """At fixed v the aging law theta' = 1 - v theta / Dc has the solution
+    theta = Dc/v + (theta0 - Dc/v) exp(-v t / Dc); psi = f0 + b ln(v0 theta/Dc)."""
+    P, K, a, b, dc, sinc, vpl = _toy()
+    i, v, th0 = 2, 3e-7, 1e3
+    psi_of = lambda th: 0.6 + b[i] * np.log(1e-6 * th / dc[i])
+    for t in (0.0, 1e4, 5e4):
+        th = dc[i] / v + (th0 - dc[i] / v) * np.exp(-v * t / dc[i])
+        dth = -(v / dc[i]) * (th0 - dc[i] / v) * np.exp(-v * t / dc[i])
+        S, _, _ = P.state_rate(i, psi_of(th), v)
+        assert abs(S - b[i] * dth / th) <= 1e-12 * abs(b[i] / th)
+
+
+@pytest.mark.parametrize("law", [1, 2, 3, 4, 5])
+def test_state_law_derivatives(law):
+    P, *_ = _toy(law=law)
+    for psi, v in ((0.7, 1e-7), (0.55, 3e-3), (0.8, 1e-10)):
+        S, dpsi, dv = P.state_rate(1, psi, v)
+        e = 1e-7
+        fd_psi = (P.state_rate(1, psi + e, v)[0] - P.state_rate(1, psi - e, v)[0]) / (2 * e)
+        ev = v * 1e-6
+        fd_v = (P.state_rate(1, psi, v + ev)[0] - P.state_rate(1, psi, v - ev)[0]) / (2 * ev)
+        assert abs(dpsi - fd_psi) <= 1e-5 * (abs(dpsi) + abs(S) + 1e-30)
+        assert abs(dv - fd_v) <= 1e-5 * (abs(dv) + abs(S) / v + 1e-30)
+
+
+def test_vector_mode_reduces_to_scalar():
+    """Two slip components with zero cross coupling and zero dip traction
+    give exactly the scalar (reference) right-hand side."""
+    Ps, Ks, a, b, dc, sinc, vpl = _toy(n=5)
+    n = len(a)
+    Kv = np.zeros((2 * n, 2 * n))
+    Kv[0::2, 0::2] = Ks
+    Kv[1::2, 1::2] = Ks
+    sv = np.zeros(2 * n)
+    sv[0::2] = sinc
+    Pv = oracle.RSFProblem(Kv, a, b, dc, sv, ncomp=2, eta=Ps.p.eta, nthreads=1)
+    rng = np.random.default_rng(2)
+    sig = rng.uniform(20e6, 30e6, n)
+    v = 10 ** rng.uniform(-9, -3, n) * rng.choice([-1, 1], n)
+    psi = 0.6 + b * np.log(1e-6 / np.abs(v))
+    tau = sig * a * np.arcsinh(v / 2e-6 * np.exp(psi / a))
+    Xs = np.stack([psi, tau, sig, np.zeros(n)], 1)
+    Xv = np.stack([psi, tau, np.zeros(n), sig, np.zeros(n), np.zeros(n)], 1)
+    Fs = Ps.rhs(Xs.ravel()).reshape(-1, 4)
+    Fv = Pv.rhs(Xv.ravel()).reshape(-1, 6)
+    d = np.abs(Fv[:, [0, 1, 3, 4]] - Fs)
+    # psi_dot is a cancellation (b/Dc)(v0 e - |v|): scale it by its terms
+    scale = np.abs(Fs).copy()
+    scale[:, 0] = b / dc * np.abs(v)
+    scale[:, 1] = np.abs(Fs[:, 1]).max()
+    assert (d <= 1e-13 * scale).all()
+    assert np.abs(Fv[:, [2, 5]]).max() == 0.0
+
+
+def test_domain_check():
+    P, K, a, b, dc, sinc, vpl = _toy(n=3)
+    X = np.tile([0.7, 18e6, 30e6, 0.0], 3)
+    assert P.domain_ok(X)
+    for k, val in ((2, -1.0), (2, 0.0), (0, np.nan), (1, np.inf), (1, 1e12), (0, 50.0)):
+        Y = X.copy()
+        Y[k] = val
+        assert not P.domain_ok(Y), (k, val)
+
+
+# ------------------------------------------------------------ Runge-Kutta pairs
+@pytest.mark.parametrize("method,name", [(0, "RK45"), (1, "RK23")])
+def test_rk_pairs_match_scipy(method, name):
+    """One step of the oracle's rk5dp / rk3bs equals scipy's Dormand-Prince /
+    Bogacki-Shampine step; the error estimate equals h K^T E."""
+    from scipy.integrate import _ivp
+    cls = getattr(_ivp.rk, name)
+    P, *_ = _toy(n=4)
+    n = 4
+    X = np.tile([0.72, 17.5e6, 30e6, 0.0], n) * (1 + 1e-3 * np.arange(4 * n) / (4 * n))
+    F0 = P.rhs(X)
+    h = 3e5
+    Xn, Fn, en, ok = P.rk_attempt(X, F0, h, method=method, rtol=1e-6)
+    fun = lambda t, y: P.rhs(y)
+    Kst = np.zeros((cls.n_stages + 1, len(X)))
+    y_new, f_new = _ivp.rk.rk_step(fun, 0.0, X, F0, h, cls.A, cls.B, cls.C, Kst)
+    np.testing.assert_allclose(Xn, y_new, rtol=1e-13, atol=1e-9)
+    err = h * (Kst.T @ cls.E)
+    atol = np.tile([1e-12, 1e-3, 1e-3, 1e-3], n)
+    yemb = y_new - err
+    tol = atol + 1e-6 * np.maximum(np.abs(y_new), np.abs(yemb))
+    ref = np.max(np.abs(err) / tol)
+    # the reference (PETSc) forms the error as a difference of the two
+    # solutions, which carries rounding of order eps |y| / tol
+    slack = np.max(8 * np.finfo(float).eps * np.abs(y_new) / tol)
+    assert abs(en - ref) <= 1e-6 * ref + slack
+
+
+# ------------------------------------------------------------ whole-cycle pins
+def _slider_catalog(events):
+    """(onset, arrest, peak, drop) rows of events whose cells ruptured."""
+    rows, onset = [], None
+    for e in events:
+        if e[1] > 0:
+            onset = e[0]
+        elif onset is not None and e[7] > 0:
+            rows.append((onset, e[0], e[5], e[6]))
+            onset = None
+    return np.array(rows)
+
+
+def test_slider_physics_matches_reference():
+    """Single 10 km patch, lab parameters, aging law, rk5dp rtol 1e-9
+    (slider/run_slider:20-57) vs slider/slider_physics.dat."""
+    G, cs = 3.204e10, 3464.0
+    g = oracle.read_patches(os.path.join(GOLDEN, "ref_results", "geom_slider10.in"))
+    K = oracle.traction_tensor(g, 0, 1)[:, :, 0, 0] * G / 1e4
+    vpl, sig = 1e-9, 50e6
+    P = oracle.RSFProblem(K, [0.01], [0.015], [0.008], K @ [-vpl],
+                          eta=G / (2 * cs), nthreads=1)
+    X0 = initial_state(0.6 * sig + G / (2 * cs) * vpl, sig, vpl, np.array([0.01]), 1e-6)
+    r = P.solve(X0, 450 * SEC_PER_YEAR, method=0, rtol=1e-9, rdx_monitor=1e-5,
+                rupture_vth=0.1)
+    cat = _slider_catalog(r["events"])
+    ref = {l.split()[0]: [float(v) for v in l.split()[1:]]
+           for l in open(os.path.join(GOLDEN, "ref_results", "slider_physics.dat"))
+           if not l.startswith("#")}["aging"]
+    n_ev, recur, drop, peakv, dur = ref
+    assert len(cat) == n_ev
+    on = cat[:, 0] / SEC_PER_YEAR
+    assert abs(np.diff(on)[1:].mean() - recur) < 0.005
+    assert abs(cat[1:, 3].mean() / 1e6 - drop) < 0.005
+    assert abs(cat[1:, 2].mean() - peakv) < 0.0005
+    assert abs((cat[1:, 1] - cat[1:, 0]).mean() - dur) < 0.05
+
+
+def test_bp5_2km_recurrence_matches_reference():
+    """BP5-QD 2 km, dense, rk5dp rtol 1e-4: first spontaneous event at
+    236.81 yr (seas_bp_tests/bp5/README.md:128-129), with the HBI
+    prestress convention the published runs used."""
+    from interact_amd import faults
+    g, _ = bp5_2km()
+    f = faults.bp5_fields(faults.planar_grid(2.0), prestress="hbi")
+    K = oracle.traction_tensor(g, 0, 8)[:, :, 0, 0] * 3.204e10 / 1e4
+    P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ np.full(1000, -1e-9),
+                          eta=f["eta"], nthreads=8)
+    X0 = initial_state(f["tau"], 25e6, f["vel"], f["a"], 1e-6)
+    r = P.solve(X0, 238 * SEC_PER_YEAR, method=0, rtol=1e-4)
+    assert r["status"] == 0
+    assert abs(first_recurrence(r["events"]) - 236.81) < 0.005
+
+
+def test_bp5_1km_rk3bs_matches_reference_event_file():
+    """BP5-QD 1 km dense, rk3bs rtol 1e-4: the reference's own run
+    (seas_bp_tests/bp5/ntest.0/rsf_events.1km.0.inf.dat) puts the first
+    spontaneous onset at 234.2722431887 yr with mean slip 7.62304391 m."""
+    from interact_amd import faults
+    g = faults.planar_grid(1.0)
+    f = faults.bp5_fields(g, prestress="hbi")
+    K = oracle.traction_tensor(g, 0, 8)[:, :, 0, 0] * 3.204e10 / 1e4
+    P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ np.full(len(f["a"]), -1e-9),
+                          eta=f["eta"], nthreads=8)
+    X0 = initial_state(f["tau"], 25e6, f["vel"], f["a"], 1e-6)
+    r = P.solve(X0, 234.5 * SEC_PER_YEAR, method=1, rtol=1e-4)
+    ref = np.loadtxt(os.path.join(GOLDEN, "ref_results", "bp5_1km_dense_3bs_rtol1e-4_events.dat"))
+    ev = r["events"]
+    on = ev[(ev[:, 1] > 0) & (ev[:, 0] > SEC_PER_YEAR)][0]
+    assert abs(on[0] / SEC_PER_YEAR - ref[2, 1]) < 1e-4
+    assert abs(on[3] - ref[2, 4]) < 1e-3
+    assert abs(on[4] - ref[2, 5]) < 2e-6
