@@ -1,0 +1,219 @@
+"""Benchmark of the dense rsf_solve hot path on one MI355X.
+
+Workload (the 1 km planar configuration): vertical strike-slip fault
+100 x 40 km at 1 km patches = 4000 patches x 2 slip components = 8000 DOF,
+dense K of 8000 x 8000; aging-law rate-and-state friction with radiation
+damping; seeded initial slip rate vpl (1 + 1e-3 N(0,1)); rk5dp with
+relative tolerance 1e-8 over 400 yr.  One bench "step" is one pass of the
+whole cycle integration over that window.
+
+Reported: simulated years per wall-clock hour of the mixed-precision path
+(K stored in FP32, FP32 accumulation, FP64 state and error control) as
+`value`, the FP64 and bf16 hi+lo paths beside it, the error of each against
+the FP64 path at matched events (first-recurrence time, peak slip rate)
+and in K v, the roofline of the fused stage kernel (K v + RHS epilogue)
+measured with HIP events on its stream, operator assembly throughput, and
+the CPU oracle timed on the host cores on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--years 400]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SEC_PER_YEAR = 31557600.0
+HBM_PEAK_GBS = 8000.0
+METRIC = "sim-years/hour + K·v HBM GB/s, mixed-prec vs FP64 GPU at matched event times"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def first_onset(ev, t_min_yr=0.0):
+    on = ev[ev[:, 1] > 0, 0] / SEC_PER_YEAR
+    on = on[on > t_min_yr]
+    return float(on[0]) if len(on) else float("nan")
+
+
+def peak_after(ev, t_yr):
+    """peak |v| of the event whose onset is at t_yr (from its arrest row)."""
+    rows = ev[(ev[:, 0] / SEC_PER_YEAR >= t_yr - 1e-9) & (ev[:, 1] < 0)]
+    return float(rows[0, 5]) if len(rows) else float("nan")
+
+
+def cpu_baseline(g, f, K, rhs_evals_ref, years, budget_s, log_fn):
+    """Time the FP64 CPU oracle on the same workload (same operator values,
+    same state): dense K v + RHS evaluations for about budget_s seconds,
+    scaled by the RHS count of the GPU FP64 run to simulated years/hour;
+    assembly throughput timed on a 400-patch sample of the mesh."""
+    import oracle
+    from interact_amd.model import initial_state
+    threads = len(os.sched_getaffinity(0))
+    n = len(g["l"])
+    D = K.shape[0]
+    rng = np.random.default_rng(0)
+    sub = 400
+    idx = np.sort(rng.choice(n, sub, replace=False))
+    gs = {k: g[k][idx] for k in ("xyz", "strike", "dip", "l", "w")}
+    t0 = time.perf_counter()
+    oracle.traction_tensor(gs, 0, threads)
+    t_pairs = (time.perf_counter() - t0) / (sub * sub)
+    xpl = np.zeros(D)
+    xpl[0::2] = -f["vpl"]
+    P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ xpl, ncomp=2, eta=f["eta"],
+                          v0=f["v0"], f0=f["f0"], nthreads=threads)
+    X = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
+    P.rhs(X)
+    nev, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        P.rhs(X)
+        nev += 1
+    t_rhs = (time.perf_counter() - t0) / nev
+    proj_s = rhs_evals_ref * t_rhs
+    log_fn(f"cpu oracle: {nev} RHS in {budget_s:.0f}s -> {t_rhs*1e3:.2f} ms/RHS, "
+           f"{t_pairs*1e9:.0f} ns/pair on {threads} threads")
+    return dict(value=years / (proj_s / 3600.0), unit="sim-years/hour", cores=threads,
+                kind="port",
+                sample=(f"{nev} FP64 right-hand sides (dense {D}x{D} K v + RHS, OpenMP "
+                        f"over rows, {threads} threads) on this workload, {t_rhs*1e3:.2f} ms "
+                        f"each, scaled by the {rhs_evals_ref} RHS evaluations the FP64 GPU "
+                        f"run needed for {years:g} yr; assembly {t_pairs*1e9:.0f} ns/pair "
+                        f"timed on a {sub}x{sub} sample"),
+                rhs_ms=t_rhs * 1e3, assembly_pairs_per_s=1.0 / t_pairs)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--years", type=float, default=400.0)
+    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--ds", type=float, default=1.0)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precisions", default="fp64,fp32,bf16x2")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
+                          "designed but not built (see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from interact_amd import faults, lib
+    from interact_amd.model import FaultSystem, initial_state
+
+    g = faults.planar_grid(args.ds)
+    f = faults.cycle_fields(g)
+    n = len(g["l"])
+    D = 2 * n
+    store = lib.PREC_FP32 | lib.PREC_BF16X2 | lib.PREC_BF16
+    fs = FaultSystem(g)
+    asm_ms = fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"], store=store)
+    asm_ms = fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"], store=store)
+    pairs = n * n
+    log(f"assembly: {n} patches, {pairs} pairs in {asm_ms:.3f} ms = {pairs/asm_ms*1e3:.3e} pairs/s")
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=f["v0"], f0=f["f0"], eta=f["eta"])
+    fs.set_loading(f["vpl"])
+    X0 = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
+    fs.set_state(X0)
+
+    # K v accuracy against the FP64 operator on x_i = sin(i+1) + 0.3 cos(3i)
+    i = np.arange(D)
+    xa = np.sin(i + 1.0) + 0.3 * np.cos(3.0 * i)
+    y64 = fs.matvec(xa, lib.PREC_FP64)
+    kv = {}
+    for name in ("fp64", "fp32", "bf16x2", "bf16"):
+        p = lib.PREC_BY_NAME[name]
+        y = fs.matvec(xa, p)
+        mv_ms = fs.matvec_bench(p, 50)
+        st_ms = fs.stage_bench(p, 50)
+        bytes_alg = D * D * lib.PREC_BYTES[p] + 16.0 * D
+        kv[name] = dict(relerr=float(np.linalg.norm(y - y64) / np.linalg.norm(y64)),
+                        matvec_ms=mv_ms, stage_ms=st_ms,
+                        matvec_gbs=bytes_alg / mv_ms / 1e6, stage_gbs=bytes_alg / st_ms / 1e6,
+                        bytes=bytes_alg)
+        log(f"K.v {name}: relerr {kv[name]['relerr']:.2e}  matvec {mv_ms*1e3:.1f} us "
+            f"{kv[name]['matvec_gbs']:.0f} GB/s  stage {st_ms*1e3:.1f} us {kv[name]['stage_gbs']:.0f} GB/s")
+
+    t_end = args.years * SEC_PER_YEAR
+    runs = {}
+    for name in args.precisions.split(","):
+        p = lib.PREC_BY_NAME[name]
+        headline = (name == "fp32")
+        nrep = args.steps if headline else 1
+        if headline:
+            for _ in range(args.warmup):
+                fs.set_state(X0)
+                fs.solve(t_end, rtol=args.rtol, prec=p, steps_per_launch=64)
+        walls = []
+        for _ in range(nrep):
+            fs.set_state(X0)
+            t0 = time.perf_counter()
+            st = fs.solve(t_end, rtol=args.rtol, prec=p, steps_per_launch=64)
+            walls.append(time.perf_counter() - t0)
+        ev = fs.events()
+        t1 = first_onset(ev, 1.0)
+        runs[name] = dict(wall_s=float(np.mean(walls)), steps=st["steps"], rejects=st["rejects"],
+                          rhs_evals=st["rhs_evals"], events=int(len(ev)),
+                          first_recurrence_yr=t1, peak_slip_rate=peak_after(ev, t1),
+                          sim_years_per_hour=args.years / (np.mean(walls) / 3600.0))
+        log(f"cycle {name}: {args.years:g} yr in {np.mean(walls):.2f} s "
+            f"({runs[name]['sim_years_per_hour']:.3e} sim-yr/h), steps {st['steps']} "
+            f"rejects {st['rejects']}, first event {t1:.4f} yr, peak v {runs[name]['peak_slip_rate']:.4f}")
+    ref = runs.get("fp64")
+    errors = {}
+    for name, r in runs.items():
+        e = dict(kv_relerr=kv[name]["relerr"])
+        if ref is not None:
+            e["first_recurrence_dt_yr"] = r["first_recurrence_yr"] - ref["first_recurrence_yr"]
+            e["peak_slip_rate_rel"] = (r["peak_slip_rate"] - ref["peak_slip_rate"]) / ref["peak_slip_rate"]
+        errors[name] = e
+
+    head = runs.get("fp32") or next(iter(runs.values()))
+    stg = kv["fp32"]
+    roof = dict(kernel="k_stage<float,2,2,4> (fused K v + RHS epilogue)", bound="hbm",
+                achieved=stg["stage_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=stg["stage_gbs"] / HBM_PEAK_GBS, traffic=None,
+                bytes_per_launch=stg["bytes"], launch_us=stg["stage_ms"] * 1e3)
+    out = {
+        "metric": METRIC,
+        "value": head["sim_years_per_hour"],
+        "unit": "sim-years/hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 K.v (f32 acc) + f64 state",
+        "data": "synthetic (seeded fault and friction fields)",
+        "config": {"workload": f"planar strike-slip fault 100x40 km, {args.ds:g} km patches, "
+                               f"{n} patches x 2 slip components = {D} DOF, dense K {D}x{D}, "
+                               f"{args.years:g} yr, rk5dp rtol {args.rtol:g}",
+                   "parallelism": "1 GPU"},
+        "speedup_mixed_vs_fp64": (head["sim_years_per_hour"] / ref["sim_years_per_hour"]) if ref else None,
+        "runs": runs,
+        "kv": kv,
+        "errors_vs_fp64": errors,
+        "assembly": dict(pairs=pairs, ms=asm_ms, pairs_per_s=pairs / asm_ms * 1e3),
+        "roofline": roof,
+    }
+    if not args.no_cpu and ref is not None:
+        K64 = fs.dense()
+        out["cpu_baseline"] = cpu_baseline(g, f, K64, ref["rhs_evals"], args.years,
+                                           args.cpu_budget, log)
+    fs.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

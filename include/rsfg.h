@@ -210,6 +210,13 @@ typedef struct {
 int rsfg_solve(rsfg_handle *h, const rsfg_solver_opts *o, double t_end,
                rsfg_stats *st);
 
+/* times nrep back-to-back launches of the fused stage kernel (K v with the
+   RHS epilogue, as used inside rsfg_solve) at the current state with the
+   operator of precision prec; *ms is the mean kernel duration measured with
+   HIP events on the handle's stream.  No reference counterpart: the
+   reference's per-stage cost is MatMult + rsf_ODE_RHSFunction. */
+int rsfg_stage_bench(rsfg_handle *h, int prec, int nrep, double *ms);
+
 /* event rows (rsf_events.dat + catalog, src/rsf_output.c:1000-1150):
    8 doubles each: t [s], +1 onset / -1 arrest, log10 max|v|, mean slip,
    mean mu, peak |v| since onset, mean stress drop over ruptured cells

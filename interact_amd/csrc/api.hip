@@ -31,6 +31,7 @@ int launch_assemble(rsfg_ctx* h, double* gpu_ms);
 int launch_pairs(rsfg_ctx* h, long np, const int* d_recv, const int* d_src, int full,
                  double* d_out);
 int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st);
+int stage_bench(rsfg_ctx* h, int prec, int nrep, double* ms);
 
 constexpr double kDeg2Rad = 0.0174532925199432957692369076848861271344287188854172545609719144;
 constexpr double kOkEps = 1.0e-8;
@@ -821,6 +822,19 @@ int rsfg_solve(rsfg_handle* h, const rsfg_solver_opts* o, double t_end, rsfg_sta
     h->xcur = 0;
   }
   return solve_device(h, o, t_end, st);
+}
+
+int rsfg_stage_bench(rsfg_handle* h, int prec, int nrep, double* ms) {
+  RSFG_CHECK(h && nrep > 0 && ms, RSFG_EINVAL, "rsfg_stage_bench: bad args");
+  RSFG_CHECK(h->have_state && h->have_loading, RSFG_ESTATE,
+             "rsfg_stage_bench: set friction, loading and state first");
+  RSFG_HIP(hipSetDevice(h->device));
+  if (h->xcur != 0) {
+    RSFG_HIP(hipMemcpyAsync(h->Xb[0], h->Xb[h->xcur], sizeof(double) * h->dim * h->n,
+                            hipMemcpyDeviceToDevice, h->stream));
+    h->xcur = 0;
+  }
+  return stage_bench(h, prec, nrep, ms);
 }
 
 int rsfg_events(rsfg_handle* h, double* rows, int* n) {

@@ -683,21 +683,8 @@ static Launcher* make_launcher(int prec, int ncomp, int G) {
   }
 }
 
-int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st) {
-  const auto wall0 = std::chrono::steady_clock::now();
-  const void* K = operator_ptr(h, o->prec);
-  RSFG_CHECK(K != nullptr, RSFG_ESTATE, "rsfg_solve: operator precision not assembled");
-  RSFG_CHECK(o->method == RSFG_RK5DP || o->method == RSFG_RK3BS, RSFG_EINVAL,
-             "rsfg_solve: unknown method");
+static void fill_args(rsfg_ctx* h, const rsfg_solver_opts* o, const void* K, StageArgs& a) {
   const int n = h->n;
-  Launcher* L = make_launcher(o->prec, h->ncomp, h->G);
-  const int nwg = L->wg_count(n);
-  if ((size_t)nwg * kNRed > h->partial_count) {
-    delete L;
-    set_error("rsfg_solve: partial buffer too small");
-    return RSFG_ESTATE;
-  }
-  StageArgs a;
   memset(&a, 0, sizeof(a));
   a.K = K;
   a.ld = h->ld;
@@ -731,7 +718,12 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
   a.so.hyst = o->vel_event_hyst;
   a.so.rupture_vth = o->rupture_vth;
   a.so.max_domain_reject = o->max_domain_reject;
-  // controller state for this solve
+}
+
+// controller state for a solve from the handle's current (t, X); then K_0
+static int start_solve(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, StageArgs& a,
+                       Launcher* L) {
+  const int n = h->n;
   DevCtl* c = h->h_ctl;
   memset(c, 0, sizeof(*c));
   c->t = h->t;
@@ -756,9 +748,33 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
   RSFG_HIP(hipMemcpyAsync(c, h->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->stream));
   RSFG_HIP(hipStreamSynchronize(h->stream));
   if (c->status == RSFG_EDOMAIN) {
-    delete L;
     set_error("rsfg_solve: initial state fails the domain check");
     return RSFG_EDOMAIN;
+  }
+  return RSFG_OK;
+}
+
+int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st) {
+  const auto wall0 = std::chrono::steady_clock::now();
+  const void* K = operator_ptr(h, o->prec);
+  RSFG_CHECK(K != nullptr, RSFG_ESTATE, "rsfg_solve: operator precision not assembled");
+  RSFG_CHECK(o->method == RSFG_RK5DP || o->method == RSFG_RK3BS, RSFG_EINVAL,
+             "rsfg_solve: unknown method");
+  Launcher* L = make_launcher(o->prec, h->ncomp, h->G);
+  if ((size_t)L->wg_count(h->n) * kNRed > h->partial_count) {
+    delete L;
+    set_error("rsfg_solve: partial buffer too small");
+    return RSFG_ESTATE;
+  }
+  StageArgs a;
+  fill_args(h, o, K, a);
+  DevCtl* c = h->h_ctl;
+  {
+    const int rc0 = start_solve(h, o, t_end, a, L);
+    if (rc0 != RSFG_OK) {
+      delete L;
+      return rc0;
+    }
   }
   // capture one graph of step attempts
   const int S = a.tb.s;
@@ -814,6 +830,42 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
     st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
   }
   return RSFG_OK;
+}
+
+// Mean duration of the fused stage kernel (stage 1 of rk5dp: K v(Y_1), the
+// RHS epilogue and Y_2) at the current state.  Repeated launches read the
+// same inputs and rewrite the same outputs, so every launch does identical
+// work; the state of the handle is left as it was.
+int stage_bench(rsfg_ctx* h, int prec, int nrep, double* ms) {
+  const void* K = operator_ptr(h, prec);
+  RSFG_CHECK(K != nullptr, RSFG_ESTATE, "rsfg_stage_bench: precision not assembled");
+  rsfg_solver_opts o;
+  rsfg_solver_defaults(&o);
+  o.prec = prec;
+  Launcher* L = make_launcher(prec, h->ncomp, h->G);
+  StageArgs a;
+  fill_args(h, &o, K, a);
+  int rc = start_solve(h, &o, h->t + 1e30, a, L);
+  if (rc == RSFG_OK) {
+    StageArgs b = a;
+    b.s = 0;
+    b.vout = h->vin[1];
+    L->prep(b, h->stream);
+    b.s = 1;
+    b.vin = h->vin[1];
+    b.vout = h->vin[0];
+    L->stage(b, h->stream);  // warm
+    hipEventRecord(h->ev0, h->stream);
+    for (int i = 0; i < nrep; ++i) L->stage(b, h->stream);
+    hipEventRecord(h->ev1, h->stream);
+    hipError_t e = hipEventSynchronize(h->ev1);
+    float t = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, h->ev0, h->ev1);
+    if (e != hipSuccess) rc = hip_fail(e, "rsfg_stage_bench", __FILE__, __LINE__);
+    *ms = t / nrep;
+  }
+  delete L;
+  return rc;
 }
 
 }  // namespace rsfg

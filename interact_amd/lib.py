@@ -32,7 +32,7 @@ EXPORTS = [
     "rsfg_matvec", "rsfg_matvec_dev", "rsfg_matvec_t", "rsfg_matvec_bench",
     "rsfg_set_friction", "rsfg_set_loading", "rsfg_rhs",
     "rsfg_solver_defaults", "rsfg_set_state", "rsfg_get_state", "rsfg_solve",
-    "rsfg_events", "rsfg_monitor",
+    "rsfg_events", "rsfg_monitor", "rsfg_stage_bench",
 ]
 
 
@@ -126,6 +126,7 @@ def load(path=None):
                              C.POINTER(Stats)]
     L.rsfg_events.argtypes = [h, _dp, C.POINTER(C.c_int)]
     L.rsfg_monitor.argtypes = [h, _dp, C.POINTER(C.c_int)]
+    L.rsfg_stage_bench.argtypes = [h, C.c_int, C.c_int, C.POINTER(C.c_double)]
     for name in EXPORTS:
         if name not in ("rsfg_last_error", "rsfg_version"):
             getattr(L, name).restype = C.c_int

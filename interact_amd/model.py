@@ -188,6 +188,12 @@ class FaultSystem:
                 "rsfg_solve")
         return {k: getattr(st, k) for k, _ in L.Stats._fields_}
 
+    def stage_bench(self, prec=L.PREC_FP64, nrep=50):
+        ms = C.c_double()
+        L.check(self._lib.rsfg_stage_bench(self.h, prec, nrep, C.byref(ms)),
+                "rsfg_stage_bench")
+        return ms.value
+
     def events(self, cap=4096):
         rows = np.zeros(8 * cap)
         n = C.c_int(cap)
