@@ -649,7 +649,8 @@ int rsfg_set_friction(rsfg_handle* h, const rsfg_friction* f, const double* a, c
   memcpy(c.data() + 2 * n, dc, sizeof(double) * n);
   if (!h->d_coef) RSFG_HIP(hipMalloc(&h->d_coef, sizeof(double) * 3 * n));
   RSFG_HIP(hipMemcpy(h->d_coef, c.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-  if (!h->d_sinc) RSFG_HIP(hipMalloc(&h->d_sinc, sizeof(double) * 3 * n));
+  // loading block: 2 (ncomp + 1) n rates + n plate rates, ncomp <= 2
+  if (!h->d_sinc) RSFG_HIP(hipMalloc(&h->d_sinc, sizeof(double) * 7 * n));
   h->have_friction = true;
   return alloc_state(h);
 }
@@ -679,15 +680,23 @@ int rsfg_set_loading(rsfg_handle* h, double vpl, const double* vpl_patch, const 
   hipFree(dy);
   if (rc) return rc;
   if (e != hipSuccess) return hip_fail(e, "rsfg_set_loading", __FILE__, __LINE__);
-  std::vector<double> s(3 * (size_t)n, 0.0);
+  // [full shear | full normal] = backslip products + additive rates;
+  // [additive shear | additive normal] and the plate rate per patch serve
+  // the reduced-precision paths, which fold the backslip into K (v - vpl)
+  const size_t nl = (size_t)(nc + 1) * n;
+  std::vector<double> s(2 * nl + n, 0.0);
   for (int p = 0; p < n; ++p) {
+    const double tadd = taudot_add ? taudot_add[p] : 0.0;
+    const double sadd = (sigdot_add && h->fp.calc_sigma_dot) ? sigdot_add[p] : 0.0;
     for (int k = 0; k < nc; ++k) s[(size_t)nc * p + k] = y[(size_t)p * h->G + k];
-    if (taudot_add) s[(size_t)nc * p] += taudot_add[p];
+    s[(size_t)nc * p] += tadd;
+    s[nl + (size_t)nc * p] = tadd;
     const double sn = (h->G > nc) ? y[(size_t)p * h->G + nc] : 0.0;
-    s[(size_t)nc * n + p] = sn + ((sigdot_add && h->fp.calc_sigma_dot) ? sigdot_add[p] : 0.0);
+    s[(size_t)nc * n + p] = sn + sadd;
+    s[nl + (size_t)nc * n + p] = sadd;
+    s[2 * nl + p] = vpl_patch ? vpl_patch[p] : vpl;
   }
-  RSFG_HIP(hipMemcpy(h->d_sinc, s.data(), sizeof(double) * (nc + 1) * (size_t)n,
-                     hipMemcpyHostToDevice));
+  RSFG_HIP(hipMemcpy(h->d_sinc, s.data(), sizeof(double) * s.size(), hipMemcpyHostToDevice));
   h->have_loading = true;
   return RSFG_OK;
 }

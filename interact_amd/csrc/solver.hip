@@ -40,7 +40,8 @@ struct StageArgs {
   long ld;
   int rows_all, cols, n, G;
   const double* coef;  // a, b, dc
-  const double* sinc;  // ncomp n shear rows, then n normal
+  const double* sinc;  // loading: ncomp n shear rows, then n normal
+  const double* vpl;   // plate rate per patch when folded into K v, else null
   FricParams fp;
   double* Xb[2];
   double* Yb[2];
@@ -80,11 +81,18 @@ __device__ __forceinline__ double vel_mag(const FricParams& fp, double a, const 
   return (NC == 1) ? fabs(v[0]) : sqrt(v[0] * v[0] + v[1] * v[1]);
 }
 
+// product input of patch p: its slip rate, minus the plate rate along the
+// slip direction when the backslip loading is folded into the product
+// (vpl != nullptr, reduced-precision operators: K (v - vpl) instead of
+// K v + K (-vpl), so the storage rounding scales with the deviation from
+// plate rate instead of acting as a spurious loading)
 template <typename VT, int NC>
 __device__ __forceinline__ void write_vel(const FricParams& fp, double a,
-                                          const double* s, VT* vout, int p) {
+                                          const double* s, VT* vout, int p,
+                                          const double* vpl) {
   double v[2];
   rsf_patch_vel<NC>(fp, a, s, v);
+  if (vpl) v[0] -= vpl[p];
 #pragma unroll
   for (int c = 0; c < NC; ++c) vout[NC * p + c] = static_cast<VT>(v[c]);
 }
@@ -227,6 +235,20 @@ __device__ void controller(const StageArgs& a, const double* red) {
       if (!acc) {
         c->rejects++;
         c->attempt_failed = 1;
+      }
+    }
+    // rejection-storm guard: consecutive attempts that fail the domain
+    // check (-domain_check_max_reject); step-size underflow ends the solve
+    if (c->stage_fail) c->storm++;
+    else c->storm = 0;
+    if (!acc) {
+      if (o.max_domain_reject > 0 && c->storm >= o.max_domain_reject) {
+        c->status = RSFG_ESTORM;
+        c->halted = 1;
+      }
+      if (!(hnew > 0.0) || !(c->t + hnew != c->t) || !isfinite(c->t)) {
+        c->status = RSFG_EDOMAIN;
+        c->halted = 1;
       }
     }
     s_int[0] = acc;
@@ -421,7 +443,7 @@ __global__ __launch_bounds__(256) void k_prep(StageArgs a) {
     y[f] = x[f] + h * (w * K0[f * n + p]);
     Y[f * n + p] = y[f];
   }
-  write_vel<VT, NC>(a.fp, cf.a, y, static_cast<VT*>(a.vout), p);
+  write_vel<VT, NC>(a.fp, cf.a, y, static_cast<VT*>(a.vout), p, a.vpl);
   if (!rsf_patch_ok<NC>(a.fp, cf, y)) atomicOr(&a.ctl->stage_fail, 1);
 }
 
@@ -437,7 +459,7 @@ __global__ __launch_bounds__(256) void k_init_vel(StageArgs a) {
   double x[D];
 #pragma unroll
   for (int f = 0; f < D; ++f) x[f] = X[f * n + p];
-  write_vel<VT, NC>(a.fp, cf.a, x, static_cast<VT*>(a.vout), p);
+  write_vel<VT, NC>(a.fp, cf.a, x, static_cast<VT*>(a.vout), p, a.vpl);
   if (!rsf_patch_ok<NC>(a.fp, cf, x)) atomicOr(&a.ctl->stage_fail, 1);
 }
 
@@ -508,7 +530,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
           yn[f] = X[f * n + p] + h * acc[f];
           Yn[f * n + p] = yn[f];
         }
-        write_vel<VT, NC>(a.fp, cf.a, yn, static_cast<VT*>(a.vout), p);
+        write_vel<VT, NC>(a.fp, cf.a, yn, static_cast<VT*>(a.vout), p, a.vpl);
         if (!rsf_patch_ok<NC>(a.fp, cf, yn)) atomicOr(&c->stage_fail, 1);
       } else {
         // reductions of the state y (initial state or the new solution)
@@ -693,7 +715,11 @@ static void fill_args(rsfg_ctx* h, const rsfg_solver_opts* o, const void* K, Sta
   a.n = n;
   a.G = h->G;
   a.coef = h->d_coef;
-  a.sinc = h->d_sinc;
+  // d_sinc: [full shear | full normal | additive shear | additive normal | vpl]
+  const size_t nl = (size_t)(h->ncomp + 1) * n;
+  const bool fold = (o->prec != RSFG_PREC_FP64);
+  a.sinc = fold ? h->d_sinc + nl : h->d_sinc;
+  a.vpl = fold ? h->d_sinc + 2 * nl : nullptr;
   a.fp = h->fp;
   for (int i = 0; i < 2; ++i) {
     a.Xb[i] = h->Xb[i];
@@ -736,6 +762,9 @@ static int start_solve(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, Sta
   c->max_mon = kMaxMon;
   c->xcur = h->xcur;
   RSFG_HIP(hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, h->stream));
+  // the GEMV reads the input vector up to a whole chunk past the last
+  // column: its padding must be zero in this precision's element type
+  RSFG_HIP(hipMemsetAsync(h->vin[0], 0, 2 * h->vin_elems * sizeof(double), h->stream));
   RSFG_HIP(hipMemsetAsync(h->d_rupt, 0, sizeof(int) * n, h->stream));
   // K_0 at the initial state
   a.s = 0;
@@ -825,6 +854,16 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
     st->t = c->t;
     st->dt = c->h;
     st->status = c->status == 2 ? RSFG_OK : c->status;
+  }
+  if (c->status == RSFG_ESTORM) {
+    set_error("rsfg_solve: rejection storm (consecutive out-of-domain trial steps)");
+    return RSFG_ESTORM;
+  }
+  if (c->status == RSFG_EDOMAIN) {
+    set_error("rsfg_solve: step size underflow or non-finite time");
+    return RSFG_EDOMAIN;
+  }
+  if (st) {
     st->nevent = c->nevent;
     st->nmonitor = c->nmon;
     st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();

@@ -322,3 +322,18 @@ def test_solver_vector_mode_matches_oracle():
         assert np.abs(X[:, k] - Xo[:, k]).max() <= 1e-8 * np.abs(Xo[:, k]).max() + 1e-30
     assert np.abs(Xo[:, 5]).max() > 0.0    # dip slip is excited by K_ds
     fs.close()
+
+
+def test_solver_rejection_storm_guard():
+    """A first trial step far outside the domain with the guard at 1 ends
+    the solve with ESTORM instead of retrying (host-forced condition)."""
+    from interact_amd import lib
+    g, f, fs, X0 = _bp5(2.0)
+    fs.set_state(X0)
+    with pytest.raises(lib.RsfgError, match="ESTORM"):
+        fs.solve(10 * SEC_PER_YEAR, rtol=1e-4, dt_init=1e9, max_domain_reject=1)
+    # the handle stays usable
+    fs.set_state(X0)
+    st = fs.solve(0.001 * SEC_PER_YEAR, rtol=1e-4)
+    assert st["status"] == 0 and st["steps"] > 0
+    fs.close()
