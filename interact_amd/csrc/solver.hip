@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -805,40 +806,54 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
       return rc0;
     }
   }
-  // capture one graph of step attempts
+  // one batch = `per` step attempts; captured once into a graph and
+  // replayed (RSFG_NO_GRAPH=1 launches the same kernels directly, e.g.
+  // under tools that do not follow graph launches)
   const int S = a.tb.s;
   const int per = o->steps_per_launch > 0 ? o->steps_per_launch : 32;
+  const char* ng = getenv("RSFG_NO_GRAPH");
+  const bool use_graph = !(ng && ng[0] == '1');
+  auto enqueue_batch = [&]() {
+    for (int k = 0; k < per; ++k) {
+      StageArgs b = a;
+      b.s = 0;
+      b.vout = h->vin[1];
+      L->prep(b, h->stream);
+      for (int s = 1; s < S; ++s) {
+        b.s = s;
+        b.vin = h->vin[s & 1];
+        b.vout = (s < S - 1) ? h->vin[(s + 1) & 1] : nullptr;
+        L->stage(b, h->stream);
+      }
+    }
+  };
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  RSFG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < per; ++k) {
-    StageArgs b = a;
-    b.s = 0;
-    b.vout = h->vin[1];
-    L->prep(b, h->stream);
-    for (int s = 1; s < S; ++s) {
-      b.s = s;
-      b.vin = h->vin[s & 1];
-      b.vout = (s < S - 1) ? h->vin[(s + 1) & 1] : nullptr;
-      L->stage(b, h->stream);
-    }
+  if (use_graph) {
+    RSFG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_batch();
+    RSFG_HIP(hipStreamEndCapture(h->stream, &graph));
+    RSFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
-  RSFG_HIP(hipStreamEndCapture(h->stream, &graph));
-  RSFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   long launches = 0;
   int rc = RSFG_OK;
   while (!c->halted) {
-    hipError_t e = hipGraphLaunch(exec, h->stream);
+    hipError_t e = hipSuccess;
+    if (use_graph) e = hipGraphLaunch(exec, h->stream);
+    else {
+      enqueue_batch();
+      e = hipGetLastError();
+    }
     if (e == hipSuccess) e = hipMemcpyAsync(c, h->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) {
-      rc = hip_fail(e, "rsfg_solve graph launch", __FILE__, __LINE__);
+      rc = hip_fail(e, "rsfg_solve batch launch", __FILE__, __LINE__);
       break;
     }
     ++launches;
   }
-  hipGraphExecDestroy(exec);
-  hipGraphDestroy(graph);
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
   delete L;
   if (rc != RSFG_OK) return rc;
   h->xcur = c->xcur;
