@@ -118,16 +118,20 @@ __global__ void k_to_vt(const double* __restrict__ x, VT* __restrict__ y, int n,
   if (i < npad) y[i] = (i < n) ? static_cast<VT>(x[i]) : VT(0);
 }
 
+// R rows per wave, four waves per workgroup
 template <typename TK, int R>
 __global__ __launch_bounds__(256) void k_matvec(const TK* __restrict__ K, long ld, int rows,
                                                 int cols,
                                                 const typename KTraits<TK>::VT* __restrict__ x,
                                                 double* __restrict__ y) {
-  __shared__ double rowsum[R];
-  __shared__ double red[4 * R];
-  const int row0 = blockIdx.x * R;
-  gemv_rows<TK, R>(K, ld, row0, rows, cols, x, rowsum, red);
-  if (threadIdx.x < R && row0 + (int)threadIdx.x < rows) y[row0 + threadIdx.x] = rowsum[threadIdx.x];
+  const int row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * R;
+  double s[R];
+  gemv_wave<TK, R>(K, ld, row0, rows, cols, x, s);
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (row0 + r < rows) y[row0 + r] = s[r];
+  }
 }
 
 // y_part[split][j] = sum over rows of this split of K[i][j] x[i]
@@ -198,21 +202,19 @@ __global__ __launch_bounds__(256) void k_rhs(const TK* __restrict__ K, long ld, 
                                              const double* __restrict__ coef,
                                              const double* __restrict__ sinc, FricParams fp, int n,
                                              double* __restrict__ F) {
-  constexpr int PPW = (G == 3) ? 2 : (G == 1 ? 8 : 4);
-  constexpr int R = G * PPW, D = 2 * NC + 2;
-  __shared__ double rowsum[R];
-  __shared__ double red[4 * R];
-  gemv_rows<TK, R>(K, ld, blockIdx.x * R, rows, cols, v, rowsum, red);
-  const int t = threadIdx.x;
-  const int p = blockIdx.x * PPW + t;
-  if (t >= PPW || p >= n) return;
+  // one patch (its G rows) per wave, as in the fused stage kernel
+  constexpr int D = 2 * NC + 2;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  double rowsum[G];
+  gemv_wave<TK, G>(K, ld, min(p, n - 1) * G, rows, cols, v, rowsum);
+  if ((threadIdx.x & 63) != 0 || p >= n) return;
   double y[D], f[D], td[NC], sn[NC];
   for (int k = 0; k < D; ++k) y[k] = Y[k * n + p];
   for (int k = 0; k < NC; ++k) {
-    td[k] = rowsum[t * G + k];
+    td[k] = rowsum[k];
     sn[k] = sinc[NC * p + k];
   }
-  const double sdn = (G > NC) ? rowsum[t * G + NC] : 0.0;
+  const double sdn = (G > NC) ? rowsum[NC] : 0.0;
   PatchCoef c{coef[p], coef[n + p], coef[2 * n + p]};
   rsf_patch_rhs<NC>(fp, c, y, td, sdn, sn, sinc[NC * n + p], f);
   for (int k = 0; k < D; ++k) F[k * n + p] = f[k];
@@ -234,9 +236,9 @@ static int matvec_t_launch(rsfg_ctx* h, const TK* K, const double* d_x, double* 
 
 template <typename TK>
 static int matvec_launch(rsfg_ctx* h, const TK* K, const void* d_xvt, double* d_y) {
-  constexpr int R = 8;
+  constexpr int R = 2;
   using VT = typename KTraits<TK>::VT;
-  const int nb = (h->rows_all + R - 1) / R;
+  const int nb = (h->rows_all + 4 * R - 1) / (4 * R);
   hipLaunchKernelGGL((k_matvec<TK, R>), dim3(nb), dim3(256), 0, h->stream, K, (long)h->ld,
                      h->rows_all, h->cols, static_cast<const VT*>(d_xvt), d_y);
   RSFG_HIP(hipGetLastError());
@@ -764,8 +766,7 @@ int rsfg_rhs(rsfg_handle* h, int prec, const double* X, double* F, int* domain_o
   const int rows = h->rows_all, cols = h->cols;
 #define RSFG_RHS(TK, NC, GG)                                                                  \
   {                                                                                           \
-    constexpr int PPW = (GG == 3) ? 2 : (GG == 1 ? 8 : 4);                                    \
-    hipLaunchKernelGGL((k_rhs<TK, NC, GG>), dim3((n + PPW - 1) / PPW), dim3(256), 0,          \
+    hipLaunchKernelGGL((k_rhs<TK, NC, GG>), dim3((n + 3) / 4), dim3(256), 0,                  \
                        h->stream, static_cast<const TK*>(operator_ptr(h, prec)), ld, rows,    \
                        cols, reinterpret_cast<const typename KTraits<TK>::VT*>(h->vin[0]), Y, \
                        h->d_coef, h->d_sinc, h->fp, n, Fd);                                   \

@@ -46,7 +46,7 @@ enum { RED_ERR = 0, RED_VMAX, RED_VSUM, RED_SLIP, RED_MU, RED_SMAX, RED_SMIN, RE
 struct DevCtl {
   double t, h, t_end, hmax;
   double t_mon;
-  double fprev;         // event function at the accepted state
+  double vprev;         // max|v| where the event function was last evaluated
   double peakv;         // running peak |v| since the last onset
   double last_vmax;
   long long steps, rejects, domain_rejects, rhs_evals, max_steps;
@@ -60,7 +60,6 @@ struct DevCtl {
   int mon_copy;         // next prep copies X into Xmon
   int rupt_update;      // next prep marks ruptured cells
   int storm;
-  unsigned int counter; // last-workgroup ticket
   int nevent, nmon;
   int max_event, max_mon;
 };

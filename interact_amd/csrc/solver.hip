@@ -1,15 +1,18 @@
 // Device-resident adaptive Runge-Kutta cycle integration.
 //
 // One attempted step of an s-stage FSAL pair (rk5dp: s = 7, rk3bs: s = 4)
-// is s launches:
+// is s + 1 launches:
 //   k_prep            Y_1 = X + h a10 K_0, v(Y_1), stage domain check
 //   k_stage(i), i=1..s-1
 //                     y = K v(Y_i) streamed from HBM; epilogue per patch:
 //                     K_i = F(Y_i, y) (rsf_ODE_RHSFunction), then either
 //                     Y_{i+1} and v(Y_{i+1}) for the next launch, or (last
-//                     stage) the weighted error and the reductions of the
-//                     new state.  The last workgroup to finish the final
-//                     stage runs the controller below.
+//                     stage) the weighted error and per-workgroup partial
+//                     reductions of the new state
+//   k_ctrl            one workgroup: grid reduction and the controller below.
+// A separate control launch replaces a last-workgroup ticket on purpose: a
+// device-scope fence in each of ~10^3 workgroups costs more than the whole
+// stage's HBM stream on gfx950 (measured +50 us on a 40 us stage).
 // The controller restates the PETSc behaviour the reference relies on
 // (src/rsf_solve.c:440-546): TSAdapt basic (safety 0.9, x0.5 after a
 // failed attempt, exponent -1/order, clip [0.5, 2], h <= min(dt_max,
@@ -99,20 +102,30 @@ __device__ __forceinline__ void write_vel(const FricParams& fp, double a,
 }
 
 // ------------------------------------------------------------ block helpers
+__device__ __forceinline__ double red_op(int op, double v, double o) {
+  return (op == 0) ? v + o : (op == 1 ? fmax(v, o) : fmin(v, o));
+}
+
+// combination of reduction slot k (RED_*): 0 sum, 1 max, 2 min
+__device__ __forceinline__ int red_kind(int k) {
+  return (k == RED_ERR || k == RED_VMAX || k == RED_SMAX) ? 1 : (k == RED_SMIN ? 2 : 0);
+}
+
+__device__ __forceinline__ double red_identity(int op) {
+  return op == 0 ? 0.0 : (op == 1 ? -1e30 : 1e30);
+}
+
+// op 0 sum, 1 max, 2 min over the whole workgroup (up to 16 waves, sh holds
+// 16 doubles); fixed combination order, so sums are reproducible
 __device__ __forceinline__ double block_reduce(double v, int op, double* sh) {
-  // op 0 sum, 1 max, 2 min; 256 threads; deterministic order
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double o = __shfl_xor(v, off, 64);
-    v = (op == 0) ? v + o : (op == 1 ? fmax(v, o) : fmin(v, o));
-  }
+  for (int off = 32; off > 0; off >>= 1) v = red_op(op, v, __shfl_xor(v, off, 64));
   __syncthreads();
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  double r;
-  if (op == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-  else if (op == 1) r = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
-  else r = fmin(fmin(sh[0], sh[1]), fmin(sh[2], sh[3]));
+  const int nw = blockDim.x >> 6;
+  double r = sh[0];
+  for (int w = 1; w < nw; ++w) r = red_op(op, r, sh[w]);
   __syncthreads();
   return r;
 }
@@ -167,15 +180,15 @@ __device__ double dense_vmax(const StageArgs& a, const double* X0, const double*
 }
 
 // ------------------------------------------------------------ controller
-// Runs in the last workgroup of the final stage (or of the initial K_0
-// evaluation when a.s == 0).  red[] holds the grid reductions.
+// Runs in the single workgroup of k_ctrl after the final stage (or after the
+// initial K_0 evaluation when a.s == 0) on an LDS copy c of the control
+// block.  red[] holds the grid reductions.
 template <int NC>
-__device__ void controller(const StageArgs& a, const double* red) {
+__device__ void controller(const StageArgs& a, const double* red, DevCtl* c) {
   constexpr int D = Dim<NC>::v;
-  __shared__ double sh[8];
+  __shared__ double sh[16];
   __shared__ double s_par[8];
   __shared__ int s_int[4];
-  DevCtl* c = a.ctl;
   const int n = a.n, S = a.tb.s, tid = threadIdx.x;
   const SolveOpts& o = a.so;
   if (a.s == 0) {
@@ -186,8 +199,7 @@ __device__ void controller(const StageArgs& a, const double* red) {
       }
       c->rhs_evals += 1;
       c->slipping = red[RED_VMAX] > o.vel_event;
-      const double thr = o.vel_event * (c->slipping ? o.hyst : 1.0);
-      c->fprev = evfun(red[RED_VMAX], thr);
+      c->vprev = red[RED_VMAX];
       c->peakv = 0.0;
       c->last_vmax = red[RED_VMAX];
       // first monitor row (old time forced to t0 - 2 dt_monitor)
@@ -206,7 +218,6 @@ __device__ void controller(const StageArgs& a, const double* red) {
       c->stage_fail = 0;
       c->attempt_failed = 0;
       if (!(c->t < c->t_end)) c->halted = 1;
-      c->counter = 0;
     }
     return;
   }
@@ -260,10 +271,7 @@ __device__ void controller(const StageArgs& a, const double* red) {
     if (!acc) c->h = hnew;
   }
   __syncthreads();
-  if (!s_int[0]) {
-    if (tid == 0) c->counter = 0;
-    return;
-  }
+  if (!s_int[0]) return;
   // ---- accepted: events on the dense output of [t, t + h]
   const int xo = c->xcur;
   const double* X0 = a.Xb[xo];
@@ -274,15 +282,23 @@ __device__ void controller(const StageArgs& a, const double* red) {
   const double vnew = red[RED_VMAX];
   double th_lo = 0.0;
   for (int guard = 0; guard < 8; ++guard) {
-    // thread 0 decides whether [th_lo, 1] holds a crossing
+    // thread 0 decides whether [th_lo, 1] holds a crossing of the event
+    // function f = log10(max|v|) - log10(thr).  Both ends clearly on one side
+    // of thr (relative margin 1e-9, far above the rounding of the logs)
+    // cannot change sign; only otherwise are the logs evaluated.
     if (tid == 0) {
       const double thr = o.vel_event * (c->slipping ? o.hyst : 1.0);
-      const double fnew = evfun(vnew, thr);
-      const double fp = c->fprev;
-      const bool cross = (fp < 0.0 && fnew > 0.0) || (fp > 0.0 && fnew < 0.0) || fnew == 0.0;
+      const double vp = c->vprev;
+      const double lo_t = thr * (1.0 - 1e-9), hi_t = thr * (1.0 + 1e-9);
+      bool cross = false;
+      if (!((vp < lo_t && vnew < lo_t) || (vp > hi_t && vnew > hi_t))) {
+        const double fnew = evfun(vnew, thr);
+        const double fp = evfun(vp, thr);
+        cross = (fp < 0.0 && fnew > 0.0) || (fp > 0.0 && fnew < 0.0) || fnew == 0.0;
+        s_par[1] = th_lo; s_par[2] = fp; s_par[3] = 1.0; s_par[4] = fnew; s_par[5] = thr;
+      }
       s_int[1] = cross;
-      s_par[1] = th_lo; s_par[2] = fp; s_par[3] = 1.0; s_par[4] = fnew; s_par[5] = thr;
-      if (!cross) c->fprev = fnew;
+      if (!cross) c->vprev = vnew;
     }
     __syncthreads();
     if (!s_int[1]) break;
@@ -356,16 +372,12 @@ __device__ void controller(const StageArgs& a, const double* red) {
         row[7] = cnt;
         c->nevent++;
       }
-      const double thr = o.vel_event * (c->slipping ? o.hyst : 1.0);
-      c->fprev = evfun(vm, thr);
+      c->vprev = vm;
     }
     __syncthreads();
     th_lo = th;
     if (th_lo >= 1.0) {
-      if (tid == 0) {
-        const double thr = o.vel_event * (c->slipping ? o.hyst : 1.0);
-        c->fprev = evfun(vnew, thr);
-      }
+      if (tid == 0) c->vprev = vnew;
       break;
     }
   }
@@ -409,7 +421,6 @@ __device__ void controller(const StageArgs& a, const double* red) {
       c->halted = 1;
       c->status = 2;
     }
-    c->counter = 0;
   }
 }
 
@@ -464,33 +475,60 @@ __global__ __launch_bounds__(256) void k_init_vel(StageArgs a) {
   if (!rsf_patch_ok<NC>(a.fp, cf, x)) atomicOr(&a.ctl->stage_fail, 1);
 }
 
-template <typename TK, int NC, int G, int PPW>
-__global__ __launch_bounds__(256) void k_stage(StageArgs a) {
+// Wave w of workgroup b owns patch p = 4 b + w: it streams the patch's G
+// operator rows (ncomp shear rows, then the normal row when present) and
+// leaves the row sums in LDS.  Wave 0 then runs the epilogue with one lane
+// per patch of the workgroup: the epilogue is a long FP64 chain (exp, sinh,
+// log), and running it on kStagePPB lanes of one wave instead of one lane in
+// each of kStagePPB waves divides its share of the SIMDs' issue time by
+// kStagePPB.  Stages that reduce (the initial evaluation and the final
+// stage) combine the workgroup's patches with cross-lane shuffles and write
+// one column of partials[kNRed][gridDim.x]; k_ctrl reduces them.
+#ifndef RSFG_STAGE_PPB
+#define RSFG_STAGE_PPB 8
+#endif
+constexpr int kStagePPB = RSFG_STAGE_PPB;  // patches (= waves) per workgroup
+static_assert(kStagePPB >= 1 && kStagePPB <= 16, "one wave per patch, at most 1024 threads");
+
+__device__ __forceinline__ double wave_reduce(double v, int op) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = red_op(op, v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+template <typename TK, int NC, int G>
+__global__ __launch_bounds__(64 * kStagePPB) void k_stage(StageArgs a) {
   using VT = typename KTraits<TK>::VT;
   constexpr int D = Dim<NC>::v;
-  constexpr int R = G * PPW;
-  __shared__ double rowsum[R];
-  __shared__ double red[4 * R];
-  __shared__ double pred[PPW][kNRed];
-  __shared__ double gred[kNRed];
-  __shared__ int s_last;
+  __shared__ double rs[kStagePPB][G];
   DevCtl* c = a.ctl;
   if (c->halted) return;
   const int n = a.n, S = a.tb.s, s = a.s;
-  gemv_rows<TK, R>(static_cast<const TK*>(a.K), a.ld, blockIdx.x * R, a.rows_all,
-                   a.cols, static_cast<const VT*>(a.vin), rowsum, red);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  {
+    const int pw = blockIdx.x * kStagePPB + wave;
+    double rowsum[G];
+    gemv_wave<TK, G>(static_cast<const TK*>(a.K), a.ld, min(pw, n - 1) * G, a.rows_all,
+                     a.cols, static_cast<const VT*>(a.vin), rowsum);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) rs[wave][k] = rowsum[k];
+    }
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  const int t = lane;
+  const int p = blockIdx.x * kStagePPB + t;
   const int xc = c->xcur;
   const double* X = a.Xb[xc];
   double* Xn = a.Xb[xc ^ 1];
   const bool reduce = (s == 0) || (s == S - 1);
-  const int t = threadIdx.x;
-  if (t < PPW) {
-    const int p = blockIdx.x * PPW + t;
+  {
     double r9[kNRed];
     r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
     r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
     r9[RED_DX2] = 0.0;
-    if (p < n) {
+    if (t < kStagePPB && p < n) {
       const double* Yc = (s == 0) ? X : (s == S - 1 ? Xn : a.Yb[s & 1]);
       const PatchCoef cf = load_coef(a.coef, n, p);
       double y[D], kf[D], td[NC], sn[NC];
@@ -498,10 +536,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
 #pragma unroll
       for (int k = 0; k < NC; ++k) {
-        td[k] = rowsum[t * G + k];
+        td[k] = rs[t][k];
         sn[k] = a.sinc[NC * p + k];
       }
-      const double sdn = (G > NC) ? rowsum[t * G + NC] : 0.0;
+      const double sdn = (G > NC) ? rs[t][NC] : 0.0;
       rsf_patch_rhs<NC>(a.fp, cf, y, td, sdn, sn, a.sinc[NC * n + p], kf);
       double* Ks = a.Stg[c->kslot[s]];
 #pragma unroll
@@ -566,72 +604,66 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
       }
     }
     if (reduce) {
+      // lanes without a patch hold the neutral values set above
 #pragma unroll
-      for (int k = 0; k < kNRed; ++k) pred[t][k] = r9[k];
+      for (int k = 0; k < kNRed; ++k) {
+        const double q = wave_reduce(r9[k], red_kind(k));
+        if (t == 0) a.partials[(size_t)k * gridDim.x + blockIdx.x] = q;
+      }
     }
   }
-  if (!reduce) return;
+}
+
+// Step control after the final stage (a.s == S - 1) or after the initial
+// evaluation (a.s == 0): one workgroup reduces the stage kernel's partials
+// in a fixed order and runs the controller on an LDS copy of the control
+// block, which it writes back at the end (one round trip to memory instead
+// of a chain of dependent global accesses).
+constexpr int kCtrlThreads = 256;
+
+template <int NC>
+__global__ __launch_bounds__(kCtrlThreads) void k_ctrl(StageArgs a, int nb) {
+  __shared__ DevCtl sc;
+  __shared__ double gred[kNRed];
+  __shared__ double wred[kCtrlThreads / 64][kNRed];
+  constexpr int NW = sizeof(DevCtl) / sizeof(long long);
+  static_assert(sizeof(DevCtl) % sizeof(long long) == 0, "DevCtl copy granularity");
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // partials and the control block are fetched together (one memory round
+  // trip); the partials are simply unused when the solve has halted
+  double q[kNRed];
+#pragma unroll
+  for (int k = 0; k < kNRed; ++k) q[k] = red_identity(red_kind(k));
+  for (int b = tid; b < nb; b += kCtrlThreads) {
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k)
+      q[k] = red_op(red_kind(k), q[k], a.partials[(size_t)k * nb + b]);
+  }
+  long long* cw = reinterpret_cast<long long*>(a.ctl);
+  long long* sw = reinterpret_cast<long long*>(&sc);
+  for (int i = tid; i < NW; i += kCtrlThreads) sw[i] = cw[i];
+  // all nine reductions in one pass: shuffles within each wave, then the
+  // wave results in a fixed order
+#pragma unroll
+  for (int k = 0; k < kNRed; ++k) q[k] = wave_reduce(q[k], red_kind(k));
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) wred[wave][k] = q[k];
+  }
   __syncthreads();
-  if (t == 0) {
-    double q[kNRed];
-#pragma unroll
-    for (int k = 0; k < kNRed; ++k) q[k] = pred[0][k];
-    for (int w = 1; w < PPW; ++w) {
-      q[RED_ERR] = fmax(q[RED_ERR], pred[w][RED_ERR]);
-      q[RED_VMAX] = fmax(q[RED_VMAX], pred[w][RED_VMAX]);
-      q[RED_SMAX] = fmax(q[RED_SMAX], pred[w][RED_SMAX]);
-      q[RED_SMIN] = fmin(q[RED_SMIN], pred[w][RED_SMIN]);
-      q[RED_VSUM] += pred[w][RED_VSUM];
-      q[RED_SLIP] += pred[w][RED_SLIP];
-      q[RED_MU] += pred[w][RED_MU];
-      q[RED_XN2] += pred[w][RED_XN2];
-      q[RED_DX2] += pred[w][RED_DX2];
-    }
-    double* out = a.partials + (size_t)blockIdx.x * kNRed;
-#pragma unroll
-    for (int k = 0; k < kNRed; ++k) out[k] = q[k];
-    __threadfence();
-    const unsigned tk = atomicAdd(&c->counter, 1u);
-    s_last = (tk == gridDim.x - 1);
+  if (sc.halted) return;
+  if (tid < kNRed) {
+    double r = wred[0][tid];
+    for (int w = 1; w < kCtrlThreads / 64; ++w) r = red_op(red_kind(tid), r, wred[w][tid]);
+    gred[tid] = r;
   }
   __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  // grid reduction over the partials in fixed order
-  {
-    __shared__ double sh[8];
-    const int nb = gridDim.x;
-    double q[kNRed];
-    q[RED_ERR] = 0.0; q[RED_VMAX] = 0.0; q[RED_VSUM] = 0.0; q[RED_SLIP] = 0.0;
-    q[RED_MU] = 0.0; q[RED_SMAX] = -1e30; q[RED_SMIN] = 1e30; q[RED_XN2] = 0.0;
-    q[RED_DX2] = 0.0;
-    for (int b = t; b < nb; b += blockDim.x) {
-      const double* pb = a.partials + (size_t)b * kNRed;
-      q[RED_ERR] = fmax(q[RED_ERR], pb[RED_ERR]);
-      q[RED_VMAX] = fmax(q[RED_VMAX], pb[RED_VMAX]);
-      q[RED_SMAX] = fmax(q[RED_SMAX], pb[RED_SMAX]);
-      q[RED_SMIN] = fmin(q[RED_SMIN], pb[RED_SMIN]);
-      q[RED_VSUM] += pb[RED_VSUM];
-      q[RED_SLIP] += pb[RED_SLIP];
-      q[RED_MU] += pb[RED_MU];
-      q[RED_XN2] += pb[RED_XN2];
-      q[RED_DX2] += pb[RED_DX2];
-    }
-    const int ops[kNRed] = {1, 1, 0, 0, 0, 1, 2, 0, 0};
-    for (int k = 0; k < kNRed; ++k) {
-      const double r = block_reduce(q[k], ops[k], sh);
-      if (t == 0) gred[k] = r;
-    }
-    __syncthreads();
-  }
-  controller<NC>(a, gred);
+  controller<NC>(a, gred, &sc);
+  __syncthreads();
+  for (int i = tid; i < NW; i += kCtrlThreads) cw[i] = sw[i];
 }
 
 // ------------------------------------------------------------ host side
-template <typename TK, int NC, int G>
-struct StageCfg {
-  static constexpr int PPW = (G == 3) ? 2 : (G == 1 ? 8 : 4);
-};
 
 static void fill_tableau(Tableau& tb, int method) {
   memset(&tb, 0, sizeof(tb));
@@ -668,13 +700,13 @@ struct Launcher {
   virtual void init_vel(const StageArgs& a, hipStream_t st) = 0;
   virtual void prep(const StageArgs& a, hipStream_t st) = 0;
   virtual void stage(const StageArgs& a, hipStream_t st) = 0;
+  virtual void ctrl(const StageArgs& a, hipStream_t st) = 0;
   virtual int wg_count(int n) const = 0;
 };
 
 template <typename TK, int NC, int G>
 struct LauncherT : Launcher {
   using VT = typename KTraits<TK>::VT;
-  static constexpr int PPW = StageCfg<TK, NC, G>::PPW;
   void init_vel(const StageArgs& a, hipStream_t st) override {
     hipLaunchKernelGGL((k_init_vel<VT, NC>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);
   }
@@ -682,9 +714,14 @@ struct LauncherT : Launcher {
     hipLaunchKernelGGL((k_prep<VT, NC>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);
   }
   void stage(const StageArgs& a, hipStream_t st) override {
-    hipLaunchKernelGGL((k_stage<TK, NC, G, PPW>), dim3(wg_count(a.n)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_stage<TK, NC, G>), dim3(wg_count(a.n)), dim3(64 * kStagePPB), 0, st,
+                       a);
   }
-  int wg_count(int n) const override { return (n + PPW - 1) / PPW; }
+  void ctrl(const StageArgs& a, hipStream_t st) override {
+    hipLaunchKernelGGL((k_ctrl<NC>), dim3(1), dim3(kCtrlThreads), 0, st, a, wg_count(a.n));
+  }
+  // one patch per wave, kStagePPB waves per workgroup
+  int wg_count(int n) const override { return (n + kStagePPB - 1) / kStagePPB; }
 };
 
 template <typename TK>
@@ -774,6 +811,7 @@ static int start_solve(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, Sta
   a.vin = h->vin[0];
   a.vout = nullptr;
   L->stage(a, h->stream);
+  L->ctrl(a, h->stream);
   RSFG_HIP(hipGetLastError());
   RSFG_HIP(hipMemcpyAsync(c, h->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->stream));
   RSFG_HIP(hipStreamSynchronize(h->stream));
@@ -825,6 +863,7 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
         b.vout = (s < S - 1) ? h->vin[(s + 1) & 1] : nullptr;
         L->stage(b, h->stream);
       }
+      L->ctrl(b, h->stream);
     }
   };
   hipGraph_t graph = nullptr;
