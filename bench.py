@@ -8,7 +8,8 @@ relative tolerance 1e-8 over 400 yr.  One bench "step" is one pass of the
 whole cycle integration over that window.
 
 Reported: simulated years per wall-clock hour of the mixed-precision path
-(K stored in FP32, FP32 accumulation, FP64 state and error control) as
+(K stored in FP32, FP32 products summed per 16-byte slice and accumulated
+in FP64, FP64 state and error control) as
 `value`, the FP64 and bf16 hi+lo paths beside it, the error of each against
 the FP64 path at matched events (first-recurrence time, peak slip rate)
 and in K v, the roofline of the fused stage kernel (K v + RHS epilogue)
@@ -30,6 +31,16 @@ sys.path.insert(0, ROOT)
 
 SEC_PER_YEAR = 31557600.0
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the fused stage kernel on the 1 km, 8000-DOF
+# workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors that a plain
+# 256 MiB device copy reads on the same counters
+# (profiles/pmc_traffic_planar1km_8000dof.md)
+STAGE_TRAFFIC_1KM = {"fp64": 517.3e6, "fp32": 261.8e6, "bf16x2": 261.4e6, "bf16": 131.9e6}
+# algorithmic FP64 operations per (receiver, source) pair of the Okada path
+# (two DC3D evaluations + stress + tractions, half space, vertical coplanar
+# patches), counted on the oracle by oracle/opcount.cpp
+F_PAIR = 6205
+FP64_PEAK_TFLOPS = 78.6
 METRIC = "sim-years/hour + K·v HBM GB/s, mixed-prec vs FP64 GPU at matched event times"
 
 
@@ -90,6 +101,35 @@ def cpu_baseline(g, f, K, rhs_evals_ref, years, budget_s, log_fn):
                 rhs_ms=t_rhs * 1e3, assembly_pairs_per_s=1.0 / t_pairs)
 
 
+def kernels_only(fs, store):
+    """Launch each dominant kernel a few times and one plain device copy of
+    256 MiB (read 256 MiB, write 256 MiB) that calibrates the FETCH_SIZE /
+    WRITE_SIZE counters.  The order is fixed: assembly (already done by the
+    caller), then per precision 5 stage and 5 matvec launches, then the copy."""
+    import ctypes as C
+    from interact_amd import lib
+    for name in ("fp64", "fp32", "bf16x2", "bf16"):
+        p = lib.PREC_BY_NAME[name]
+        if p != lib.PREC_FP64 and not (store & p):
+            continue
+        fs.stage_bench(p, 5)
+        fs.matvec_bench(p, 5)
+    # device-to-device copy through the HIP runtime librsfg already loaded
+    # (the runtime's copy kernel: reads n bytes, writes n bytes)
+    hip = C.CDLL("libamdhip64.so")
+    n = 256 << 20
+    src, dst = C.c_void_p(), C.c_void_p()
+    ok = hip.hipMalloc(C.byref(src), C.c_size_t(n)) == 0
+    ok = ok and hip.hipMalloc(C.byref(dst), C.c_size_t(n)) == 0
+    ok = ok and hip.hipMemcpy(dst, src, C.c_size_t(n), 3) == 0   # hipMemcpyDeviceToDevice
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    if not ok:
+        raise SystemExit("kernels-only: device copy failed")
+    log(f"kernels-only: copy of {n} bytes done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,10 +141,14 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default="fp64,fp32,bf16x2")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="one assembly, a few launches of the stage and matvec kernels "
+                         "per precision and a 256 MiB device copy of known size, for "
+                         "hardware-counter runs; no cycle, no JSON bench line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
-                          "designed but not built (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "runs on one GPU only: the row-block sharding over "
+                          "several GPUs is not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     from interact_amd import faults, lib
@@ -124,6 +168,9 @@ def main():
     fs.set_loading(f["vpl"])
     X0 = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
     fs.set_state(X0)
+    if args.kernels_only:
+        kernels_only(fs, store)
+        return
 
     # K v accuracy against the FP64 operator on x_i = sin(i+1) + 0.3 cos(3i)
     i = np.arange(D)
@@ -179,9 +226,10 @@ def main():
 
     head = runs.get("fp32") or next(iter(runs.values()))
     stg = kv["fp32"]
-    roof = dict(kernel="k_stage<float,2,2,4> (fused K v + RHS epilogue)", bound="hbm",
+    traffic = STAGE_TRAFFIC_1KM["fp32"] if args.ds == 1.0 else None
+    roof = dict(kernel="k_stage<float,2,2> (fused K v + RHS epilogue)", bound="hbm",
                 achieved=stg["stage_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=stg["stage_gbs"] / HBM_PEAK_GBS, traffic=None,
+                frac=stg["stage_gbs"] / HBM_PEAK_GBS, traffic=traffic,
                 bytes_per_launch=stg["bytes"], launch_us=stg["stage_ms"] * 1e3)
     out = {
         "metric": METRIC,
@@ -204,7 +252,9 @@ def main():
         "runs": runs,
         "kv": kv,
         "errors_vs_fp64": errors,
-        "assembly": dict(pairs=pairs, ms=asm_ms, pairs_per_s=pairs / asm_ms * 1e3),
+        "assembly": dict(pairs=pairs, ms=asm_ms, pairs_per_s=pairs / asm_ms * 1e3,
+                         f_pair=F_PAIR, tflops=pairs / asm_ms * 1e3 * F_PAIR / 1e12,
+                         frac_fp64_peak=pairs / asm_ms * 1e3 * F_PAIR / 1e12 / FP64_PEAK_TFLOPS),
         "roofline": roof,
     }
     if not args.no_cpu and ref is not None:

@@ -45,9 +45,9 @@ extern "C" {
 
 /* storage precisions of the operator K (bit mask for rsfg_assemble) */
 #define RSFG_PREC_FP64 1     /* master copy, always kept */
-#define RSFG_PREC_FP32 2     /* FP32 entries, FP32 accumulation */
+#define RSFG_PREC_FP32 2     /* FP32 entries */
 #define RSFG_PREC_BF16X2 4   /* bf16 hi + bf16 lo pair (hi+lo = FP32 value
-                                rounded to 16 significant bits), FP32 acc. */
+                                rounded to 16 significant bits) */
 #define RSFG_PREC_BF16 8     /* bf16 hi only */
 
 /* time integrators (PETSc -ts_rk_type 5dp / 3bs) */
@@ -126,8 +126,10 @@ int rsfg_traction_pairs(rsfg_handle *h, long npairs, const int *recv,
 
 /* ---- stress product ---------------------------------------------------
    MatMult(Is, x, y) (src/rsf_engine.c:168-170): y = K x with the stored
-   operator of precision prec (one RSFG_PREC_* bit).  FP64 accumulates in
-   FP64, the others in FP32.  x has cols entries, y rows entries. */
+   operator of precision prec (one RSFG_PREC_* bit).  The reduced formats
+   multiply in FP32 against an FP32 copy of x, sum each 16-byte slice of a
+   row (4 or 8 entries) in FP32 and accumulate the slice sums in FP64; FP64
+   is FP64 throughout.  x has cols entries, y rows entries. */
 int rsfg_matvec(rsfg_handle *h, int prec, const double *x, double *y);
 int rsfg_matvec_dev(rsfg_handle *h, int prec, const void *x_dev,
                     void *y_dev);
@@ -173,10 +175,12 @@ int rsfg_rhs(rsfg_handle *h, int prec, const double *X, double *F,
 /* ---- adaptive cycle integration ---------------------------------------
    PETSc TSRK (5dp/3bs) + TSAdapt basic + TSEvent + rsf_TS_Monitor as set
    up in src/rsf_solve.c:440-643, run entirely on the device: stage
-   kernels stream K with the RHS fused into their epilogue, and the
-   last workgroup of each step applies the step-size controller, locates
-   log10 max|v| threshold crossings on the step's dense output, writes
-   monitor and event rows, and advances the state. */
+   kernels stream K with the RHS fused into their epilogue, and a
+   single-workgroup control kernel after each step attempt applies the
+   step-size controller, locates log10 max|v| threshold crossings on the
+   step's dense output, writes monitor and event rows, and advances the
+   state.  The host replays captured graphs of step attempts and only
+   polls a halted flag. */
 typedef struct {
   int method;           /* RSFG_RK5DP / RSFG_RK3BS */
   int prec;             /* operator precision used in the stages */

@@ -24,8 +24,19 @@
 #include <string.h>
 #include "rsfo.h"
 
+/* The scalar type of the per-pair arithmetic.  double for the oracle; the
+   operation-count harness (opcount.cpp) compiles this file as C++ with a
+   counting type, which must not change what is computed: OKD_TO_DOUBLE
+   reads a value back and OKD_NOTE_SINCOS records the one sin/cos pair of
+   the dip that DC3D evaluates per call. */
 #ifndef OKD_REAL
 #define OKD_REAL double
+#endif
+#ifndef OKD_TO_DOUBLE
+#define OKD_TO_DOUBLE(v) (v)
+#endif
+#ifndef OKD_NOTE_SINCOS
+#define OKD_NOTE_SINCOS()
 #endif
 typedef OKD_REAL real;
 
@@ -48,6 +59,7 @@ static void okd_const_init(okd_const *k, real alpha, double dip_deg)
   k->alp4 = 1.0 - alpha;
   k->alp5 = alpha;
   sincos(dip_deg * OKD_DEG2RAD, &s, &c);    /* my_sincos_deg_ftnd, src/util/mysincos.c:28-33 */
+  OKD_NOTE_SINCOS();
   k->sd = s;
   k->cd = c;
   if (fabs(c) < OKD_EPS) {
@@ -321,15 +333,15 @@ static void ext_flags(real q, const real xi[2], const real et[2], int kxi[2],
   ket[1] = (et[0] < 0.0) && (r22 + et[1] < OKD_EPS);
 }
 
-int rsfo_dc3d(double alpha, double x, double y, double z, double depth,
-              double dip_deg, double al1, double al2, double aw1, double aw2,
-              double d1, double d2, int full_space, double *u)
+static int okd_dc3d(real alpha, real x, real y, real z, real depth, double dip_deg,
+                    real al1, real al2, real aw1, real aw2, real d1, real d2,
+                    int full_space, real *u)
 {
   okd_const k;
   okd_corner g;
   real xi[2], et[2], q, d, p, ua[12], ub[12], uc[12], du[12];
   int kxi[2], ket[2], i, jx, ke, src;
-  memset(u, 0, 12 * sizeof(double));
+  memset(u, 0, 12 * sizeof(real));
   okd_const_init(&k, alpha, dip_deg);
   xi[0] = snap(x - al1);
   xi[1] = snap(x - al2);
@@ -341,14 +353,12 @@ int rsfo_dc3d(double alpha, double x, double y, double z, double depth,
     et[0] = snap(p - aw1);
     et[1] = snap(p - aw2);
     if (on_edge(q, xi, et)) {
-      memset(u, 0, 12 * sizeof(double));
+      memset(u, 0, 12 * sizeof(real));
       return 1;
     }
     ext_flags(q, xi, et, kxi, ket);
     for (ke = 0; ke < 2; ke++) {
       for (jx = 0; jx < 2; jx++) {
-        /* corner sign: + for (xi1,et1),(xi2,et2), - for the mixed ones */
-        real sgn = (jx + ke == 1) ? -1.0 : 1.0;
         okd_corner_init(&g, xi[jx], et[ke], q, &k, kxi[ke], ket[jx]);
         okd_part_a(&g, &k, d1, d2, ua);
         if (src == 0) {
@@ -378,11 +388,27 @@ int rsfo_dc3d(double alpha, double x, double y, double z, double depth,
           du[10] = du[10] + uc[1] * k.cd - uc[2] * k.sd;
           du[11] = du[11] - uc[1] * k.sd - uc[2] * k.cd;
         }
-        for (i = 0; i < 12; i++) u[i] += sgn * du[i];
+        /* corner sum: + for (xi1,et1),(xi2,et2), - for the mixed ones */
+        if (jx + ke == 1) {
+          for (i = 0; i < 12; i++) u[i] -= du[i];
+        } else {
+          for (i = 0; i < 12; i++) u[i] += du[i];
+        }
       }
     }
   }
   return 0;
+}
+
+int rsfo_dc3d(double alpha, double x, double y, double z, double depth,
+              double dip_deg, double al1, double al2, double aw1, double aw2,
+              double d1, double d2, int full_space, double u[12])
+{
+  real ur[12];
+  int i, iret = okd_dc3d(alpha, x, y, z, depth, dip_deg, al1, al2, aw1, aw2, d1, d2,
+                         full_space, ur);
+  for (i = 0; i < 12; i++) u[i] = OKD_TO_DOUBLE(ur[i]);
+  return iret;
 }
 
 /* ---------------- frames, stress, traction ---------------- */
@@ -414,29 +440,29 @@ void rsfo_patch_frame(float strike, float dip, double *fr)
 #define OKD_MU 1.0e4
 #define OKD_NU 0.25
 
-void rsfo_traction_pair(const double *xr, const double *frr, const double *xs,
-                        const double *frs, double ls, double ws,
-                        int full_space, double out[2][3], int iret[2])
+static void okd_traction_pair(const double *xr, const double *frr, const double *xs,
+                              const double *frs, double ls, double ws, int full_space,
+                              real out[2][3], int iret[2])
 {
   const double mu = OKD_MU, mu2 = 2.0 * OKD_MU;
   const double lambda = (mu2 * OKD_NU) / (1.0 - 2.0 * OKD_NU);
   const double alpha = (lambda + mu) / (lambda + mu2);
-  double dx, dy, xl, yl, u[12], iso, s[3][3], g[3][3], tr[3];
-  double c = frs[0], sn = -frs[1];     /* rotate back with (cos a, -sin a) */
-  double cc, ssq, sc;
+  real dx, dy, xl, yl, u[12], iso, s[3][3], g[3][3], tr[3];
+  const real xr0 = xr[0], xr1 = xr[1], xs0 = xs[0], xs1 = xs[1];
+  const real c = frs[0], sn = -frs[1];     /* rotate back with (cos a, -sin a) */
+  real cc, ssq, sc;
   int sidx, i, j;
   /* shift, keep z absolute, rotate into the strike frame (eval_okada.c:95-99) */
-  dx = xr[0] - xs[0];
-  dy = xr[1] - xs[1];
+  dx = xr0 - xs0;
+  dy = xr1 - xs1;
   xl = frs[0] * dx + frs[1] * dy;     /* rotate_vec, src/geometry/llgeo.c:30-36 */
   yl = -frs[1] * dx + frs[0] * dy;
   cc = c * c;
   ssq = sn * sn;
   sc = c * sn;
   for (sidx = 0; sidx < 2; sidx++) {
-    iret[sidx] = rsfo_dc3d(alpha, xl, yl, xr[2], -xs[2], frs[11], -ls, ls, -ws,
-                           ws, sidx == 0 ? 1.0 : 0.0, sidx == 1 ? 1.0 : 0.0,
-                           full_space, u);
+    iret[sidx] = okd_dc3d(alpha, xl, yl, xr[2], -xs[2], frs[11], -ls, ls, -ws, ws,
+                          sidx == 0 ? 1.0 : 0.0, sidx == 1 ? 1.0 : 0.0, full_space, u);
     if (iret[sidx]) {
       out[sidx][0] = out[sidx][1] = out[sidx][2] = 0.0;
       continue;
@@ -466,6 +492,17 @@ void rsfo_traction_pair(const double *xr, const double *frr, const double *xs,
     out[sidx][1] = tr[0] * frr[8] + tr[1] * frr[9] + tr[2] * frr[10];
     out[sidx][2] = tr[0] * frr[2] + tr[1] * frr[3] + tr[2] * frr[4];
   }
+}
+
+void rsfo_traction_pair(const double *xr, const double *frr, const double *xs,
+                        const double *frs, double ls, double ws,
+                        int full_space, double out[2][3], int iret[2])
+{
+  real o[2][3];
+  int sidx, t;
+  okd_traction_pair(xr, frr, xs, frs, ls, ws, full_space, o, iret);
+  for (sidx = 0; sidx < 2; sidx++)
+    for (t = 0; t < 3; t++) out[sidx][t] = OKD_TO_DOUBLE(o[sidx][t]);
 }
 
 void rsfo_traction_tensor(int n, const double *xyz, const float *strike,
