@@ -54,15 +54,20 @@ extern "C" {
 #define RSFG_RK5DP 0
 #define RSFG_RK3BS 1
 
-/* one fault patch: a rectangle as in the reference's patch file
+/* one fault patch as in the reference's patch file
    "x y z strike dip L W group" (src/read_geometry.c:115-117, strike and
-   dip kept as float like struct flt, src/includes/structures.h:69) */
+   dip kept as float like struct flt, src/includes/structures.h:69).
+   L < 0 and W < 0 mark a triangle whose three vertices follow on the
+   line (src/read_geometry.c:148-215): vert holds them, x y z strike dip
+   are then ignored (the receiver point is the vertex mean and the plane
+   vectors are the triangle's TDCS vectors). */
 typedef struct {
   double x, y, z;       /* centroid [m], z <= 0 in a half space */
   float strike, dip;    /* degrees */
-  double l, w;          /* half length, half width [m] */
+  double l, w;          /* half length, half width [m]; both < 0: triangle */
   double rake[2];       /* (strike, dip) fractions for RSFG_SLIP_RAKE */
   int group;
+  double vert[9];       /* triangle vertices x1 y1 z1 x2 ... z3 [m], z <= 0 */
 } rsfg_patch;
 
 typedef struct rsfg_ctx rsfg_handle;

@@ -37,6 +37,43 @@ constexpr double kDeg2Rad = 0.01745329251994329576923690768488612713442871888541
 constexpr double kOkEps = 1.0e-8;
 constexpr size_t kChunkMax = 512;   // largest GEMV chunk (bf16: 64 lanes x 8)
 
+// Geometry of a triangular patch as read_geometry derives it
+// (src/read_geometry.c:148-215 -> get_tri_prop_based_on_gh,
+// src/green/eval_triangle_nw.c:99-140): receiver point = vertex mean
+// (calc_centroid_tri, src/geometry/geometry.c:1037-1045), receiver plane
+// vectors = the TDCS (normal, strike, dip) of get_tdcs_base_vectors
+// (src/green/tddisphs.f90:790-817).  The rectangle fields stay zero.
+static void tri_geometry(const rsfg_patch& q, int n, int i, double* g) {
+  const double* v = q.vert;
+  g[kGeoX * n + i] = (v[0] + v[3] + v[6]) / 3.0;
+  g[kGeoY * n + i] = (v[1] + v[4] + v[7]) / 3.0;
+  g[kGeoZ * n + i] = (v[2] + v[5] + v[8]) / 3.0;
+  const double a[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+  const double b[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+  double vn[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2],
+                  a[0] * b[1] - a[1] * b[0]};
+  double len = sqrt(vn[0] * vn[0] + vn[1] * vn[1] + vn[2] * vn[2]);
+  for (int k = 0; k < 3; ++k) vn[k] /= len;
+  double vs[3] = {-vn[1], vn[0], 0.0};
+  if (sqrt(vs[0] * vs[0] + vs[1] * vs[1]) == 0.0) {
+    vs[0] = 0.0;
+    vs[1] = (v[2] > 0.0) ? -vn[2] : vn[2];
+  }
+  len = sqrt(vs[0] * vs[0] + vs[1] * vs[1] + vs[2] * vs[2]);
+  for (int k = 0; k < 3; ++k) vs[k] /= len;
+  const double vd[3] = {vn[1] * vs[2] - vn[2] * vs[1], vn[2] * vs[0] - vn[0] * vs[2],
+                        vn[0] * vs[1] - vn[1] * vs[0]};
+  for (int k = 0; k < 3; ++k) {
+    g[(kGeoN0 + k) * n + i] = vn[k];
+    g[(kGeoS0 + k) * n + i] = vs[k];
+    g[(kGeoD0 + k) * n + i] = vd[k];
+  }
+  g[kGeoR0 * n + i] = q.rake[0];
+  g[kGeoR1 * n + i] = q.rake[1];
+  g[kGeoTri * n + i] = 1.0;
+  for (int k = 0; k < 9; ++k) g[(kGeoV0 + k) * n + i] = v[k];
+}
+
 int ensure_tmp(rsfg_ctx* h, size_t bytes) {
   if (bytes <= h->tmp_bytes) return RSFG_OK;
   if (h->d_tmp) hipFree(h->d_tmp);
@@ -298,14 +335,30 @@ int rsfg_read_patch_file(const char* fname, rsfg_patch* patches, int* n) {
   int k = 0;
   rsfg_patch p;
   memset(&p, 0, sizeof p);
+  int bad = 0;
   while (k < *n && fscanf(in, "%lf %lf %lf %f %f %lf %lf %i", &p.x, &p.y, &p.z, &p.strike,
                           &p.dip, &p.l, &p.w, &p.group) == 8) {
     p.rake[0] = 1.0;
     p.rake[1] = 0.0;
+    for (int j = 0; j < 9; ++j) p.vert[j] = 0.0;
+    if (p.l < 0 && p.w < 0) {
+      // triangle: nine vertex coordinates follow (src/read_geometry.c:148-175)
+      if (fscanf(in, "%lf %lf %lf %lf %lf %lf %lf %lf %lf", p.vert, p.vert + 1, p.vert + 2,
+                 p.vert + 3, p.vert + 4, p.vert + 5, p.vert + 6, p.vert + 7, p.vert + 8) != 9) {
+        bad = 1;
+        break;
+      }
+    } else if (p.l <= 0 || p.w <= 0) {
+      bad = 2;   // point sources, 2-D segments and irregular quads are not supported
+      break;
+    }
     patches[k++] = p;
   }
   fclose(in);
   *n = k;
+  RSFG_CHECK(bad != 1, RSFG_EINVAL, "rsfg_read_patch_file: truncated triangle line");
+  RSFG_CHECK(bad != 2, RSFG_EINVAL,
+             "rsfg_read_patch_file: only rectangles and triangles are supported");
   return RSFG_OK;
 }
 
@@ -316,8 +369,14 @@ int rsfg_create(int device, const rsfg_patch* pt, int n, rsfg_handle** out) {
   RSFG_HIP(hipGetDeviceCount(&ndev));
   RSFG_CHECK(device >= 0 && device < ndev, RSFG_EINVAL, "rsfg_create: no such device");
   for (int i = 0; i < n; ++i) {
-    RSFG_CHECK(pt[i].l > 0 && pt[i].w > 0, RSFG_EINVAL,
-               "rsfg_create: rectangles need positive half length and width");
+    const bool tri = pt[i].l < 0 && pt[i].w < 0;
+    RSFG_CHECK(tri || (pt[i].l > 0 && pt[i].w > 0), RSFG_EINVAL,
+               "rsfg_create: rectangles need positive half length and width "
+               "(both negative marks a triangle)");
+    if (tri) {
+      RSFG_CHECK(pt[i].vert[2] <= 0.0 && pt[i].vert[5] <= 0.0 && pt[i].vert[8] <= 0.0,
+                 RSFG_EINVAL, "rsfg_create: triangle vertex above the surface (z > 0)");
+    }
   }
   RSFG_HIP(hipSetDevice(device));
   rsfg_ctx* h = new rsfg_ctx();
@@ -325,8 +384,14 @@ int rsfg_create(int device, const rsfg_patch* pt, int n, rsfg_handle** out) {
   h->n = n;
   h->h_geo = new double[(size_t)kGeoFields * n];
   double* g = h->h_geo;
+  for (int i = 0; i < (int)kGeoFields * n; ++i) g[i] = 0.0;
   for (int i = 0; i < n; ++i) {
     const rsfg_patch& q = pt[i];
+    if (q.l < 0 && q.w < 0) {
+      tri_geometry(q, n, i, g);
+      h->has_tri = true;
+      continue;
+    }
     // angle range fix-up of check_fault_angles (src/geometry/geometry.c:985-1015)
     double dip = (double)q.dip, strike = (double)q.strike;
     if (dip > 90.0) { dip = 180.0 - dip; strike += 180.0; }

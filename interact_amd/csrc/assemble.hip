@@ -9,6 +9,7 @@
 
 #include "okada_dev.h"
 #include "rsfg_common.h"
+#include "tri_dev.h"
 
 namespace rsfg {
 
@@ -92,6 +93,53 @@ __device__ __forceinline__ bool pair_traction(const double* __restrict__ geo,
   return false;
 }
 
+// Tractions at receiver i from unit strike (k = 0) and dip (k = 1) slip on
+// the triangular source j (always a half space, as eval_triangle_general
+// evaluates triangles, src/eval_green.c:156-168): T[k] = (strike, dip,
+// normal) components of S n on the receiver plane (resolve_force /
+// resolve_stress_on_fault, src/geometry/geometry.c:73-111).  Returns true
+// when the stress is not finite (T = 0, the reference's entry 0).
+__device__ __forceinline__ bool tri_pair_traction(const double* __restrict__ geo, int n, int i,
+                                                  int j, double lambda, double mu,
+                                                  double T[2][3]) {
+  double v[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v[k] = geo[(kGeoV0 + k) * n + j];
+  const double P[3] = {geo[kGeoX * n + i], geo[kGeoY * n + i], geo[kGeoZ * n + i]};
+  double S[2][3][3];
+  if (!tri_unit_stress(P, v, 2.0 * mu, lambda, S)) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) T[k][0] = T[k][1] = T[k][2] = 0.0;
+    return true;
+  }
+  double nr[3], sr[3], dr[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    nr[k] = geo[(kGeoN0 + k) * n + i];
+    sr[k] = geo[(kGeoS0 + k) * n + i];
+    dr[k] = geo[(kGeoD0 + k) * n + i];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    double t[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) t[a] = nr[0] * S[k][a][0] + nr[1] * S[k][a][1] + nr[2] * S[k][a][2];
+    T[k][0] = t[0] * sr[0] + t[1] * sr[1] + t[2] * sr[2];
+    T[k][1] = t[0] * dr[0] + t[1] * dr[1] + t[2] * dr[2];
+    T[k][2] = t[0] * nr[0] + t[1] * nr[1] + t[2] * nr[2];
+  }
+  return false;
+}
+
+// source-type dispatch: meshes without triangles compile the Okada path only
+template <bool TRI>
+__device__ __forceinline__ bool any_pair_traction(const double* __restrict__ geo, int n, int i,
+                                                  int j, bool full, const OkMedium& med,
+                                                  double lambda, double mu, double T[2][3]) {
+  if (TRI && geo[kGeoTri * n + j] != 0.0) return tri_pair_traction(geo, n, i, j, lambda, mu, T);
+  return pair_traction(geo, n, i, j, full, med, lambda, mu, T);
+}
+
 __device__ __forceinline__ void store_entry(const AsmArgs& a, size_t idx,
                                             double v) {
   a.K64[idx] = v;
@@ -107,12 +155,13 @@ __device__ __forceinline__ void store_entry(const AsmArgs& a, size_t idx,
   }
 }
 
+template <bool TRI>
 __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= a.n) return;
   double T[2][3];
-  pair_traction(a.geo, a.n, i, j, a.full != 0, a.med, a.lambda, a.mu, T);
+  any_pair_traction<TRI>(a.geo, a.n, i, j, a.full != 0, a.med, a.lambda, a.mu, T);
   const int n = a.n, G = a.G, nc = a.ncomp;
   const double sc = a.scale;
   switch (a.mode) {
@@ -150,6 +199,7 @@ __global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
 }
 
 // bulk traction blocks for arbitrary pairs (unscaled, internal units)
+template <bool TRI>
 __global__ __launch_bounds__(256) void k_pairs(const double* geo, int n,
                                                long np, const int* recv,
                                                const int* src, int full,
@@ -158,7 +208,7 @@ __global__ __launch_bounds__(256) void k_pairs(const double* geo, int n,
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   if (p >= np) return;
   double T[2][3];
-  pair_traction(geo, n, recv[p], src[p], full != 0, med, lambda, mu, T);
+  any_pair_traction<TRI>(geo, n, recv[p], src[p], full != 0, med, lambda, mu, T);
 #pragma unroll
   for (int k = 0; k < 2; ++k)
 #pragma unroll
@@ -197,7 +247,10 @@ int launch_assemble(rsfg_ctx* h, double* gpu_ms) {
   a.Kb16 = h->Kb16;
   dim3 grid((h->n + 255) / 256, h->n);
   RSFG_HIP(hipEventRecord(h->ev0, h->stream));
-  hipLaunchKernelGGL(k_assemble, grid, dim3(256), 0, h->stream, a);
+  if (h->has_tri)
+    hipLaunchKernelGGL(k_assemble<true>, grid, dim3(256), 0, h->stream, a);
+  else
+    hipLaunchKernelGGL(k_assemble<false>, grid, dim3(256), 0, h->stream, a);
   RSFG_HIP(hipGetLastError());
   RSFG_HIP(hipEventRecord(h->ev1, h->stream));
   RSFG_HIP(hipEventSynchronize(h->ev1));
@@ -213,9 +266,12 @@ int launch_pairs(rsfg_ctx* h, long np, const int* d_recv, const int* d_src,
   double lambda, mu;
   internal_medium(m, lambda, mu);
   const long nb = (np + 255) / 256;
-  hipLaunchKernelGGL(k_pairs, dim3((unsigned)nb), dim3(256), 0, h->stream,
-                     h->d_geo, h->n, np, d_recv, d_src, full, m, lambda, mu,
-                     d_out);
+  if (h->has_tri)
+    hipLaunchKernelGGL(k_pairs<true>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->d_geo,
+                       h->n, np, d_recv, d_src, full, m, lambda, mu, d_out);
+  else
+    hipLaunchKernelGGL(k_pairs<false>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->d_geo,
+                       h->n, np, d_recv, d_src, full, m, lambda, mu, d_out);
   RSFG_HIP(hipGetLastError());
   return RSFG_OK;
 }

@@ -41,8 +41,8 @@ constexpr int kMaxStages = 7;
 constexpr int kNRed = 9;
 enum { RED_ERR = 0, RED_VMAX, RED_VSUM, RED_SLIP, RED_MU, RED_SMAX, RED_SMIN, RED_XN2, RED_DX2 };
 
-// Controller state, resident in device memory and updated by the last
-// workgroup of every final-stage kernel.
+// Controller state, resident in device memory; the single-workgroup
+// control kernel updates it after every step attempt.
 struct DevCtl {
   double t, h, t_end, hmax;
   double t_mon;
@@ -90,6 +90,7 @@ struct rsfg_ctx {
   double* h_geo = nullptr;    // host copy
   // operator
   bool assembled = false;
+  bool has_tri = false;        // any triangular patch (selects the assembly kernel)
   int slip_mode = 0, ncomp = 1, with_normal = 0, full_space = 0, store = 0;
   int G = 1;                  // rows per patch (ncomp shear + normal)
   int rows_all = 0;           // G * n
@@ -143,7 +144,9 @@ enum {
   kGeoS0, kGeoS1, kGeoS2,       // strike vector
   kGeoD0, kGeoD1, kGeoD2,       // dip vector
   kGeoR0, kGeoR1,               // rake fractions
-  kGeoFields
+  kGeoTri,                      // 1 for a triangle (vertices below), else 0
+  kGeoV0,                       // triangle vertices: 9 fields x1 y1 z1 ... z3
+  kGeoFields = kGeoV0 + 9
 };
 
 int ensure_tmp(rsfg_ctx* h, size_t bytes);

@@ -26,15 +26,68 @@ def planar_grid(ds_km, length_km=100.0, depth_km=40.0):
                 along=(i + 0.5 - nx / 2.0) * ds_km, depth=(j + 0.5) * ds_km)
 
 
+def rough_offset(nx, nz, ds_km, hurst=0.8, alpha=1e-3, seed=20261202):
+    """Self-affine random offset field on the (nx + 1) x (nz + 1) node grid
+    of a planar_grid outline: 2-D Fourier synthesis with amplitude spectrum
+    |k|^-(H + 1) (Hurst exponent H) and seeded Gaussian coefficients (PCG64),
+    zero mean, scaled to an rms of alpha times the along-strike length
+    (amplitude-to-wavelength ratio alpha at the longest wavelength) [m]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ds = ds_km * 1e3
+    kx = np.fft.fftfreq(nx + 1, d=ds)[:, None]
+    kz = np.fft.fftfreq(nz + 1, d=ds)[None, :]
+    k = np.sqrt(kx**2 + kz**2)
+    amp = np.zeros_like(k)
+    amp[k > 0] = k[k > 0] ** -(hurst + 1.0)
+    coef = rng.standard_normal(k.shape) + 1j * rng.standard_normal(k.shape)
+    h = np.real(np.fft.ifft2(coef * amp))
+    h -= h.mean()
+    return h * (alpha * nx * ds / np.sqrt(np.mean(h**2)))
+
+
+def planar_tri_grid(ds_km, length_km=100.0, depth_km=40.0, offset=None):
+    """The planar_grid outline meshed with two triangles per square cell
+    (2 nx nz triangles, triangle 2 (i nz + j) + t for cell (i, j)).  Nodes
+    at y = (i - nx/2) ds, z = -j ds, x = offset[i, j] (0 for the plane; the
+    rough variant passes rough_offset).  Cell (i, j) with nodes A = (i, j+1),
+    B = (i+1, j+1), C = (i, j), D = (i+1, j) splits into (A, B, C) and
+    (B, D, C): on the plane their TDCS normal is +x, strike +y and dip +z,
+    the rectangles' frame.  Patches carry l = w = -1 (the reference's
+    triangle marker), the vertex mean as xyz, and vert[9]."""
+    nx, nz = int(round(length_km / ds_km)), int(round(depth_km / ds_km))
+    ds = ds_km * 1e3
+    off = np.zeros((nx + 1, nz + 1)) if offset is None else np.asarray(offset, np.float64)
+
+    def node(i, j):
+        return np.stack([off[i, j], (i - nx / 2.0) * ds, -j * ds * np.ones_like(i * 1.0)], axis=-1)
+
+    i, j = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    A, B, C, D = node(i, j + 1), node(i + 1, j + 1), node(i, j), node(i + 1, j)
+    t1 = np.concatenate([A, B, C], axis=1)
+    t2 = np.concatenate([B, D, C], axis=1)
+    vert = np.stack([t1, t2], axis=1).reshape(-1, 9)
+    n = len(vert)
+    xyz = (vert[:, 0:3] + vert[:, 3:6] + vert[:, 6:9]) / 3.0
+    return dict(xyz=xyz, strike=np.zeros(n, np.float32), dip=np.full(n, 90.0, np.float32),
+                l=np.full(n, -1.0), w=np.full(n, -1.0), vert=vert, nx=nx, nz=nz,
+                ds_km=ds_km, along=xyz[:, 1] / 1e3, depth=-xyz[:, 2] / 1e3)
+
+
 def write_patch_file(fname, g):
-    """Reference patch-file format "x y z strike dip L W group"."""
+    """Reference patch-file format "x y z strike dip L W group"; triangles
+    (l, w < 0) append their nine vertex coordinates (src/read_geometry.c:
+    148-175), written with round-trip precision."""
     grp = g.get("group", np.zeros(len(g["l"]), int))
     with open(fname, "w") as f:
         for k in range(len(g["l"])):
             x, y, z = g["xyz"][k]
-            f.write(f"{x:.6e} {y:.6e} {z:.6e} {float(g['strike'][k]):.6f} "
+            line = (f"{x:.6e} {y:.6e} {z:.6e} {float(g['strike'][k]):.6f} "
                     f"{float(g['dip'][k]):.6f} {g['l'][k]:.6e} {g['w'][k]:.6e} "
-                    f"{int(grp[k])}\n")
+                    f"{int(grp[k])}")
+            if g["l"][k] < 0 and g["w"][k] < 0:
+                line += " " + " ".join(f"{v:.17e}" for v in g["vert"][k])
+            f.write(line + "\n")
 
 
 def bp5_fields(g, a0=0.004, a_max=0.04, b0=0.03, dc0=0.14, dc_nuc=0.13,

@@ -40,7 +40,8 @@ class Patch(C.Structure):
     _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double),
                 ("strike", C.c_float), ("dip", C.c_float),
                 ("l", C.c_double), ("w", C.c_double),
-                ("rake", C.c_double * 2), ("group", C.c_int)]
+                ("rake", C.c_double * 2), ("group", C.c_int),
+                ("vert", C.c_double * 9)]
 
 
 class Assembly(C.Structure):
@@ -156,7 +157,9 @@ def read_patch_file(fname, capacity=None):
     return arr, n.value
 
 
-def patches_from_arrays(xyz, strike, dip, l, w, rake=None, group=None):
+def patches_from_arrays(xyz, strike, dip, l, w, rake=None, group=None, vert=None):
+    """ctypes patch array; l, w < 0 with vert[i] (9 coordinates) marks a
+    triangle."""
     n = len(l)
     arr = (Patch * n)()
     for i in range(n):
@@ -171,4 +174,7 @@ def patches_from_arrays(xyz, strike, dip, l, w, rake=None, group=None):
         else:
             p.rake[0], p.rake[1] = 1.0, 0.0
         p.group = int(group[i]) if group is not None else 0
+        if vert is not None:
+            for k in range(9):
+                p.vert[k] = float(vert[i][k])
     return arr

@@ -19,12 +19,13 @@ G_INTERNAL = 1.0e4   # interact's internal shear modulus (src/includes/propertie
 class FaultSystem:
     def __init__(self, patches, device=0):
         """patches: ctypes array of lib.Patch or a dict of arrays
-        (xyz, strike, dip, l, w[, rake, group])."""
+        (xyz, strike, dip, l, w[, rake, group, vert]); l, w < 0 with the
+        9 vertex coordinates in vert marks a triangle."""
         if isinstance(patches, dict):
             patches = L.patches_from_arrays(
                 patches["xyz"], patches["strike"], patches["dip"],
                 patches["l"], patches["w"], patches.get("rake"),
-                patches.get("group"))
+                patches.get("group"), patches.get("vert"))
         self._lib = L.load()
         self.n = len(patches)
         self._patches = patches

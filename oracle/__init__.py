@@ -85,6 +85,14 @@ def _setup(L):
     L.rsfo_traction_tensor.restype = None
     L.rsfo_traction_tensor.argtypes = [C.c_int, _dp, _fp, _fp, _dp, _dp,
                                        C.c_int, C.c_int, _dp, C.c_void_p]
+    L.rsfo_tri_frame.restype = None
+    L.rsfo_tri_frame.argtypes = [_dp] * 7
+    L.rsfo_tri_stress.restype = C.c_int
+    L.rsfo_tri_stress.argtypes = [_dp] * 4 + [C.c_double] * 6 + [_dp]
+    L.rsfo_mesh_traction_tensor.restype = None
+    L.rsfo_mesh_traction_tensor.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp,
+                                            _dp, C.c_int, C.c_int, _dp,
+                                            C.c_void_p]
     L.rsfo_read_patches.restype = C.c_int
     L.rsfo_read_patches.argtypes = [C.c_char_p, C.c_int, _dp, _fp, _fp, _dp,
                                     _dp, _ip]
@@ -153,6 +161,63 @@ def traction_tensor(g, full_space=0, nthreads=8, want_iret=False):
         np.ascontiguousarray(g["w"], np.float64), int(full_space),
         int(nthreads), T,
         ir.ctypes.data_as(C.c_void_p) if want_iret else None)
+    T = T.reshape(n, n, 2, 3)
+    if want_iret:
+        return T, ir.reshape(n, n, 2)
+    return T
+
+
+def tri_frame(vert):
+    """(normal, strike, dip, area) of a triangle, vert = 9 coordinates."""
+    v = np.ascontiguousarray(vert, np.float64).ravel()
+    vn, vs, vd, ar = np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(1)
+    lib().rsfo_tri_frame(v[0:3].copy(), v[3:6].copy(), v[6:9].copy(), vn, vs, vd, ar)
+    return vn, vs, vd, float(ar[0])
+
+
+def tri_stress(P, vert, ss, ds, ts, nu=0.25, mu=1.0e4):
+    """Half-space stress tensor at P from slip on a triangle; (S, rc)."""
+    v = np.ascontiguousarray(vert, np.float64).ravel()
+    S = np.zeros(9)
+    lam = 2.0 * mu * nu / (1.0 - 2.0 * nu)
+    rc = lib().rsfo_tri_stress(np.ascontiguousarray(P, np.float64), v[0:3].copy(),
+                               v[3:6].copy(), v[6:9].copy(), ss, ds, ts, nu, 2.0 * mu, lam, S)
+    return S.reshape(3, 3), rc
+
+
+def mesh_receivers(g):
+    """(is_tri, receiver points, 13-frames) of a mesh dict: rectangles use
+    their centroid and float-angle frame, triangles (l, w < 0, vertices in
+    g["vert"]) their vertex mean and TDCS vectors, as read_geometry does
+    (src/read_geometry.c:148-215, get_tri_prop_based_on_gh,
+    src/green/eval_triangle_nw.c:99-140)."""
+    n = len(g["l"])
+    l, w = np.asarray(g["l"]), np.asarray(g["w"])
+    is_tri = ((l < 0) & (w < 0)).astype(np.int32)
+    xyz = np.array(g["xyz"], np.float64).reshape(n, 3)
+    fr = np.zeros((n, 13))
+    vert = np.asarray(g["vert"], np.float64).reshape(n, 9) if "vert" in g else np.zeros((n, 9))
+    for i in range(n):
+        if is_tri[i]:
+            v = vert[i]
+            xyz[i] = (v[0:3] + v[3:6] + v[6:9]) / 3.0
+            vn, vs, vd, _ = tri_frame(v)
+            fr[i, 2:5], fr[i, 5:8], fr[i, 8:11] = vn, vs, vd
+        else:
+            fr[i] = patch_frame(g["strike"][i], g["dip"][i])
+    return is_tri, xyz, fr, vert
+
+
+def mesh_traction_tensor(g, full_space=0, nthreads=8, want_iret=False):
+    """T[i, j, slip, trac] for a mesh of rectangles and triangles."""
+    n = len(g["l"])
+    is_tri, xyz, fr, vert = mesh_receivers(g)
+    T = np.zeros(n * n * 6)
+    ir = np.zeros(2 * n * n, np.int32) if want_iret else None
+    lib().rsfo_mesh_traction_tensor(
+        n, is_tri, xyz.ravel(), fr.ravel(), np.ascontiguousarray(g["l"], np.float64),
+        np.ascontiguousarray(g["w"], np.float64), vert.ravel(), int(full_space),
+        int(nthreads), T, ir.ctypes.data_as(C.c_void_p) if want_iret else None)
     T = T.reshape(n, n, 2, 3)
     if want_iret:
         return T, ir.reshape(n, n, 2)

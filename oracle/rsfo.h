@@ -46,6 +46,37 @@ void rsfo_traction_pair(const double *xr, const double *fr_rec,
                         double ws, int full_space, double out[2][3],
                         int iret[2]);
 
+/* ---- triangular dislocations (Nikkhoo & Walter 2015), rsfo_tri.c ---- */
+
+/* TDCS base vectors and area of the triangle p1 p2 p3
+   (get_tdcs_base_vectors, src/green/tddisphs.f90:790-817) */
+void rsfo_tri_frame(const double *p1, const double *p2, const double *p3,
+                    double *vnorm, double *vstrike, double *vdip, double *area);
+
+/* stress S at point P (z <= 0) from slip (strike ss, dip ds, tensile ts)
+   on the triangle p1 p2 p3 in a half space (tdstresshs,
+   src/green/tdstresshs.f90:9-148) with Poisson ratio nu, 2 mu and lambda.
+   Returns 0, 1 if the stress is not finite (P on an edge line in the
+   triangle's plane: the reference's NaN -> entry 0), -1 if a point lies
+   above the surface.  A triangle with all vertices at z = 0 gives S = 0. */
+int rsfo_tri_stress(const double *P, const double *p1, const double *p2,
+                    const double *p3, double ss, double ds, double ts,
+                    double nu, double mu2, double lambda, double S[3][3]);
+
+/* tractions (strike, dip, normal) at every receiver for unit strike and
+   unit dip slip on every source of a mesh of rectangles and triangles,
+   internal units (shear modulus 1e4, nu 0.25), as GenKEntries_petsc sees
+   them.  Per patch: is_tri, receiver point xyz[3] (the centroid),
+   fr[13] in rsfo_patch_frame's layout (for triangles only the normal,
+   strike and dip vectors, fr[2..10], are used: the TDCS vectors), l, w
+   (rectangles), verts[9] (triangles).  T and iret as rsfo_traction_tensor;
+   triangle sources are always evaluated in the half space. */
+void rsfo_mesh_traction_tensor(int n, const int *is_tri, const double *xyz,
+                               const double *fr, const double *l,
+                               const double *w, const double *verts,
+                               int full_space, int nthreads, double *T,
+                               int *iret);
+
 /* patch-file reader with the reference's scan format
    "%lf %lf %lf %f %f %lf %lf %i" (src/includes/precision_double.h:31,
    read loop src/read_geometry.c:115-117).  Rectangles only.  Returns the

@@ -15,6 +15,12 @@ Outputs
                         on a 12 x 5 planar grid with 1 km cells, half space
   ktensor_random.npz    same on 40 random rectangles (strike, dip, size),
                         half and full space
+  ktensor_tri.npz       reference tractions on triangle meshes (Nikkhoo-Walter
+                        sources, TDCS receiver frames): planar split grid,
+                        rough (Hurst 0.8) grid, random triangles incl. a
+                        surface-touching, a horizontal and a vertical-edge
+                        one, and a mixed rectangle + triangle mesh (half and
+                        full space)
   dc3d_cases.npz        direct DC3D calls: random points, points on edges,
                         on negative edge extensions, at the surface, and
                         dips 0 / 90 / 89.9999995 (cos snap) / 45
@@ -65,14 +71,76 @@ def kmat(geom_file, full):
     return T, ir
 
 
+def write_mesh(fname, g):
+    """rectangles as write_geom; triangles (l, w < 0) with their nine
+    vertex coordinates appended (src/read_geometry.c:148-175)"""
+    with open(fname, "w") as f:
+        for k in range(len(g["l"])):
+            x, y, z = g["xyz"][k]
+            line = "%.12e %.12e %.12e %.6f %.6f %.12e %.12e 0" % (
+                x, y, z, g["strike"][k], g["dip"][k], g["l"][k], g["w"][k])
+            if g["l"][k] < 0 and g["w"][k] < 0:
+                line += " " + " ".join("%.17e" % v for v in g["vert"][k])
+            f.write(line + "\n")
+
+
+def triangles(tmp):
+    sys.path.insert(0, ROOT)
+    from interact_amd import faults
+    rng = np.random.default_rng(4242)
+    sets = {}
+    sets["planar"] = faults.planar_tri_grid(2.0, 12.0, 6.0)
+    sets["rough"] = faults.planar_tri_grid(
+        1.0, 8.0, 5.0, offset=faults.rough_offset(8, 5, 1.0, alpha=2e-2, seed=7))
+    m = 30
+    c = np.stack([rng.uniform(-4e3, 4e3, m), rng.uniform(-4e3, 4e3, m),
+                  rng.uniform(-6e3, -1.2e3, m)], 1)
+    vert = np.repeat(c, 3, axis=0).reshape(m, 9) + rng.uniform(-900, 900, (m, 9))
+    vert[:, 2::3] = np.minimum(vert[:, 2::3], -50.0)
+    vert[0] = [0, 0, 0, 800, 0, 0, 300, 0, -900]           # touches the surface
+    vert[1] = [1500, 1500, -2000, 2500, 1500, -2000, 1500, 2600, -2000]   # horizontal
+    vert[2] = [-2500, 0, -1000, -2500, 0, -2200, -1700, 600, -1500]       # vertical edge
+    sets["random"] = dict(xyz=(vert[:, 0:3] + vert[:, 3:6] + vert[:, 6:9]) / 3.0,
+                          strike=np.zeros(m), dip=np.full(m, 90.0), l=np.full(m, -1.0),
+                          w=np.full(m, -1.0), vert=vert)
+    mr, mt = 10, 10
+    xyz = np.stack([rng.uniform(-4e3, 4e3, mr), rng.uniform(-4e3, 4e3, mr),
+                    rng.uniform(-6e3, -1.5e3, mr)], 1)
+    mixed = dict(xyz=np.concatenate([xyz, sets["random"]["xyz"][3:3 + mt]]),
+                 strike=np.concatenate([np.round(rng.uniform(0, 360, mr), 3), np.zeros(mt)]),
+                 dip=np.concatenate([np.round(rng.uniform(5, 90, mr), 3), np.full(mt, 90.0)]),
+                 l=np.concatenate([rng.uniform(150, 700, mr), np.full(mt, -1.0)]),
+                 w=np.concatenate([rng.uniform(150, 500, mr), np.full(mt, -1.0)]),
+                 vert=np.concatenate([np.zeros((mr, 9)), sets["random"]["vert"][3:3 + mt]]))
+    sets["mixed"] = mixed
+    out = {}
+    for name, g in sets.items():
+        gfile = os.path.join(tmp, name + ".in")
+        write_mesh(gfile, g)
+        T, ir = kmat(gfile, 0)
+        out[name + "_geom"] = open(gfile).read()
+        out[name + "_T"], out[name + "_iret"] = T, ir
+        if name == "mixed":
+            Tf, irf = kmat(gfile, 1)
+            out[name + "_Tfull"], out[name + "_iretfull"] = Tf, irf
+    np.savez_compressed(os.path.join(HERE, "ktensor_tri.npz"), **out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--only", choices=["tri"], default=None,
+                    help="write only one group of fixtures")
     args = ap.parse_args()
     ref = args.ref
     if not os.path.exists(os.path.join(REFBIN, "ref_kgen")):
         sys.exit("build oracle/_ref first: make -f oracle/Makefile.ref")
     tmp = tempfile.mkdtemp()
+    if args.only == "tri":
+        triangles(tmp)
+        shutil.rmtree(tmp)
+        print("triangle fixtures written to", HERE)
+        return
 
     # 1. the full_vs_half mesh (scaling_tests/full_vs_half_test.sh:85-86)
     mf = os.path.join(REFBIN, "makefault")
@@ -158,6 +226,8 @@ def main():
                     os.path.join(rr, "geom_slider10.in"))
     shutil.copyfile(os.path.join(ref, "scaling_tests", "full_vs_half.csv"),
                     os.path.join(rr, "full_vs_half.csv"))
+    # 6. triangle meshes
+    triangles(tmp)
     shutil.rmtree(tmp)
     print("golden vectors written to", HERE)
 

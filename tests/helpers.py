@@ -23,6 +23,29 @@ def geom_from_text(txt):
         os.unlink(name)
 
 
+def mesh_from_text(txt):
+    """Parse a patch file with rectangles and triangles (l, w < 0 followed
+    by nine vertex coordinates, src/read_geometry.c:148-175) into a mesh
+    dict: xyz, strike, dip (float32), l, w, vert (zeros for rectangles)."""
+    rows = [line.split() for line in str(txt).strip().split("\n") if line.strip()]
+    return dict(
+        xyz=np.array([[float(r[0]), float(r[1]), float(r[2])] for r in rows]),
+        strike=np.array([np.float32(r[3]) for r in rows]),
+        dip=np.array([np.float32(r[4]) for r in rows]),
+        l=np.array([float(r[5]) for r in rows]), w=np.array([float(r[6]) for r in rows]),
+        vert=np.array([[float(v) for v in r[8:17]] if len(r) >= 17 else [0.0] * 9
+                       for r in rows]))
+
+
+def tri_golden(name):
+    """(mesh, T, iret[, T_full, iret_full]) of a triangle fixture set."""
+    d = np.load(os.path.join(GOLDEN, "ktensor_tri.npz"))
+    out = [mesh_from_text(d[name + "_geom"]), d[name + "_T"], d[name + "_iret"]]
+    if name + "_Tfull" in d.files:
+        out += [d[name + "_Tfull"], d[name + "_iretfull"]]
+    return out
+
+
 def fvh_geometry():
     with gzip.open(os.path.join(GOLDEN, "geom_fvh.in.gz"), "rt") as f:
         return geom_from_text(f.read())

@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from conftest import GOLDEN
-from helpers import (SEC_PER_YEAR, bp5_2km, first_recurrence, fvh_geometry,
+from helpers import (SEC_PER_YEAR, bp5_2km, first_recurrence, fvh_geometry, tri_golden,
                      geom_from_text, initial_state, row_rel, strike_operator,
                      vector_operator)
 
@@ -84,6 +84,51 @@ def test_traction_pairs_random_rectangles(full):
     assert row_rel(T, T0) < 1e-9
     big = np.abs(T0) > 1e-3 * np.abs(T0).max(axis=(1, 2, 3))[:, None, None, None]
     assert (np.abs(T - T0)[big] / np.abs(T0[big])).max() < 1e-9
+    fs.close()
+
+
+@pytest.mark.parametrize("name,tol", [("planar", 1e-12), ("random", 1e-12),
+                                      ("mixed", 1e-10), ("rough", 5e-9)])
+def test_traction_pairs_triangles(name, tol):
+    """Nikkhoo-Walter triangle sources and TDCS receiver frames on the
+    device vs the reference-built golden tensors (tests/golden/ktensor_tri.npz)
+    and vs the oracle; the mixed mesh also in full space (rectangle sources
+    full space, triangle sources always half space).  The rough grid's
+    near-coplanar pairs are conditioned at ~1e-9 of the row scale (see the
+    oracle test of the same fixtures); the mixed mesh's random rectangles
+    include near-edge pairs, where the device Okada path (gradient-only,
+    reciprocals) differs from the reference's operation order at ~1e-12."""
+    sets = tri_golden(name)
+    g, T0, ir0 = sets[:3]
+    m = T0.shape[0]
+    fs = _fs(g)
+    r, s = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    for k, full in enumerate((0, 1) if len(sets) > 3 else (0,)):
+        Tref = sets[3] if full else T0
+        T = fs.traction_pairs(r.ravel(), s.ravel(), full_space=bool(full)).reshape(m, m, 2, 3)
+        assert row_rel(T, Tref) <= tol
+        To = oracle.mesh_traction_tensor(g, full, 8)
+        assert row_rel(T, To) <= tol
+    fs.close()
+
+
+def test_assembly_triangles_vector_mode():
+    """Assembled two-component shear operator of the planar triangle grid
+    (with the normal rows present in the layout) = scaled golden tractions;
+    the FP32 copy is the rounded FP64 one; the self entries unload the
+    patch."""
+    L = _L()
+    g, T0, _ = tri_golden("planar")
+    fs = _fs(g)
+    scale = G_BP5 / 1e4
+    fs.assemble(L.SLIP_VECTOR, shear_modulus=G_BP5, with_normal=True, store=L.PREC_FP32)
+    K = fs.dense()
+    Kref = vector_operator(T0, scale)
+    assert row_rel(K, Kref) <= 1e-12
+    assert (np.diag(K) < 0).all()
+    x = np.sin(np.arange(K.shape[1]) + 1.0)
+    y32 = fs.matvec(x, L.PREC_FP32)
+    assert np.linalg.norm(y32 - K @ x) / np.linalg.norm(K @ x) < 1e-6
     fs.close()
 
 
