@@ -1,22 +1,25 @@
 """Benchmark of the dense rsf_solve hot path on one MI355X.
 
-Workload (the 1 km planar configuration): vertical strike-slip fault
-100 x 40 km at 1 km patches = 4000 patches x 2 slip components = 8000 DOF,
-dense K of 8000 x 8000; aging-law rate-and-state friction with radiation
-damping; seeded initial slip rate vpl (1 + 1e-3 N(0,1)); rk5dp with
-relative tolerance 1e-8 over 400 yr.  One bench "step" is one pass of the
-whole cycle integration over that window.
+Workload (default, the 1 km planar configuration): vertical strike-slip
+fault 100 x 40 km at 1 km patches = 4000 patches x 2 slip components =
+8000 DOF, dense K of 8000 x 8000; aging-law rate-and-state friction with
+radiation damping; seeded initial slip rate vpl (1 + 1e-3 N(0,1)); rk5dp
+with relative tolerance 1e-8 over 400 yr.  --config c3 runs the same fault
+at 0.5 km patches (16000 patches, 32000 DOF) over 200 yr.  One bench
+"step" is one pass of the whole cycle integration over that window.
 
 Reported: simulated years per wall-clock hour of the mixed-precision path
-(K stored in FP32, FP32 products summed per 16-byte slice and accumulated
-in FP64, FP64 state and error control) as
-`value`, the FP64 and bf16 hi+lo paths beside it, the error of each against
-the FP64 path at matched events (first-recurrence time, peak slip rate)
-and in K v, the roofline of the fused stage kernel (K v + RHS epilogue)
-measured with HIP events on its stream, operator assembly throughput, and
-the CPU oracle timed on the host cores on a bounded sample.
+as `value` (K as bf16 hi for every entry plus the bf16 lo of each patch's
+near-field column chunks, FP32 products summed per 16-byte slice and
+accumulated in FP64, FP64 state and error control); the FP64 path, FP32 and
+bf16 hi+lo storage (and bf16 hi alone on c2) beside it with the error of
+each against the FP64 path at matched events (first-recurrence time, peak
+slip rate) and in K v; the roofline of the headline fused stage kernel
+(K v + RHS epilogue) measured with HIP events on its stream; operator
+assembly throughput; and the CPU oracle timed on the host cores on a
+bounded sample.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--years 400]
+    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--config c2|c3]
 """
 import argparse
 import json
@@ -35,7 +38,19 @@ HBM_PEAK_GBS = 8000.0
 # workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors that a plain
 # 256 MiB device copy reads on the same counters
 # (profiles/pmc_traffic_planar1km_8000dof.md)
-STAGE_TRAFFIC_1KM = {"fp64": 517.3e6, "fp32": 261.8e6, "bf16x2": 261.4e6, "bf16": 131.9e6}
+STAGE_TRAFFIC = {"c2": {"fp64": 517.3e6, "fp32": 261.8e6, "bf16x2": 261.4e6, "bf16": 131.9e6},
+                 "c3": {}}
+# the mixed path reported as value: bf16 hi for every entry plus the bf16 lo
+# of each patch's near-field column chunks (compensated where it matters)
+HEADLINE = "bf16nf"
+STAGE_TYPES = {"fp64": "double", "fp32": "float", "bf16x2": "bf16x2_t", "bf16": "bf16_t",
+               "bf16nf": "bf16nf_t"}
+DTYPES = {"fp64": "f64 K.v + f64 state",
+          "fp32": "f32 K (f32 slice products, f64 accumulation) + f64 state",
+          "bf16x2": "bf16 hi+lo K (f32 slice products, f64 accumulation) + f64 state",
+          "bf16": "bf16 K (f32 slice products, f64 accumulation) + f64 state",
+          "bf16nf": "bf16 hi K + bf16 lo in near-field chunks (f32 slice products, f64 "
+                    "accumulation) + f64 state"}
 # algorithmic FP64 operations per (receiver, source) pair of the Okada path
 # (two DC3D evaluations + stress + tractions, half space, vertical coplanar
 # patches), counted on the oracle by oracle/opcount.cpp
@@ -135,17 +150,27 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--years", type=float, default=400.0)
+    ap.add_argument("--config", choices=["c2", "c3"], default="c2",
+                    help="c2: 1 km patches, 8000 DOF, 400 yr (default); c3: 0.5 km patches, "
+                         "32000 DOF, 200 yr")
+    ap.add_argument("--years", type=float, default=None)
     ap.add_argument("--rtol", type=float, default=1e-8)
-    ap.add_argument("--ds", type=float, default=1.0)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precisions", default="fp64,fp32,bf16x2")
+    ap.add_argument("--precisions", default=None,
+                    help="cycle runs, comma separated (fp64 first is the reference); "
+                         "default fp64,bf16nf,fp32,bf16x2,bf16 (c3: fp64,bf16nf,fp32)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one assembly, a few launches of the stage and matvec kernels "
                          "per precision and a 256 MiB device copy of known size, for "
                          "hardware-counter runs; no cycle, no JSON bench line")
     args = ap.parse_args()
+    args.ds = {"c2": 1.0, "c3": 0.5}[args.config]
+    if args.years is None:
+        args.years = {"c2": 400.0, "c3": 200.0}[args.config]
+    if args.precisions is None:
+        args.precisions = ("fp64,bf16nf,fp32,bf16x2,bf16" if args.config == "c2"
+                           else "fp64,bf16nf,fp32")
     if args.gpus != 1:
         print(json.dumps({"error": "runs on one GPU only: the row-block sharding over "
                           "several GPUs is not built", "n_gpus": args.gpus}))
@@ -158,7 +183,7 @@ def main():
     f = faults.cycle_fields(g)
     n = len(g["l"])
     D = 2 * n
-    store = lib.PREC_FP32 | lib.PREC_BF16X2 | lib.PREC_BF16
+    store = lib.PREC_FP32 | lib.PREC_BF16X2 | lib.PREC_BF16 | lib.PREC_BF16NF
     fs = FaultSystem(g)
     asm_ms = fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"], store=store)
     asm_ms = fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"], store=store)
@@ -177,12 +202,12 @@ def main():
     xa = np.sin(i + 1.0) + 0.3 * np.cos(3.0 * i)
     y64 = fs.matvec(xa, lib.PREC_FP64)
     kv = {}
-    for name in ("fp64", "fp32", "bf16x2", "bf16"):
+    for name in ("fp64", "fp32", "bf16x2", "bf16", "bf16nf"):
         p = lib.PREC_BY_NAME[name]
         y = fs.matvec(xa, p)
         mv_ms = fs.matvec_bench(p, 50)
         st_ms = fs.stage_bench(p, 50)
-        bytes_alg = D * D * lib.PREC_BYTES[p] + 16.0 * D
+        bytes_alg = fs.operator_bytes(p) + 16.0 * D
         kv[name] = dict(relerr=float(np.linalg.norm(y - y64) / np.linalg.norm(y64)),
                         matvec_ms=mv_ms, stage_ms=st_ms,
                         matvec_gbs=bytes_alg / mv_ms / 1e6, stage_gbs=bytes_alg / st_ms / 1e6,
@@ -194,7 +219,7 @@ def main():
     runs = {}
     for name in args.precisions.split(","):
         p = lib.PREC_BY_NAME[name]
-        headline = (name == "fp32")
+        headline = (name == HEADLINE)
         nrep = args.steps if headline else 1
         if headline:
             for _ in range(args.warmup):
@@ -224,10 +249,11 @@ def main():
             e["peak_slip_rate_rel"] = (r["peak_slip_rate"] - ref["peak_slip_rate"]) / ref["peak_slip_rate"]
         errors[name] = e
 
-    head = runs.get("fp32") or next(iter(runs.values()))
-    stg = kv["fp32"]
-    traffic = STAGE_TRAFFIC_1KM["fp32"] if args.ds == 1.0 else None
-    roof = dict(kernel="k_stage<float,2,2> (fused K v + RHS epilogue)", bound="hbm",
+    hname = HEADLINE if HEADLINE in runs else next(iter(runs))
+    head = runs[hname]
+    stg = kv[hname]
+    traffic = STAGE_TRAFFIC[args.config].get(hname)
+    roof = dict(kernel=f"k_stage<{STAGE_TYPES[hname]},2,2> (fused K v + RHS epilogue)", bound="hbm",
                 achieved=stg["stage_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=stg["stage_gbs"] / HBM_PEAK_GBS, traffic=traffic,
                 bytes_per_launch=stg["bytes"], launch_us=stg["stage_ms"] * 1e3)
@@ -242,13 +268,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 K.v (f32 acc) + f64 state",
+        "dtype": DTYPES[hname],
         "data": "synthetic (seeded fault and friction fields)",
-        "config": {"workload": f"planar strike-slip fault 100x40 km, {args.ds:g} km patches, "
-                               f"{n} patches x 2 slip components = {D} DOF, dense K {D}x{D}, "
-                               f"{args.years:g} yr, rk5dp rtol {args.rtol:g}",
-                   "parallelism": "1 GPU"},
+        "config": {"workload": f"{args.config}: planar strike-slip fault 100x40 km, {args.ds:g} km "
+                               f"patches, {n} patches x 2 slip components = {D} DOF, dense K "
+                               f"{D}x{D}, {args.years:g} yr, rk5dp rtol {args.rtol:g}",
+                   "headline_precision": hname, "parallelism": "1 GPU"},
         "speedup_mixed_vs_fp64": (head["sim_years_per_hour"] / ref["sim_years_per_hour"]) if ref else None,
+        "speedup_vs_fp64": ({k: r["sim_years_per_hour"] / ref["sim_years_per_hour"]
+                             for k, r in runs.items()} if ref else None),
         "runs": runs,
         "kv": kv,
         "errors_vs_fp64": errors,

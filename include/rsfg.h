@@ -49,6 +49,12 @@ extern "C" {
 #define RSFG_PREC_BF16X2 4   /* bf16 hi + bf16 lo pair (hi+lo = FP32 value
                                 rounded to 16 significant bits) */
 #define RSFG_PREC_BF16 8     /* bf16 hi only */
+#define RSFG_PREC_BF16NF 16  /* bf16 hi for every entry plus the bf16 lo of the
+                                BF16X2 pair in each patch's near-field column
+                                chunks (512 columns), chosen per patch so that
+                                the chunks left without lo hold at most a
+                                fraction nf_tol^2 of the patch rows' energy
+                                (sum of squared entries) */
 
 /* time integrators (PETSc -ts_rk_type 5dp / 3bs) */
 #define RSFG_RK5DP 0
@@ -108,12 +114,19 @@ typedef struct {
   int full_space;       /* 0 half space (default), 1 real-source term only */
   int store;            /* RSFG_PREC_* mask (FP64 always kept) */
   double scale;
+  double nf_tol;        /* RSFG_PREC_BF16NF energy tolerance (<= 0: 2^-6) */
 } rsfg_assembly;
 
 int rsfg_assemble(rsfg_handle *h, const rsfg_assembly *a, double *gpu_ms);
 
 /* operator dimensions after assembly: rows (shear), cols, leading dim */
 int rsfg_operator_dims(rsfg_handle *h, int *rows, int *cols, int *ld);
+
+/* bytes of the stored operator of precision prec that one product reads
+   (all rows, shear and normal; RSFG_PREC_BF16NF includes its lo blocks).
+   No reference counterpart: the algorithmic traffic of the product, for
+   measurement. */
+int rsfg_operator_bytes(rsfg_handle *h, int prec, double *bytes);
 
 /* GenKEntries_petsc semantics (src/interact_matrix_assembly.c:102): the
    m x k block of the assembled FP64 shear operator with rows[] and cols[]

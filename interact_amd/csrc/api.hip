@@ -90,7 +90,8 @@ const void* operator_ptr(rsfg_ctx* h, int prec) {
     case RSFG_PREC_FP64: return h->K64;
     case RSFG_PREC_FP32: return h->K32;
     case RSFG_PREC_BF16X2: return h->Kbx2;
-    case RSFG_PREC_BF16: return h->Kb16;
+    case RSFG_PREC_BF16: return (h->store & RSFG_PREC_BF16) ? h->Kb16 : nullptr;
+    case RSFG_PREC_BF16NF: return h->Knf_lo || h->nf_off ? h->Kb16 : nullptr;
     default: return nullptr;
   }
 }
@@ -104,10 +105,17 @@ static void free_operator(rsfg_ctx* h) {
   hipFree(h->K32);
   hipFree(h->Kbx2);
   hipFree(h->Kb16);
+  hipFree(h->Knf_lo);
+  hipFree(h->nf_off);
+  hipFree(h->nf_list);
   h->K64 = nullptr;
   h->K32 = nullptr;
   h->Kbx2 = nullptr;
   h->Kb16 = nullptr;
+  h->Knf_lo = nullptr;
+  h->nf_off = nullptr;
+  h->nf_list = nullptr;
+  h->nf_chunks = 0;
   h->assembled = false;
 }
 
@@ -168,6 +176,74 @@ __global__ __launch_bounds__(256) void k_matvec(const TK* __restrict__ K, long l
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (row0 + r < rows) y[row0 + r] = s[r];
+  }
+}
+
+// bf16 + near-field lo product: one patch (its G rows) per wave
+template <int G>
+__global__ __launch_bounds__(256) void k_matvec_nf(const bf16_t* __restrict__ Khi,
+                                                   const bf16_t* __restrict__ lo,
+                                                   const int* __restrict__ off,
+                                                   const uint16_t* __restrict__ list, long ld,
+                                                   int n, int cols, const float* __restrict__ x,
+                                                   double* __restrict__ y) {
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int pc = min(p, n - 1);
+  const int o = off[pc];
+  double s[G];
+  gemv_nf_wave<G>(Khi, lo + (size_t)o * G * kNfChunk, list + o, off[pc + 1] - o, ld, pc * G,
+                  n * G, cols, x, s);
+  if ((threadIdx.x & 63) == 0 && p < n) {
+#pragma unroll
+    for (int r = 0; r < G; ++r) y[p * G + r] = s[r];
+  }
+}
+
+// energy (sum of squares) of the G rows of patch p in column chunk c of
+// 512 entries: one wave per (p, c)
+__global__ __launch_bounds__(256) void k_chunk_energy(const double* __restrict__ K, long ld,
+                                                      int n, int G, int cols, int nch,
+                                                      double* __restrict__ E) {
+  const long w = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (w >= (long)n * nch) return;
+  const int p = (int)(w / nch), c = (int)(w % nch);
+  const int l8 = (threadIdx.x & 63) * 8;
+  double s = 0.0;
+  for (int r = 0; r < G; ++r) {
+    const double* row = K + (size_t)(p * G + r) * ld;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int col = c * kNfChunk + l8 + e;
+      const double v = (col < cols) ? row[col] : 0.0;
+      s = fma(v, v, s);
+    }
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) E[w] = s;
+}
+
+// lo blocks of the listed chunks: lo = RN-bf16 of (K - hi), hi = RN-bf16 of
+// the FP32 value (the pair stored by the bf16 hi+lo format); one wave per
+// listed chunk q (owner[q] = its patch)
+__global__ __launch_bounds__(256) void k_fill_nf_lo(const double* __restrict__ K, long ld, int G,
+                                                    int cols, long nq,
+                                                    const int* __restrict__ owner,
+                                                    const uint16_t* __restrict__ list,
+                                                    bf16_t* __restrict__ lo) {
+  const long q = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= nq) return;
+  const int p = owner[q], c = list[q];
+  const int l8 = (threadIdx.x & 63) * 8;
+  for (int r = 0; r < G; ++r) {
+    const double* row = K + (size_t)(p * G + r) * ld;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int col = c * kNfChunk + l8 + e;
+      const double v = (col < cols) ? row[col] : 0.0;
+      const uint32_t hi = bf16_rn(static_cast<float>(v));
+      const double rem = v - static_cast<double>(__uint_as_float(hi << 16));
+      lo[((size_t)q * G + r) * kNfChunk + l8 + e] = static_cast<bf16_t>(bf16_rn(static_cast<float>(rem)));
+    }
   }
 }
 
@@ -288,6 +364,19 @@ static int matvec_any(rsfg_ctx* h, int prec, const void* d_xvt, double* d_y) {
     case RSFG_PREC_FP32: return matvec_launch<float>(h, h->K32, d_xvt, d_y);
     case RSFG_PREC_BF16X2: return matvec_launch<bf16x2_t>(h, h->Kbx2, d_xvt, d_y);
     case RSFG_PREC_BF16: return matvec_launch<bf16_t>(h, h->Kb16, d_xvt, d_y);
+    case RSFG_PREC_BF16NF: {
+      const dim3 grid((h->n + 3) / 4), block(256);
+      const float* x = static_cast<const float*>(d_xvt);
+#define RSFG_NF(GG)                                                                            \
+  hipLaunchKernelGGL(k_matvec_nf<GG>, grid, block, 0, h->stream, h->Kb16, h->Knf_lo, h->nf_off, \
+                     h->nf_list, (long)h->ld, h->n, h->cols, x, d_y)
+      if (h->G == 1) RSFG_NF(1);
+      else if (h->G == 2) RSFG_NF(2);
+      else RSFG_NF(3);
+#undef RSFG_NF
+      RSFG_HIP(hipGetLastError());
+      return RSFG_OK;
+    }
   }
   set_error("matvec: bad precision");
   return RSFG_EINVAL;
@@ -308,6 +397,67 @@ static int stage_input(rsfg_ctx* h, int prec, const double* d_x, void* dst) {
 
 static int prec_ok(rsfg_ctx* h, int prec) {
   return operator_ptr(h, prec) != nullptr;
+}
+
+// Near-field chunk lists of the bf16 + lo format from the assembled FP64
+// operator: per patch, chunks in decreasing energy until the chunks left
+// out hold at most tol^2 of the patch rows' energy; then the lo blocks.
+static int build_nf(rsfg_ctx* h, double tol) {
+  const int n = h->n, G = h->G, cols = h->cols;
+  const int nch = (cols + kNfChunk - 1) / kNfChunk;
+  RSFG_CHECK(nch <= 65536, RSFG_EINVAL, "rsfg_assemble: too many columns for the nf format");
+  const size_t ne = (size_t)n * nch;
+  int rc = ensure_tmp(h, ne * sizeof(double));
+  if (rc) return rc;
+  const long waves = (long)ne;
+  hipLaunchKernelGGL(k_chunk_energy, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, h->stream,
+                     h->K64, (long)h->ld, n, G, cols, nch, h->d_tmp);
+  RSFG_HIP(hipGetLastError());
+  std::vector<double> E(ne);
+  RSFG_HIP(hipMemcpyAsync(E.data(), h->d_tmp, ne * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+  RSFG_HIP(hipStreamSynchronize(h->stream));
+  std::vector<int> off(n + 1, 0), owner;
+  std::vector<uint16_t> list;
+  std::vector<int> order(nch);
+  for (int p = 0; p < n; ++p) {
+    const double* e = E.data() + (size_t)p * nch;
+    double total = 0.0;
+    for (int c = 0; c < nch; ++c) total += e[c];
+    for (int c = 0; c < nch; ++c) order[c] = c;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return e[a] > e[b]; });
+    double left = total;
+    std::vector<uint16_t> mine;
+    for (int k = 0; k < nch && left > tol * tol * total; ++k) {
+      mine.push_back((uint16_t)order[k]);
+      left -= e[order[k]];
+    }
+    std::sort(mine.begin(), mine.end());
+    for (uint16_t c : mine) {
+      list.push_back(c);
+      owner.push_back(p);
+    }
+    off[p + 1] = (int)list.size();
+  }
+  h->nf_chunks = (long)list.size();
+  RSFG_HIP(hipMalloc(&h->nf_off, sizeof(int) * (n + 1)));
+  RSFG_HIP(hipMemcpy(h->nf_off, off.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+  const size_t nq = std::max<size_t>(1, list.size());
+  RSFG_HIP(hipMalloc(&h->nf_list, sizeof(uint16_t) * nq));
+  RSFG_HIP(hipMalloc(&h->Knf_lo, sizeof(bf16_t) * nq * G * kNfChunk));
+  if (!list.empty()) {
+    RSFG_HIP(hipMemcpy(h->nf_list, list.data(), sizeof(uint16_t) * list.size(),
+                       hipMemcpyHostToDevice));
+    rc = ensure_tmp(h, sizeof(int) * owner.size());
+    if (rc) return rc;
+    RSFG_HIP(hipMemcpy(h->d_tmp, owner.data(), sizeof(int) * owner.size(), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_fill_nf_lo, dim3((unsigned)((list.size() + 3) / 4)), dim3(256), 0,
+                       h->stream, h->K64, (long)h->ld, G, cols, (long)list.size(),
+                       reinterpret_cast<const int*>(h->d_tmp), h->nf_list, h->Knf_lo);
+    RSFG_HIP(hipGetLastError());
+    RSFG_HIP(hipStreamSynchronize(h->stream));
+  }
+  return RSFG_OK;
 }
 
 }  // namespace rsfg
@@ -498,12 +648,17 @@ int rsfg_assemble(rsfg_handle* h, const rsfg_assembly* a, double* gpu_ms) {
     RSFG_HIP(hipMalloc(&h->Kbx2, ne * sizeof(bf16x2_t)));
     RSFG_HIP(hipMemsetAsync(h->Kbx2, 0, ne * sizeof(bf16x2_t), h->stream));
   }
-  if (h->store & RSFG_PREC_BF16) {
+  if (h->store & (RSFG_PREC_BF16 | RSFG_PREC_BF16NF)) {   // the nf format streams the hi part
     RSFG_HIP(hipMalloc(&h->Kb16, ne * sizeof(bf16_t)));
     RSFG_HIP(hipMemsetAsync(h->Kb16, 0, ne * sizeof(bf16_t), h->stream));
   }
   int rc = launch_assemble(h, gpu_ms);
   if (rc != RSFG_OK) return rc;
+  if (h->store & RSFG_PREC_BF16NF) {
+    h->nf_tol = a->nf_tol > 0.0 ? a->nf_tol : 1.0 / 64.0;
+    rc = build_nf(h, h->nf_tol);
+    if (rc != RSFG_OK) return rc;
+  }
   h->assembled = true;
   h->have_loading = false;
   h->have_state = false;
@@ -516,6 +671,20 @@ int rsfg_operator_dims(rsfg_handle* h, int* rows, int* cols, int* ld) {
   if (rows) *rows = h->ncomp * h->n;
   if (cols) *cols = h->cols;
   if (ld) *ld = h->ld;
+  return RSFG_OK;
+}
+
+int rsfg_operator_bytes(rsfg_handle* h, int prec, double* bytes) {
+  RSFG_CHECK(h && bytes, RSFG_EINVAL, "rsfg_operator_bytes: null argument");
+  RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_operator_bytes: precision not assembled");
+  const double ent = (double)h->rows_all * h->cols;
+  switch (prec) {
+    case RSFG_PREC_FP64: *bytes = 8.0 * ent; break;
+    case RSFG_PREC_FP32:
+    case RSFG_PREC_BF16X2: *bytes = 4.0 * ent; break;
+    case RSFG_PREC_BF16: *bytes = 2.0 * ent; break;
+    default: *bytes = 2.0 * ent + 2.0 * (double)h->nf_chunks * h->G * kNfChunk; break;
+  }
   return RSFG_OK;
 }
 
@@ -632,6 +801,8 @@ int rsfg_matvec_t(rsfg_handle* h, int prec, const double* x, double* y) {
   RSFG_CHECK(h && x && y, RSFG_EINVAL, "rsfg_matvec_t: null argument");
   RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_matvec_t: precision not assembled");
   RSFG_CHECK(h->G == h->ncomp, RSFG_EINVAL, "rsfg_matvec_t: operator holds normal rows");
+  RSFG_CHECK(prec != RSFG_PREC_BF16NF, RSFG_EINVAL,
+             "rsfg_matvec_t: not available for the near-field bf16 format");
   RSFG_HIP(hipSetDevice(h->device));
   const int rows = h->rows_all, cols = h->cols;
   const int nsplit = std::max(1, std::min(64, rows / 64));

@@ -25,11 +25,6 @@ struct AsmArgs {
   bf16_t* Kb16;
 };
 
-__device__ __forceinline__ uint32_t bf16_bits_rn(float f) {
-  uint32_t u = __float_as_uint(f);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return u >> 16;
-}
 
 // Unscaled tractions at receiver i from unit strike (k = 0) and dip
 // (k = 1) slip on source j: T[k][0..2] = strike, dip, normal traction on
@@ -145,11 +140,11 @@ __device__ __forceinline__ void store_entry(const AsmArgs& a, size_t idx,
   a.K64[idx] = v;
   if (a.K32) a.K32[idx] = static_cast<float>(v);
   if (a.Kbx2 || a.Kb16) {
-    const uint32_t hi = bf16_bits_rn(static_cast<float>(v));
+    const uint32_t hi = bf16_rn(static_cast<float>(v));
     if (a.Kb16) a.Kb16[idx] = static_cast<bf16_t>(hi);
     if (a.Kbx2) {
       const double rem = v - static_cast<double>(__uint_as_float(hi << 16));
-      const uint32_t lo = bf16_bits_rn(static_cast<float>(rem));
+      const uint32_t lo = bf16_rn(static_cast<float>(rem));
       a.Kbx2[idx] = (hi << 16) | lo;
     }
   }

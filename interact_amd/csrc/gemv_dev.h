@@ -159,15 +159,14 @@ struct GemvUnroll<bf16_t, R> {
   static constexpr int U = (R == 1) ? 4 : 2;
 };
 
-// Row sums of rows [row0, row0 + R) computed by ONE wave over all columns
-// (rows past nrows - 1 are clamped to it; the caller discards them).  Every
-// lane holds all R sums on return.  No LDS, no barrier: the waves of a
-// workgroup run and finish independently.
+// Lane partial sums of rows [row0, row0 + R) over all columns, streamed by
+// ONE wave (rows past nrows - 1 are clamped to it; the caller discards
+// them), added into acc.
 template <typename TK, int R>
-__device__ __forceinline__ void gemv_wave(const TK* __restrict__ K, long ld, int row0, int nrows,
-                                          int cols,
-                                          const typename KTraits<TK>::VT* __restrict__ x,
-                                          double* out) {
+__device__ __forceinline__ void gemv_acc(const TK* __restrict__ K, long ld, int row0, int nrows,
+                                         int cols,
+                                         const typename KTraits<TK>::VT* __restrict__ x,
+                                         double* acc) {
   constexpr int CH = chunk_elems<TK>();
   constexpr int U = GemvUnroll<TK, R>::U;
   const int lo = (threadIdx.x & 63) * KTraits<TK>::VEC;
@@ -175,15 +174,74 @@ __device__ __forceinline__ void gemv_wave(const TK* __restrict__ K, long ld, int
   const TK* rp[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) rp[r] = K + (size_t)min(row0 + r, nrows - 1) * ld;
-  double acc[R];
-#pragma unroll
-  for (int r = 0; r < R; ++r) acc[r] = 0.0;
   int c = 0;
   for (; c + U <= nch; c += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) ChunkFma<TK, R>::run(rp, (c + u) * CH + lo, x, acc);
   }
   for (; c < nch; ++c) ChunkFma<TK, R>::run(rp, c * CH + lo, x, acc);
+}
+
+// Row sums of rows [row0, row0 + R) computed by ONE wave over all columns.
+// Every lane holds all R sums on return.  No LDS, no barrier: the waves of
+// a workgroup run and finish independently.
+template <typename TK, int R>
+__device__ __forceinline__ void gemv_wave(const TK* __restrict__ K, long ld, int row0, int nrows,
+                                          int cols,
+                                          const typename KTraits<TK>::VT* __restrict__ x,
+                                          double* out) {
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  gemv_acc<TK, R>(K, ld, row0, nrows, cols, x, acc);
+#pragma unroll
+  for (int r = 0; r < R; ++r) out[r] = wave_sum(acc[r]);
+}
+
+// ---- bf16 hi everywhere + bf16 lo in the near-field chunks ("nf")
+// The hi parts of all entries stream as the bf16 format does; for the
+// nlist column chunks of 512 entries listed for this wave's patch (sorted
+// ascending), the lo parts follow in a packed block, chunk k of row r at
+// lo[(k R + r) 512].  Entry value = hi + lo where listed, hi elsewhere.
+constexpr int kNfChunk = 512;   // = chunk_elems<bf16_t>()
+
+struct bf16nf_t {};   // tag of the format (the hi array is bf16_t)
+template <>
+struct KTraits<bf16nf_t> {
+  using VT = float;
+  static constexpr int VEC = 8;
+};
+
+template <int R>
+__device__ __forceinline__ void gemv_nf_wave(const bf16_t* __restrict__ Khi,
+                                             const bf16_t* __restrict__ lo,
+                                             const uint16_t* __restrict__ list, int nlist,
+                                             long ld, int row0, int nrows, int cols,
+                                             const float* __restrict__ x, double* out) {
+  double acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  gemv_acc<bf16_t, R>(Khi, ld, row0, nrows, cols, x, acc);
+  const int l8 = (threadIdx.x & 63) * 8;
+  for (int k = 0; k < nlist; ++k) {
+    const int col = (int)list[k] * kNfChunk + l8;
+    const v4f x0 = *reinterpret_cast<const v4f*>(x + col);
+    const v4f x1 = *reinterpret_cast<const v4f*>(x + col + 4);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const v4u kv = __builtin_nontemporal_load(
+          reinterpret_cast<const v4u*>(lo + ((size_t)k * R + r) * kNfChunk + l8));
+      float t = bf_lo(kv.x) * x0.x;
+      t = fmaf(bf_hi(kv.x), x0.y, t);
+      t = fmaf(bf_lo(kv.y), x0.z, t);
+      t = fmaf(bf_hi(kv.y), x0.w, t);
+      t = fmaf(bf_lo(kv.z), x1.x, t);
+      t = fmaf(bf_hi(kv.z), x1.y, t);
+      t = fmaf(bf_lo(kv.w), x1.z, t);
+      t = fmaf(bf_hi(kv.w), x1.w, t);
+      acc[r] += static_cast<double>(t);
+    }
+  }
 #pragma unroll
   for (int r = 0; r < R; ++r) out[r] = wave_sum(acc[r]);
 }

@@ -32,6 +32,13 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 typedef uint32_t bf16x2_t;
 typedef uint16_t bf16_t;
 
+// bf16 bits of an FP32 value, rounded to nearest even (finite values)
+__device__ __forceinline__ uint32_t bf16_rn(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
 // capacity of the device event and monitor logs (rows)
 constexpr int kMaxEv = 4096;
 constexpr int kMaxMon = 1 << 18;
@@ -102,6 +109,13 @@ struct rsfg_ctx {
   rsfg::bf16x2_t* Kbx2 = nullptr;
   rsfg::bf16_t* Kb16 = nullptr;
   size_t kslack = 0;          // extra elements allocated after the last row
+  // bf16 + near-field lo format: hi in Kb16, per patch p the chunk list
+  // nf_list[nf_off[p] .. nf_off[p+1]) and lo blocks from Knf_lo + nf_off[p] G 512
+  rsfg::bf16_t* Knf_lo = nullptr;
+  int* nf_off = nullptr;
+  uint16_t* nf_list = nullptr;
+  long nf_chunks = 0;
+  double nf_tol = 0.0;
   // friction and loading
   bool have_friction = false, have_loading = false;
   rsfg::FricParams fp{};

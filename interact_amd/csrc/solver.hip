@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "gemv_dev.h"
@@ -41,6 +42,9 @@ constexpr double kSecPerYear = 31557600.0;
 
 struct StageArgs {
   const void* K;
+  const bf16_t* Knf_lo;      // bf16 + near-field lo format: lo blocks,
+  const int* nf_off;         //   per-patch chunk offsets (n + 1)
+  const uint16_t* nf_list;   //   and chunk lists
   long ld;
   int rows_all, cols, n, G;
   const double* coef;  // a, b, dc
@@ -506,10 +510,17 @@ __global__ __launch_bounds__(64 * kStagePPB) void k_stage(StageArgs a) {
   const int n = a.n, S = a.tb.s, s = a.s;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   {
-    const int pw = blockIdx.x * kStagePPB + wave;
+    const int pw = min(blockIdx.x * kStagePPB + wave, n - 1);
     double rowsum[G];
-    gemv_wave<TK, G>(static_cast<const TK*>(a.K), a.ld, min(pw, n - 1) * G, a.rows_all,
-                     a.cols, static_cast<const VT*>(a.vin), rowsum);
+    if constexpr (std::is_same<TK, bf16nf_t>::value) {
+      const int o = a.nf_off[pw];
+      gemv_nf_wave<G>(static_cast<const bf16_t*>(a.K), a.Knf_lo + (size_t)o * G * kNfChunk,
+                      a.nf_list + o, a.nf_off[pw + 1] - o, a.ld, pw * G, a.rows_all, a.cols,
+                      static_cast<const float*>(a.vin), rowsum);
+    } else {
+      gemv_wave<TK, G>(static_cast<const TK*>(a.K), a.ld, pw * G, a.rows_all, a.cols,
+                       static_cast<const VT*>(a.vin), rowsum);
+    }
     if (lane == 0) {
 #pragma unroll
       for (int k = 0; k < G; ++k) rs[wave][k] = rowsum[k];
@@ -739,6 +750,7 @@ static Launcher* make_launcher(int prec, int ncomp, int G) {
     case RSFG_PREC_FP32: return make_launcher_t<float>(ncomp, G);
     case RSFG_PREC_BF16X2: return make_launcher_t<bf16x2_t>(ncomp, G);
     case RSFG_PREC_BF16: return make_launcher_t<bf16_t>(ncomp, G);
+    case RSFG_PREC_BF16NF: return make_launcher_t<bf16nf_t>(ncomp, G);
     default: return make_launcher_t<double>(ncomp, G);
   }
 }
@@ -747,6 +759,9 @@ static void fill_args(rsfg_ctx* h, const rsfg_solver_opts* o, const void* K, Sta
   const int n = h->n;
   memset(&a, 0, sizeof(a));
   a.K = K;
+  a.Knf_lo = h->Knf_lo;
+  a.nf_off = h->nf_off;
+  a.nf_list = h->nf_list;
   a.ld = h->ld;
   a.rows_all = h->rows_all;
   a.cols = h->cols;

@@ -17,18 +17,19 @@ ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ESTATE", -5: "ESTORM",
           -6: "EDOMAIN"}
 
 SLIP_STRIKE, SLIP_DIP, SLIP_RAKE, SLIP_VECTOR = 0, 1, 3, 8
-PREC_FP64, PREC_FP32, PREC_BF16X2, PREC_BF16 = 1, 2, 4, 8
+PREC_FP64, PREC_FP32, PREC_BF16X2, PREC_BF16, PREC_BF16NF = 1, 2, 4, 8, 16
 PREC_NAMES = {PREC_FP64: "fp64", PREC_FP32: "fp32", PREC_BF16X2: "bf16x2",
-              PREC_BF16: "bf16"}
+              PREC_BF16: "bf16", PREC_BF16NF: "bf16nf"}
 PREC_BY_NAME = {v: k for k, v in PREC_NAMES.items()}
-PREC_BYTES = {PREC_FP64: 8, PREC_FP32: 4, PREC_BF16X2: 4, PREC_BF16: 2}
+# bytes per entry; bf16nf: 2 + 2 x (fraction of chunks with lo), see FaultSystem.nf_fraction
+PREC_BYTES = {PREC_FP64: 8, PREC_FP32: 4, PREC_BF16X2: 4, PREC_BF16: 2, PREC_BF16NF: 2}
 RK5DP, RK3BS = 0, 1
 
 # every symbol include/rsfg.h declares
 EXPORTS = [
     "rsfg_last_error", "rsfg_version", "rsfg_device_count", "rsfg_create",
     "rsfg_destroy", "rsfg_read_patch_file", "rsfg_assemble",
-    "rsfg_operator_dims", "rsfg_entries", "rsfg_traction_pairs",
+    "rsfg_operator_dims", "rsfg_operator_bytes", "rsfg_entries", "rsfg_traction_pairs",
     "rsfg_matvec", "rsfg_matvec_dev", "rsfg_matvec_t", "rsfg_matvec_bench",
     "rsfg_set_friction", "rsfg_set_loading", "rsfg_rhs",
     "rsfg_solver_defaults", "rsfg_set_state", "rsfg_get_state", "rsfg_solve",
@@ -47,7 +48,7 @@ class Patch(C.Structure):
 class Assembly(C.Structure):
     _fields_ = [("slip_mode", C.c_int), ("with_normal", C.c_int),
                 ("full_space", C.c_int), ("store", C.c_int),
-                ("scale", C.c_double)]
+                ("scale", C.c_double), ("nf_tol", C.c_double)]
 
 
 class Friction(C.Structure):
@@ -107,6 +108,7 @@ def load(path=None):
     L.rsfg_read_patch_file.argtypes = [C.c_char_p, C.POINTER(Patch),
                                        C.POINTER(C.c_int)]
     L.rsfg_assemble.argtypes = [h, C.POINTER(Assembly), C.POINTER(C.c_double)]
+    L.rsfg_operator_bytes.argtypes = [h, C.c_int, C.POINTER(C.c_double)]
     L.rsfg_operator_dims.argtypes = [h, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.rsfg_entries.argtypes = [h, C.c_int, _ip, C.c_int, _ip, _dp]

@@ -58,9 +58,9 @@ class FaultSystem:
     # ------------------------------------------------------------ operator
     def assemble(self, slip_mode=L.SLIP_STRIKE, *, shear_modulus=3.204e10,
                  full_space=False, with_normal=False, store=L.PREC_FP64,
-                 scale=None):
+                 scale=None, nf_tol=0.0):
         a = L.Assembly(slip_mode, int(with_normal), int(full_space), store,
-                       shear_modulus / G_INTERNAL if scale is None else scale)
+                       shear_modulus / G_INTERNAL if scale is None else scale, nf_tol)
         ms = C.c_double()
         L.check(self._lib.rsfg_assemble(self.h, C.byref(a), C.byref(ms)),
                 "rsfg_assemble")
@@ -70,6 +70,13 @@ class FaultSystem:
         self.assembly_ms = ms.value
         self.dim = 2 * self.ncomp + 2
         return ms.value
+
+    def operator_bytes(self, prec):
+        """bytes of the stored operator that one product with precision prec
+        reads (the bf16 near-field format counts its lo blocks)"""
+        b = C.c_double()
+        L.check(self._lib.rsfg_operator_bytes(self.h, prec, C.byref(b)), "rsfg_operator_bytes")
+        return b.value
 
     def dims(self):
         r, c, ld = C.c_int(), C.c_int(), C.c_int()

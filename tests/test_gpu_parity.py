@@ -154,12 +154,12 @@ def test_matvec_precisions_and_transpose():
     from interact_amd import faults
     g = faults.planar_grid(2.0)
     fs = _fs(g)
-    fs.assemble(L.SLIP_VECTOR, shear_modulus=32e9, store=15)
+    fs.assemble(L.SLIP_VECTOR, shear_modulus=32e9, store=31)
     K = fs.dense()
     D = K.shape[0]
     i = np.arange(D)
     bound = {L.PREC_FP64: 1e-13, L.PREC_FP32: 1e-6, L.PREC_BF16X2: 5e-5,
-             L.PREC_BF16: 1e-2}
+             L.PREC_BF16: 1e-2, L.PREC_BF16NF: 1e-4}
     for x in (np.sin(i + 1.0) + 0.3 * np.cos(3.0 * i),
               np.random.default_rng(0).normal(size=D)):
         yref = K @ x
@@ -297,14 +297,40 @@ def test_mixed_precision_cycle_bp5_2km():
     """FP32 and bf16 hi+lo storage with FP64 state keep the recurrence
     within 0.05 yr of the FP64 path."""
     from interact_amd import lib
-    g, f, fs, X0 = _bp5(2.0, store=lib.PREC_FP32 | lib.PREC_BF16X2)
+    g, f, fs, X0 = _bp5(2.0, store=lib.PREC_FP32 | lib.PREC_BF16X2 | lib.PREC_BF16NF)
     t = {}
-    for prec in (lib.PREC_FP64, lib.PREC_FP32, lib.PREC_BF16X2):
+    for prec in (lib.PREC_FP64, lib.PREC_FP32, lib.PREC_BF16X2, lib.PREC_BF16NF):
         fs.set_state(X0)
         fs.solve(238 * SEC_PER_YEAR, rtol=1e-4, prec=prec)
         t[prec] = first_recurrence(fs.events())
     assert abs(t[lib.PREC_FP32] - t[lib.PREC_FP64]) < 0.05
     assert abs(t[lib.PREC_BF16X2] - t[lib.PREC_FP64]) < 0.05
+    assert abs(t[lib.PREC_BF16NF] - t[lib.PREC_FP64]) < 0.05
+    fs.close()
+
+
+def test_bf16_nearfield_format_bytes_and_limits():
+    """The near-field format stores 2 bytes per entry plus 2 per entry of
+    the chunks that get a lo part; the fraction grows as the energy
+    tolerance shrinks; with a vanishing tolerance every chunk carries lo
+    and the product reaches the bf16 hi+lo accuracy."""
+    L = _L()
+    from interact_amd import faults
+    g = faults.planar_grid(2.0)
+    fs = _fs(g)
+    D = 2 * len(g["l"])
+    x = np.sin(np.arange(D) + 1.0) + 0.3 * np.cos(3.0 * np.arange(D))
+    per_entry, err = [], []
+    for tol in (0.25, 1.0 / 64, 1e-9):
+        fs.assemble(L.SLIP_VECTOR, shear_modulus=32e9, store=L.PREC_BF16NF, nf_tol=tol)
+        b = fs.operator_bytes(L.PREC_BF16NF) / (D * D)
+        per_entry.append(b)
+        y64 = fs.matvec(x)
+        err.append(np.linalg.norm(fs.matvec(x, L.PREC_BF16NF) - y64) / np.linalg.norm(y64))
+    assert 2.0 < per_entry[0] < per_entry[1] < per_entry[2]
+    nch = -(-D // 512)   # column chunks per row; lo rows are padded to whole chunks
+    assert per_entry[2] == pytest.approx(2.0 + 2.0 * nch * 512 / D, rel=0.05)  # every chunk
+    assert err[2] < 5e-6 and err[1] < err[0]
     fs.close()
 
 
