@@ -38,7 +38,8 @@ HBM_PEAK_GBS = 8000.0
 # workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors that a plain
 # 256 MiB device copy reads on the same counters
 # (profiles/pmc_traffic_planar1km_8000dof.md)
-STAGE_TRAFFIC = {"c2": {"fp64": 517.3e6, "fp32": 261.8e6, "bf16x2": 261.4e6, "bf16": 131.9e6},
+STAGE_TRAFFIC = {"c2": {"fp64": 517.6e6, "fp32": 262.1e6, "bf16x2": 262.2e6, "bf16": 132.6e6,
+                        "bf16nf": 145.9e6},
                  "c3": {}}
 # the mixed path reported as value: bf16 hi for every entry plus the bf16 lo
 # of each patch's near-field column chunks (compensated where it matters)
@@ -123,7 +124,7 @@ def kernels_only(fs, store):
     caller), then per precision 5 stage and 5 matvec launches, then the copy."""
     import ctypes as C
     from interact_amd import lib
-    for name in ("fp64", "fp32", "bf16x2", "bf16"):
+    for name in ("fp64", "fp32", "bf16x2", "bf16", "bf16nf"):
         p = lib.PREC_BY_NAME[name]
         if p != lib.PREC_FP64 and not (store & p):
             continue

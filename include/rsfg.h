@@ -56,9 +56,13 @@ extern "C" {
                                 fraction nf_tol^2 of the patch rows' energy
                                 (sum of squared entries) */
 
-/* time integrators (PETSc -ts_rk_type 5dp / 3bs) */
+/* time integrators (PETSc -ts_rk_type 5dp / 3bs; -imex selects PETSc
+   TSARKIMEX with tableau l2, src/rsf_imex.c:257-305) */
 #define RSFG_RK5DP 0
 #define RSFG_RK3BS 1
+#define RSFG_ARKIMEX_L2 2  /* additive IMEX: stressing rates explicit, local
+                              state evolution implicit by per-cell Newton
+                              (src/rsf_imex.c); scalar slip layout only */
 
 /* one fault patch as in the reference's patch file
    "x y z strike dip L W group" (src/read_geometry.c:115-117, strike and
@@ -200,7 +204,7 @@ int rsfg_rhs(rsfg_handle *h, int prec, const double *X, double *F,
    state.  The host replays captured graphs of step attempts and only
    polls a halted flag. */
 typedef struct {
-  int method;           /* RSFG_RK5DP / RSFG_RK3BS */
+  int method;           /* RSFG_RK5DP / RSFG_RK3BS / RSFG_ARKIMEX_L2 */
   int prec;             /* operator precision used in the stages */
   double rtol;          /* scalar relative tolerance (-rtol) */
   double atol[6];       /* per state component (psi, tau, ...) */

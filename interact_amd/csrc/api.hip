@@ -137,7 +137,9 @@ int alloc_state(rsfg_ctx* h) {
   const size_t n = h->n, dim = h->dim;
   const size_t vec = dim * n;
   h->vin_elems = vin_elems_for(h->cols);
-  const size_t total = 13 * vec + 2 * h->vin_elems;
+  static_assert(PreDim<1>::v <= 12 && PreDim<2>::v <= 12, "record buffer size");
+  const size_t pre = 12 * n;  // state-only RHS records (at most 12 per patch)
+  const size_t total = 13 * vec + 2 * pre + 4 * h->vin_elems;
   RSFG_HIP(hipMalloc(&h->d_state, total * sizeof(double)));
   RSFG_HIP(hipMemsetAsync(h->d_state, 0, total * sizeof(double), h->stream));
   double* p = h->d_state;
@@ -145,7 +147,9 @@ int alloc_state(rsfg_ctx* h) {
   for (int i = 0; i < 2; ++i) { h->Yb[i] = p; p += vec; }
   for (int i = 0; i < 8; ++i) { h->Stg[i] = p; p += vec; }
   h->Xmon = p; p += vec;
+  for (int i = 0; i < 2; ++i) { h->Pb[i] = p; p += pre; }
   for (int i = 0; i < 2; ++i) { h->vin[i] = p; p += h->vin_elems; }
+  for (int i = 0; i < 2; ++i) { h->vxb[i] = p; p += h->vin_elems; }
   h->partial_count = (size_t)(n + 1) * kNRed;
   RSFG_HIP(hipMalloc(&h->partials, h->partial_count * sizeof(double)));
   RSFG_HIP(hipMalloc(&h->d_ev, (size_t)kMaxEv * 8 * sizeof(double)));

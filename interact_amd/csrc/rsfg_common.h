@@ -67,15 +67,18 @@ struct DevCtl {
   int mon_copy;         // next prep copies X into Xmon
   int rupt_update;      // next prep marks ruptured cells
   int storm;
+  int imex_g1;          // additive pair: the explicit part at X is stale
   int nevent, nmon;
   int max_event, max_mon;
 };
 
 // Butcher tableau and integrator settings (kernel argument, by value)
 struct Tableau {
-  double A[kMaxStages][kMaxStages];
+  double A[kMaxStages][kMaxStages];   // explicit tableau
   double b[kMaxStages], bh[kMaxStages];
-  int s, order;
+  double At[kMaxStages][kMaxStages];  // implicit tableau (additive pairs)
+  double d1[kMaxStages], d2[kMaxStages];  // additive pairs: dense output
+  int s, order, imex;
 };
 
 struct SolveOpts {
@@ -130,7 +133,9 @@ struct rsfg_ctx {
   double *Xb[2] = {nullptr, nullptr}, *Yb[2] = {nullptr, nullptr};
   double* Stg[8] = {nullptr};
   double* Xmon = nullptr;
+  double* Pb[2] = {nullptr, nullptr};   // state-only RHS records of stage states
   double* vin[2] = {nullptr, nullptr};  // D padded, used as double or float
+  double* vxb[2] = {nullptr, nullptr};  // additive pair: v(X) per X buffer
   size_t vin_elems = 0;
   double* partials = nullptr;
   size_t partial_count = 0;

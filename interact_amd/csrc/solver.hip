@@ -53,10 +53,13 @@ struct StageArgs {
   FricParams fp;
   double* Xb[2];
   double* Yb[2];
+  double* Pb[2];       // state-only RHS records of the stage states
+                       // (PreDim<NC>::v x n each; stage s reads Pb[s & 1])
   double* Stg[8];
   double* Xmon;
   const void* vin;
   void* vout;
+  void* vxb[2];        // additive pair: product input v(X) of each X buffer
   double* partials;
   double* ev;
   double* mon;
@@ -105,6 +108,22 @@ __device__ __forceinline__ void write_vel(const FricParams& fp, double a,
   for (int c = 0; c < NC; ++c) vout[NC * p + c] = static_cast<VT>(v[c]);
 }
 
+// the same product input from a stage state's state-only record
+template <typename VT, int NC>
+__device__ __forceinline__ void write_vel_pre(const double* P, VT* vout, int p,
+                                              const double* vpl) {
+  double v[2] = {P[0], NC == 2 ? P[1] : 0.0};
+  if (vpl) v[0] -= vpl[p];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) vout[NC * p + c] = static_cast<VT>(v[c]);
+}
+
+template <int NC>
+__device__ __forceinline__ void store_pre(const double* P, double* Pb, int n, int p) {
+#pragma unroll
+  for (int k = 0; k < PreDim<NC>::v; ++k) Pb[k * n + p] = P[k];
+}
+
 // ------------------------------------------------------------ block helpers
 __device__ __forceinline__ double red_op(int op, double v, double o) {
   return (op == 0) ? v + o : (op == 1 ? fmax(v, o) : fmin(v, o));
@@ -145,6 +164,19 @@ __device__ void dense_patch(const StageArgs& a, const double* X0, const double* 
                             double* out) {
   constexpr int D = Dim<NC>::v;
   const int n = a.n;
+  if (a.tb.imex) {
+    // additive pair: X0 + hs sum_j (d1_j th + d2_j th^2) (G_j + Fimpl_j),
+    // G_j in Kp[j], Fimpl_j in stage slot 3 + j
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+      const int i = f * n + p;
+      double acc = 0.0;
+      for (int j = 0; j < 3; ++j)
+        acc += (a.tb.d1[j] * th + a.tb.d2[j] * th * th) * (Kp[j][i] + a.Stg[3 + j][i]);
+      out[f] = X0[i] + hs * acc;
+    }
+    return;
+  }
   if (a.tb.s == 7) {
     const double d1 = -12715105075.0 / 11282082432.0, d3 = 87487479700.0 / 32700410799.0,
                  d4 = -10690763975.0 / 1880347072.0, d5 = 701980252875.0 / 199316789632.0,
@@ -227,7 +259,9 @@ __device__ void controller(const StageArgs& a, const double* red, DevCtl* c) {
   }
   // ---- accept / reject (thread 0), PETSc TSAdapt basic
   if (tid == 0) {
-    c->rhs_evals += S - 1;
+    // product evaluations of the attempt (additive pair: stages 2 and 3,
+    // plus stage 1 after an accepted step)
+    c->rhs_evals += a.tb.imex ? 2 + c->imex_g1 : S - 1;
     int acc;
     double hnew;
     if (c->stage_fail) {
@@ -269,6 +303,7 @@ __device__ void controller(const StageArgs& a, const double* red, DevCtl* c) {
     }
     s_int[0] = acc;
     s_par[0] = hnew;
+    c->imex_g1 = 0;
     c->mon_copy = 0;
     c->rupt_update = 0;
     c->stage_fail = 0;
@@ -391,9 +426,13 @@ __device__ void controller(const StageArgs& a, const double* red, DevCtl* c) {
     c->t += c->h;
     c->h = s_par[0];
     c->xcur = xo ^ 1;
-    const int k0 = c->kslot[0];
-    c->kslot[0] = c->kslot[S - 1];
-    c->kslot[S - 1] = k0;
+    if (a.tb.imex) {
+      c->imex_g1 = 1;  // stage 1 of the next step evaluates G at the new X
+    } else {
+      const int k0 = c->kslot[0];  // FSAL: K_{S-1} is K_0 of the next step
+      c->kslot[0] = c->kslot[S - 1];
+      c->kslot[S - 1] = k0;
+    }
     c->steps++;
     c->attempt_failed = 0;
     c->last_vmax = vnew;
@@ -459,7 +498,10 @@ __global__ __launch_bounds__(256) void k_prep(StageArgs a) {
     y[f] = x[f] + h * (w * K0[f * n + p]);
     Y[f * n + p] = y[f];
   }
-  write_vel<VT, NC>(a.fp, cf.a, y, static_cast<VT*>(a.vout), p, a.vpl);
+  double P[PreDim<NC>::v];
+  rsf_patch_pre<NC>(a.fp, cf, y, P);
+  store_pre<NC>(P, a.Pb[1], n, p);
+  write_vel_pre<VT, NC>(P, static_cast<VT*>(a.vout), p, a.vpl);
   if (!rsf_patch_ok<NC>(a.fp, cf, y)) atomicOr(&a.ctl->stage_fail, 1);
 }
 
@@ -501,14 +543,70 @@ __device__ __forceinline__ double wave_reduce(double v, int op) {
 }
 
 template <typename TK, int NC, int G>
-__global__ __launch_bounds__(64 * kStagePPB) void k_stage(StageArgs a) {
+__global__ __launch_bounds__(64 * kStagePPB, 4) void k_stage(StageArgs a) {
   using VT = typename KTraits<TK>::VT;
   constexpr int D = Dim<NC>::v;
+  constexpr int PD = PreDim<NC>::v;
   __shared__ double rs[kStagePPB][G];
   DevCtl* c = a.ctl;
-  if (c->halted) return;
+  // The control fields the launch needs are read with the halted flag, in
+  // one memory round trip (they share a few cache lines); the epilogue then
+  // addresses its operands without a second dependent read.  Stage slots
+  // are consecutive blocks of one allocation, so slot pointers are formed
+  // by arithmetic instead of an indexed read of the argument block.
+  const int halted = c->halted, xc = c->xcur;
+  const double hstep = c->h;
+  int ks[kMaxStages];
+#pragma unroll
+  for (int j = 0; j < kMaxStages; ++j) ks[j] = c->kslot[j];
+  if (halted) return;
+  const size_t sst = a.Stg[1] - a.Stg[0];
   const int n = a.n, S = a.tb.s, s = a.s;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = lane;
+  const int p = blockIdx.x * kStagePPB + t;
+  const double* X = xc ? a.Xb[1] : a.Xb[0];
+  double* Xn = xc ? a.Xb[0] : a.Xb[1];
+  const bool reduce = (s == 0) || (s == S - 1);
+  const bool mid = (s > 0) && (s < S - 1);
+  // Epilogue operands with addresses known at launch: the state-only
+  // record of Y_s, the loading rates, the accepted state X and the
+  // tableau-weighted sum of the earlier stage derivatives.  Wave 0 requests
+  // them before it streams its rows, so the epilogue starts on data already
+  // in registers.  (Fetching the remaining operands as well was measured
+  // slower: the two workgroups sharing a CU then stream in lockstep
+  // instead of one after the other.)
+  double P[PD], sn[NC], snn = 0.0, x0[D], ws[D];
+#pragma unroll
+  for (int k = 0; k < PD; ++k) P[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) sn[k] = 0.0;
+#pragma unroll
+  for (int f = 0; f < D; ++f) x0[f] = ws[f] = 0.0;
+  if (wave == 0 && t < kStagePPB && p < n) {
+    if (s > 0) {
+      const double* Ps = a.Pb[s & 1];
+#pragma unroll
+      for (int k = 0; k < PD; ++k) P[k] = Ps[k * n + p];
+#pragma unroll
+      for (int f = 0; f < D; ++f) x0[f] = X[f * n + p];
+      // sum_j A[s+1][j] K_j over j < s (middle stages), or
+      // sum_j bh[j] K_j over j < S - 1 (final stage)
+      const int nj = mid ? s : S - 1;
+#pragma unroll
+      for (int j = 0; j < kMaxStages; ++j) {
+        if (j >= nj) break;
+        const double w = mid ? a.tb.A[s + 1][j] : a.tb.bh[j];
+        if (w == 0.0) continue;
+        const double* Kj = a.Stg[0] + (size_t)ks[j] * sst;
+#pragma unroll
+        for (int f = 0; f < D; ++f) ws[f] += w * Kj[f * n + p];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) sn[k] = a.sinc[NC * p + k];
+    snn = a.sinc[NC * n + p];
+  }
   {
     const int pw = min(blockIdx.x * kStagePPB + wave, n - 1);
     double rowsum[G];
@@ -528,74 +626,54 @@ __global__ __launch_bounds__(64 * kStagePPB) void k_stage(StageArgs a) {
   }
   __syncthreads();
   if (wave != 0) return;
-  const int t = lane;
-  const int p = blockIdx.x * kStagePPB + t;
-  const int xc = c->xcur;
-  const double* X = a.Xb[xc];
-  double* Xn = a.Xb[xc ^ 1];
-  const bool reduce = (s == 0) || (s == S - 1);
   {
     double r9[kNRed];
     r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
     r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
     r9[RED_DX2] = 0.0;
     if (t < kStagePPB && p < n) {
-      const double* Yc = (s == 0) ? X : (s == S - 1 ? Xn : a.Yb[s & 1]);
       const PatchCoef cf = load_coef(a.coef, n, p);
-      double y[D], kf[D], td[NC], sn[NC];
+      double y[D], kf[D], td[NC];
+      if (reduce) {
+        const double* Yc = (s == 0) ? X : Xn;
 #pragma unroll
-      for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
-#pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        td[k] = rs[t][k];
-        sn[k] = a.sinc[NC * p + k];
+        for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
       }
+      if (s == 0) rsf_patch_pre<NC>(a.fp, cf, y, P);
+#pragma unroll
+      for (int k = 0; k < NC; ++k) td[k] = rs[t][k];
       const double sdn = (G > NC) ? rs[t][NC] : 0.0;
-      rsf_patch_rhs<NC>(a.fp, cf, y, td, sdn, sn, a.sinc[NC * n + p], kf);
-      double* Ks = a.Stg[c->kslot[s]];
+      rsf_patch_finish<NC>(a.fp, P, td, sdn, sn, snn, kf);
+      int kss = ks[0];
+#pragma unroll
+      for (int j = 1; j < kMaxStages; ++j) kss = (j == s) ? ks[j] : kss;
+      double* Ks = a.Stg[0] + (size_t)kss * sst;
 #pragma unroll
       for (int f = 0; f < D; ++f) Ks[f * n + p] = kf[f];
-      if (s > 0 && s < S - 1) {
-        // next stage state
-        const double h = c->h;
-        double acc[D];
-#pragma unroll
-        for (int f = 0; f < D; ++f) acc[f] = 0.0;
-        for (int j = 0; j < s; ++j) {
-          const double w = a.tb.A[s + 1][j];
-          if (w == 0.0) continue;
-          const double* Kj = a.Stg[c->kslot[j]];
-#pragma unroll
-          for (int f = 0; f < D; ++f) acc[f] += w * Kj[f * n + p];
-        }
-        {
-          const double w = a.tb.A[s + 1][s];
-#pragma unroll
-          for (int f = 0; f < D; ++f) acc[f] += w * kf[f];
-        }
+      if (mid) {
+        // next stage state Y_{s+1} = X + h (sum_{j<s} A[s+1][j] K_j + A[s+1][s] K_s)
+        // and its state-only record
+        const double w = a.tb.A[s + 1][s];
         double* Yn = (s + 1 == S - 1) ? Xn : a.Yb[(s + 1) & 1];
         double yn[D];
 #pragma unroll
         for (int f = 0; f < D; ++f) {
-          yn[f] = X[f * n + p] + h * acc[f];
+          yn[f] = x0[f] + hstep * (ws[f] + w * kf[f]);
           Yn[f * n + p] = yn[f];
         }
-        write_vel<VT, NC>(a.fp, cf.a, yn, static_cast<VT*>(a.vout), p, a.vpl);
+        double Pn[PD];
+        rsf_patch_pre<NC>(a.fp, cf, yn, Pn);
+        store_pre<NC>(Pn, a.Pb[(s + 1) & 1], n, p);
+        write_vel_pre<VT, NC>(Pn, static_cast<VT*>(a.vout), p, a.vpl);
         if (!rsf_patch_ok<NC>(a.fp, cf, yn)) atomicOr(&c->stage_fail, 1);
       } else {
         // reductions of the state y (initial state or the new solution)
         if (s == S - 1) {
-          const double h = c->h;
           double en = 0.0;
 #pragma unroll
           for (int f = 0; f < D; ++f) {
-            double e = 0.0;
-            for (int j = 0; j < S - 1; ++j) {
-              const double w = a.tb.bh[j];
-              if (w != 0.0) e += w * a.Stg[c->kslot[j]][f * n + p];
-            }
-            e += a.tb.bh[S - 1] * kf[f];
-            const double ye = X[f * n + p] + h * e;
+            const double e = ws[f] + a.tb.bh[S - 1] * kf[f];
+            const double ye = x0[f] + hstep * e;
             const double tol = a.so.atol[f] + a.so.rtol * fmax(fabs(y[f]), fabs(ye));
             if (tol > 0.0) en = fmax(en, fabs(y[f] - ye) / tol);
             const double dm = a.Xmon[f * n + p] - y[f];
@@ -603,7 +681,7 @@ __global__ __launch_bounds__(64 * kStagePPB) void k_stage(StageArgs a) {
           }
           r9[RED_ERR] = en;
         }
-        const double vm = vel_mag<NC>(a.fp, cf.a, y);
+        const double vm = pre_vmag<NC>(P);
         r9[RED_VMAX] = vm;
         r9[RED_VSUM] = vm;
         r9[RED_SLIP] = y[NC + 2];
@@ -621,6 +699,176 @@ __global__ __launch_bounds__(64 * kStagePPB) void k_stage(StageArgs a) {
         const double q = wave_reduce(r9[k], red_kind(k));
         if (t == 0) a.partials[(size_t)k * gridDim.x + blockIdx.x] = q;
       }
+    }
+  }
+}
+
+// Additive (IMEX) step of the scalar layout, one launch per stage with the
+// same wave-per-patch product as k_stage (src/rsf_imex.c with PETSc
+// TSARKIMEX l2).  Slots: G_j in Stg[j], Fimpl_j in Stg[3 + j], j = 0..2;
+// stage states Y_2, Y_3 in Yb[0], Yb[1].
+//   s = 0  initial evaluation: G_1, Fimpl_1 at X, reductions of X
+//   s = 1  G_1, Fimpl_1 at X when X is new (else kept from the rejected
+//          attempt, and the product is skipped); Newton for Y_2
+//   s = 2  G_2 at Y_2; Newton for Y_3 (initial guess Y_2)
+//   s = 3  G_3 at Y_3; X_new = X + h sum_j b_j (G_j + Fimpl_j), the
+//          embedded error, reductions of X_new and v(X_new) for the next
+//          step's stage 1
+// A cell whose Newton solve fails, or a stage state outside the domain,
+// fails the attempt (step retried at h / 4, like PETSc's failed stage).
+template <typename TK, int G>
+__global__ __launch_bounds__(64 * kStagePPB, 4) void k_imex(StageArgs a) {
+  using VT = typename KTraits<TK>::VT;
+  constexpr int D = 4;
+  __shared__ double rs[kStagePPB][G];
+  DevCtl* c = a.ctl;
+  const int halted = c->halted, xc = c->xcur, g1 = c->imex_g1;
+  const int mon_copy = c->mon_copy, rupt_update = c->rupt_update;
+  const int failed = c->stage_fail;
+  const double hstep = c->h;
+  const int n = a.n, s = a.s;
+  // nothing to do once halted, or for the later stages of an attempt whose
+  // earlier stage already failed (the controller rejects it)
+  if (halted || (failed && s >= 2)) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int t = lane;
+  const int p = blockIdx.x * kStagePPB + t;
+  const double* X = xc ? a.Xb[1] : a.Xb[0];
+  double* Xn = xc ? a.Xb[0] : a.Xb[1];
+  const bool stream = (s != 1) || g1;
+  if (stream) {
+    const void* vin = (s <= 1) ? (xc ? a.vxb[1] : a.vxb[0]) : a.vin;
+    const int pw = min(blockIdx.x * kStagePPB + wave, n - 1);
+    double rowsum[G];
+    if constexpr (std::is_same<TK, bf16nf_t>::value) {
+      const int o = a.nf_off[pw];
+      gemv_nf_wave<G>(static_cast<const bf16_t*>(a.K), a.Knf_lo + (size_t)o * G * kNfChunk,
+                      a.nf_list + o, a.nf_off[pw + 1] - o, a.ld, pw * G, a.rows_all, a.cols,
+                      static_cast<const float*>(vin), rowsum);
+    } else {
+      gemv_wave<TK, G>(static_cast<const TK*>(a.K), a.ld, pw * G, a.rows_all, a.cols,
+                       static_cast<const VT*>(vin), rowsum);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) rs[wave][k] = rowsum[k];
+    }
+    __syncthreads();
+  }
+  if (wave != 0) return;
+  const bool reduce = (s == 0) || (s == 3);
+  double r9[kNRed];
+  r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
+  r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
+  r9[RED_DX2] = 0.0;
+  if (t < kStagePPB && p < n) {
+    const PatchCoef cf = load_coef(a.coef, n, p);
+    const Tableau& tb = a.tb;
+    double x[D], y[D];
+#pragma unroll
+    for (int f = 0; f < D; ++f) x[f] = X[f * n + p];
+    if (s == 1) {
+      // attempt start: monitor snapshot and rupture marking (as k_prep)
+      if (mon_copy) {
+#pragma unroll
+        for (int f = 0; f < D; ++f) a.Xmon[f * n + p] = x[f];
+      }
+      if (rupt_update && vel_mag<1>(a.fp, cf.a, x) >= a.so.rupture_vth) a.rupt[p] = 1;
+    }
+    // state at which this launch evaluates the explicit part
+    if (s <= 1) {
+#pragma unroll
+      for (int f = 0; f < D; ++f) y[f] = x[f];
+    } else {
+      const double* Ys = a.Yb[s - 2];
+#pragma unroll
+      for (int f = 0; f < D; ++f) y[f] = Ys[f * n + p];
+    }
+    if (stream) {
+      const int j = (s <= 1) ? 0 : s - 1;
+      double g[D];
+      imex_explicit(a.fp, cf, y, rs[t][0], (G > 1) ? rs[t][1] : 0.0, a.sinc[p], a.sinc[n + p],
+                    g);
+      double* Gs = a.Stg[j];
+#pragma unroll
+      for (int f = 0; f < D; ++f) Gs[f * n + p] = g[f];
+      if (j == 0) {
+        double fi[D];
+        imex_implicit(a.fp, cf, y, fi);
+        double* Fs = a.Stg[3];
+#pragma unroll
+        for (int f = 0; f < D; ++f) Fs[f * n + p] = fi[f];
+      }
+    }
+    if (s == 1 || s == 2) {
+      // stage k = s (0-based) state: Z = X + h sum_{j<k} (At[k][j] Fimpl_j + A[k][j] G_j)
+      const int k = s;
+      double Z[D];
+#pragma unroll
+      for (int f = 0; f < D; ++f) {
+        double acc = 0.0;
+        for (int j = 0; j < k; ++j)
+          acc += tb.At[k][j] * a.Stg[3 + j][f * n + p] + tb.A[k][j] * a.Stg[j][f * n + p];
+        Z[f] = x[f] + hstep * acc;
+      }
+      double yk[D], fi[D];
+      yk[0] = y[0];
+      yk[1] = y[1];
+      const int it = imex_newton(a.fp, cf, Z, hstep * tb.At[k][k], a.so.rtol, a.so.atol, yk, fi);
+      if (it < 0) {
+        atomicOr(&c->stage_fail, 1);
+      } else {
+        double* Yk = a.Yb[k - 1];
+        double* Fk = a.Stg[3 + k];
+#pragma unroll
+        for (int f = 0; f < D; ++f) {
+          Yk[f * n + p] = yk[f];
+          Fk[f * n + p] = fi[f];
+        }
+        write_vel<VT, 1>(a.fp, cf.a, yk, static_cast<VT*>(a.vout), p, a.vpl);
+        if (!rsf_patch_ok<1>(a.fp, cf, yk)) atomicOr(&c->stage_fail, 1);
+      }
+    } else if (s == 3) {
+      // new state, embedded error and the next step's product input
+      double xn[D], en = 0.0;
+#pragma unroll
+      for (int f = 0; f < D; ++f) {
+        double acc = 0.0, ae = 0.0;
+        for (int j = 0; j < 3; ++j) {
+          const double gf = a.Stg[j][f * n + p] + a.Stg[3 + j][f * n + p];
+          acc += tb.b[j] * gf;
+          ae += tb.bh[j] * gf;
+        }
+        xn[f] = x[f] + hstep * acc;
+        ae = x[f] + hstep * ae;
+        Xn[f * n + p] = xn[f];
+        const double tol = a.so.atol[f] + a.so.rtol * fmax(fabs(xn[f]), fabs(ae));
+        if (tol > 0.0) en = fmax(en, fabs(xn[f] - ae) / tol);
+        const double dm = a.Xmon[f * n + p] - xn[f];
+        r9[RED_DX2] += dm * dm;
+      }
+      r9[RED_ERR] = en;
+      write_vel<VT, 1>(a.fp, cf.a, xn, static_cast<VT*>(xc ? a.vxb[0] : a.vxb[1]), p, a.vpl);
+#pragma unroll
+      for (int f = 0; f < D; ++f) y[f] = xn[f];
+    }
+    if (reduce) {
+      const double vm = vel_mag<1>(a.fp, cf.a, y);
+      r9[RED_VMAX] = vm;
+      r9[RED_VSUM] = vm;
+      r9[RED_SLIP] = y[3];
+      r9[RED_MU] = y[1] / y[2];
+      r9[RED_SMAX] = y[2];
+      r9[RED_SMIN] = y[2];
+#pragma unroll
+      for (int f = 0; f < D; ++f) r9[RED_XN2] += y[f] * y[f];
+    }
+  }
+  if (reduce) {
+#pragma unroll
+    for (int k = 0; k < kNRed; ++k) {
+      const double q = wave_reduce(r9[k], red_kind(k));
+      if (t == 0) a.partials[(size_t)k * gridDim.x + blockIdx.x] = q;
     }
   }
 }
@@ -678,7 +926,24 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl(StageArgs a, int nb) {
 
 static void fill_tableau(Tableau& tb, int method) {
   memset(&tb, 0, sizeof(tb));
-  if (method == RSFG_RK3BS) {
+  if (method == RSFG_ARKIMEX_L2) {
+    // PETSc 3.19.6 TSARKIMEXL2 (src/ts/impls/arkimex/arkimex.c; the
+    // reference's -imex default, src/rsf_imex.c:283): explicit first stage,
+    // L-stable implicit part with gamma = 1 - 1/sqrt 2, b = bt = last
+    // implicit row, first-order embedded bembedt, quadratic dense output
+    const double s2 = sqrt(2.0), g = 1.0 - 1.0 / s2;
+    tb.s = 3;
+    tb.order = 2;
+    tb.imex = 1;
+    tb.A[1][0] = 2.0 - s2;
+    tb.A[2][0] = 0.5; tb.A[2][1] = 0.5;
+    tb.At[1][0] = g; tb.At[1][1] = g;
+    tb.At[2][0] = 1.0 / (2.0 * s2); tb.At[2][1] = 1.0 / (2.0 * s2); tb.At[2][2] = g;
+    for (int j = 0; j < 3; ++j) tb.b[j] = tb.At[2][j];
+    tb.bh[0] = (4.0 - s2) / 8.0; tb.bh[1] = (4.0 - s2) / 8.0; tb.bh[2] = 1.0 / (2.0 * s2);
+    tb.d1[0] = 1.0 / s2; tb.d1[1] = 1.0 / s2; tb.d1[2] = 1.0 - s2;
+    tb.d2[0] = -1.0 / (2.0 * s2); tb.d2[1] = -1.0 / (2.0 * s2); tb.d2[2] = 1.0 / s2;
+  } else if (method == RSFG_RK3BS) {
     tb.s = 4;
     tb.order = 3;
     tb.A[1][0] = 0.5;
@@ -712,6 +977,7 @@ struct Launcher {
   virtual void prep(const StageArgs& a, hipStream_t st) = 0;
   virtual void stage(const StageArgs& a, hipStream_t st) = 0;
   virtual void ctrl(const StageArgs& a, hipStream_t st) = 0;
+  virtual void imex(const StageArgs& a, hipStream_t st) = 0;
   virtual int wg_count(int n) const = 0;
 };
 
@@ -730,6 +996,13 @@ struct LauncherT : Launcher {
   }
   void ctrl(const StageArgs& a, hipStream_t st) override {
     hipLaunchKernelGGL((k_ctrl<NC>), dim3(1), dim3(kCtrlThreads), 0, st, a, wg_count(a.n));
+  }
+  // additive stages exist for the scalar layout only (the solve rejects
+  // the method otherwise)
+  void imex(const StageArgs& a, hipStream_t st) override {
+    if constexpr (NC == 1) {
+      hipLaunchKernelGGL((k_imex<TK, G>), dim3(wg_count(a.n)), dim3(64 * kStagePPB), 0, st, a);
+    }
   }
   // one patch per wave, kStagePPB waves per workgroup
   int wg_count(int n) const override { return (n + kStagePPB - 1) / kStagePPB; }
@@ -777,8 +1050,11 @@ static void fill_args(rsfg_ctx* h, const rsfg_solver_opts* o, const void* K, Sta
   for (int i = 0; i < 2; ++i) {
     a.Xb[i] = h->Xb[i];
     a.Yb[i] = h->Yb[i];
+    a.Pb[i] = h->Pb[i];
   }
   for (int i = 0; i < 8; ++i) a.Stg[i] = h->Stg[i];
+  a.vxb[0] = h->vxb[0];
+  a.vxb[1] = h->vxb[1];
   a.Xmon = h->Xmon;
   a.partials = h->partials;
   a.ev = h->d_ev;
@@ -817,15 +1093,24 @@ static int start_solve(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, Sta
   RSFG_HIP(hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, h->stream));
   // the GEMV reads the input vector up to a whole chunk past the last
   // column: its padding must be zero in this precision's element type
-  RSFG_HIP(hipMemsetAsync(h->vin[0], 0, 2 * h->vin_elems * sizeof(double), h->stream));
+  // (vin[0], vin[1], vxb[0], vxb[1] are consecutive)
+  RSFG_HIP(hipMemsetAsync(h->vin[0], 0, 4 * h->vin_elems * sizeof(double), h->stream));
   RSFG_HIP(hipMemsetAsync(h->d_rupt, 0, sizeof(int) * n, h->stream));
-  // K_0 at the initial state
   a.s = 0;
-  a.vout = h->vin[0];
-  L->init_vel(a, h->stream);
-  a.vin = h->vin[0];
-  a.vout = nullptr;
-  L->stage(a, h->stream);
+  if (a.tb.imex) {
+    // G_1 and Fimpl_1 at the initial state
+    a.vout = h->vxb[h->xcur];
+    L->init_vel(a, h->stream);
+    a.vout = nullptr;
+    L->imex(a, h->stream);
+  } else {
+    // K_0 at the initial state
+    a.vout = h->vin[0];
+    L->init_vel(a, h->stream);
+    a.vin = h->vin[0];
+    a.vout = nullptr;
+    L->stage(a, h->stream);
+  }
   L->ctrl(a, h->stream);
   RSFG_HIP(hipGetLastError());
   RSFG_HIP(hipMemcpyAsync(c, h->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost, h->stream));
@@ -841,8 +1126,12 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
   const auto wall0 = std::chrono::steady_clock::now();
   const void* K = operator_ptr(h, o->prec);
   RSFG_CHECK(K != nullptr, RSFG_ESTATE, "rsfg_solve: operator precision not assembled");
-  RSFG_CHECK(o->method == RSFG_RK5DP || o->method == RSFG_RK3BS, RSFG_EINVAL,
-             "rsfg_solve: unknown method");
+  RSFG_CHECK(o->method == RSFG_RK5DP || o->method == RSFG_RK3BS ||
+                 o->method == RSFG_ARKIMEX_L2,
+             RSFG_EINVAL, "rsfg_solve: unknown method");
+  RSFG_CHECK(o->method != RSFG_ARKIMEX_L2 || h->ncomp == 1, RSFG_EINVAL,
+             "rsfg_solve: the IMEX method is defined for the scalar slip layout "
+             "(RSFG_SLIP_STRIKE / DIP / RAKE), as in the reference");
   Launcher* L = make_launcher(o->prec, h->ncomp, h->G);
   if ((size_t)L->wg_count(h->n) * kNRed > h->partial_count) {
     delete L;
@@ -867,7 +1156,22 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
   const char* ng = getenv("RSFG_NO_GRAPH");
   const bool use_graph = !(ng && ng[0] == '1');
   auto enqueue_batch = [&]() {
-    for (int k = 0; k < per; ++k) {
+    for (int k = 0; k < per && a.tb.imex; ++k) {
+      StageArgs b = a;
+      b.s = 1;
+      b.vout = h->vin[0];
+      L->imex(b, h->stream);
+      b.s = 2;
+      b.vin = h->vin[0];
+      b.vout = h->vin[1];
+      L->imex(b, h->stream);
+      b.s = 3;
+      b.vin = h->vin[1];
+      b.vout = nullptr;
+      L->imex(b, h->stream);
+      L->ctrl(b, h->stream);
+    }
+    for (int k = 0; k < per && !a.tb.imex; ++k) {
       StageArgs b = a;
       b.s = 0;
       b.vout = h->vin[1];

@@ -23,7 +23,7 @@ PREC_NAMES = {PREC_FP64: "fp64", PREC_FP32: "fp32", PREC_BF16X2: "bf16x2",
 PREC_BY_NAME = {v: k for k, v in PREC_NAMES.items()}
 # bytes per entry; bf16nf: 2 + 2 x (fraction of chunks with lo), see FaultSystem.nf_fraction
 PREC_BYTES = {PREC_FP64: 8, PREC_FP32: 4, PREC_BF16X2: 4, PREC_BF16: 2, PREC_BF16NF: 2}
-RK5DP, RK3BS = 0, 1
+RK5DP, RK3BS, ARKIMEX_L2 = 0, 1, 2
 
 # every symbol include/rsfg.h declares
 EXPORTS = [

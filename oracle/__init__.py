@@ -110,6 +110,13 @@ def _setup(L):
     L.rsfo_solve.restype = C.c_int
     L.rsfo_solve.argtypes = [C.POINTER(Problem), C.POINTER(Options), _dp,
                              C.POINTER(Result)]
+    L.rsfo_imex_parts.restype = None
+    L.rsfo_imex_parts.argtypes = [C.POINTER(Problem), _dp, _dp, _dp]
+    L.rsfo_imex_block.restype = None
+    L.rsfo_imex_block.argtypes = [C.POINTER(Problem), C.c_int, _dp, _dp]
+    L.rsfo_imex_newton.restype = C.c_int
+    L.rsfo_imex_newton.argtypes = [C.POINTER(Problem), C.c_int, _dp, C.c_double,
+                                   C.c_double, _dp, _dp, _dp]
     L.rsfo_rk_attempt.restype = C.c_int
     L.rsfo_rk_attempt.argtypes = [C.POINTER(Problem), C.c_int, C.c_double,
                                   _dp, C.c_double, _dp, _dp, _dp, _dp,
@@ -269,6 +276,28 @@ class RSFProblem:
         S = lib().rsfo_state_rate(C.byref(self.p), i, psi, v, C.byref(d1),
                                   C.byref(d2))
         return S, d1.value, d2.value
+
+    def imex_parts(self, X):
+        """(G, Fimpl) of the IMEX split (scalar layout); G + Fimpl = rhs."""
+        X = np.ascontiguousarray(X, np.float64)
+        G, Fi = np.zeros_like(X), np.zeros_like(X)
+        lib().rsfo_imex_parts(C.byref(self.p), X, G, Fi)
+        return G, Fi
+
+    def imex_block(self, i, x):
+        """dFimpl/dX of cell i at its state x (4x4)."""
+        J = np.zeros(16)
+        lib().rsfo_imex_block(C.byref(self.p), i, np.ascontiguousarray(x, np.float64), J)
+        return J.reshape(4, 4)
+
+    def imex_newton(self, i, Z, hg, guess, rtol=1e-8, atol=(1e-12, 1e-3, 1e-3, 1e-3)):
+        """Stage equation Y = Z + hg Fimpl(Y) of cell i; returns (Y, Fi, iterations)."""
+        y = np.zeros(4)
+        y[:2] = guess[:2]
+        fi = np.zeros(4)
+        it = lib().rsfo_imex_newton(C.byref(self.p), i, np.ascontiguousarray(Z, np.float64),
+                                    hg, rtol, np.ascontiguousarray(atol, np.float64), y, fi)
+        return y, fi, it
 
     def rk_attempt(self, X, F0, h, method=0, rtol=1e-8, atol=None):
         atol = np.asarray(atol if atol is not None else

@@ -115,9 +115,23 @@ void rsfo_rhs(const rsfo_problem *p, const double *X, double *F,
 /* rsf_domain_check, src/rsf_engine.c:309-362; returns 1 if acceptable */
 int rsfo_domain_ok(const rsfo_problem *p, const double *X);
 
+/* ---- IMEX split (src/rsf_imex.c), scalar layout only ---- */
+/* explicit part G (rsf_IMEX_RHSFunction) and implicit part Fimpl
+   (rsf_IMEX_IFunction as Xdot - Fimpl), AoS like rsfo_rhs; G + Fimpl = F */
+void rsfo_imex_parts(const rsfo_problem *p, const double *X, double *G, double *Fi);
+/* dFimpl/dX of cell i at x (4x4 row-major); rsf_IMEX_IJacobian is
+   shift I minus this block */
+void rsfo_imex_block(const rsfo_problem *p, int i, const double *x, double *J);
+/* stage equation Y = Z + hg Fimpl(Y) of cell i by per-cell Newton;
+   y: initial guess (psi, tau) in, stage state (4) out; fi: (Y - Z)/hg.
+   Returns the iteration count or -1 */
+int rsfo_imex_newton(const rsfo_problem *p, int i, const double *Z, double hg,
+                     double rtol, const double *atol, double *y, double *fi);
+
 /* ---- adaptive integrator, events and monitor ---- */
 typedef struct {
-  int method;                 /* 0 = rk5dp (default), 1 = rk3bs */
+  int method;                 /* 0 = rk5dp (default), 1 = rk3bs,
+                                 2 = ARKIMEX l2 (scalar layout) */
   double rtol;
   double atol[6];             /* per state component (dim of the layout) */
   double t0, t_end, dt_init, dt_max;

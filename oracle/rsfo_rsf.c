@@ -182,6 +182,143 @@ void rsfo_rhs(const rsfo_problem *p, const double *X, double *F, double *vel_out
   free(sd);
 }
 
+/* ---------------- IMEX split (src/rsf_imex.c, scalar layout) ----------------
+   Per cell x = (psi, tau, sigma, slip), eta = G/(2 cs), denom = 1 + eta dv/dtau.
+   Explicit G (rsf_IMEX_RHSFunction, src/rsf_imex.c:62-115):
+     G_psi = 0, G_sigma = sigma_dot, G_tau = (tau_dot + sinc - eta dv/dsigma G_sigma) / denom,
+     G_slip = v.
+   Implicit Fimpl (rsf_IMEX_IFunction :123-155 written as F = Xdot - Fimpl):
+     Fimpl_psi = S(psi, |v|), Fimpl_tau = -eta (dv/dpsi) S / denom, others 0.
+   G + Fimpl equals rsf_ODE_RHSFunction. */
+void rsfo_imex_parts(const rsfo_problem *p, const double *X, double *G, double *Fi)
+{
+  int n = p->n, i;
+  double *F = (double *)malloc(sizeof(double) * 4 * n);
+  double *vel = (double *)malloc(sizeof(double) * n);
+  double *td = (double *)malloc(sizeof(double) * n);
+  rsfo_rhs(p, X, F, vel, td);
+  for (i = 0; i < n; i++) {
+    const double *x = X + 4 * (size_t)i;
+    double *g = G + 4 * (size_t)i, *fi = Fi + 4 * (size_t)i;
+    double a = p->a[i], eta = p->eta, sdot = F[4 * (size_t)i + 2];
+    double mu = x[1] / x[2], st = mu / a, ef = exp(-x[0] / a);
+    double pre = 2.0 * p->v0 / (a * x[2]), cf = cosh(st) * ef;
+    double dvdtau = pre * cf, dvdsig = -pre * cf * mu, denom = 1.0 + eta * dvdtau;
+    double S = rsfo_state_rate(p, i, x[0], vel[i], NULL, NULL);
+    g[0] = 0.0;
+    g[2] = sdot;
+    g[1] = (td[i] + p->sinc[i] - eta * (dvdsig * sdot)) / denom;
+    g[3] = vel[i];
+    fi[0] = S;
+    fi[1] = -(eta * (-vel[i] / a) * S) / denom;
+    fi[2] = 0.0;
+    fi[3] = 0.0;
+  }
+  free(F); free(vel); free(td);
+}
+
+/* local implicit part of cell i at (psi, tau; sigma): Fimpl_psi, Fimpl_tau
+   and, if J != NULL, the 4x4 block dFimpl/dX (row-major, rows psi and tau
+   with columns psi, tau, sigma; the reference's IJacobian,
+   src/rsf_imex.c:162-237, is shift I minus this block) */
+static void imex_local(const rsfo_problem *p, int i, const double *x, double *fpsi,
+                       double *ftau, double *J)
+{
+  double a = p->a[i], E = p->eta, sig = x[2];
+  double mu = x[1] / sig, st = mu / a, ef = exp(-x[0] / a);
+  double v = 2.0 * p->v0 * ef * sinh(st);
+  double dSdpsi, dSdv;
+  double S = rsfo_state_rate(p, i, x[0], v, &dSdpsi, &dSdv);
+  double pre = 2.0 * p->v0 / (a * sig), cf = cosh(st) * ef;
+  double dvdtau = pre * cf, dvdsig = -pre * cf * mu, dvdpsi = -v / a;
+  double denom = 1.0 + E * dvdtau, Q = E * dvdpsi / denom;
+  *fpsi = S;
+  *ftau = -Q * S;
+  if (J) {
+    double vabs = fabs(v), sg = (v < 0.0) ? -1.0 : 1.0;
+    /* total derivatives of S(psi, |v(psi, tau, sigma)|) */
+    double Spsi = dSdpsi + dSdv * (-vabs / a);
+    double Stau = dSdv * sg * dvdtau;
+    double Ssig = dSdv * sg * dvdsig;
+    /* derivatives of dv/dtau and of Q = E (dv/dpsi) / denom */
+    double d2_psi = -dvdtau / a;
+    double d2_tau = v / (a * a * sig * sig);
+    double d2_sig = -dvdtau / sig - v * x[1] / (a * a * sig * sig * sig);
+    double k = E / (denom * denom);
+    double Qpsi = k * ((-dvdpsi / a) * denom - dvdpsi * E * d2_psi);
+    double Qtau = k * ((-dvdtau / a) * denom - dvdpsi * E * d2_tau);
+    double Qsig = k * ((-dvdsig / a) * denom - dvdpsi * E * d2_sig);
+    memset(J, 0, 16 * sizeof(double));
+    J[0] = Spsi; J[1] = Stau; J[2] = Ssig;
+    J[4] = -(Q * Spsi + S * Qpsi);
+    J[5] = -(Q * Stau + S * Qtau);
+    J[6] = -(Q * Ssig + S * Qsig);
+  }
+}
+
+void rsfo_imex_block(const rsfo_problem *p, int i, const double *x, double *J)
+{
+  double fp, ft;
+  imex_local(p, i, x, &fp, &ft, J);
+}
+
+/* Stage equation of one cell, Y = Z + hg Fimpl(Y): sigma and slip are
+   explicit (Y = Z); (psi, tau) solve R(u) = u - Z - hg Fimpl(u) = 0 by
+   Newton's method with the exact 2x2 Jacobian I - hg dFimpl/du and
+   backtracking on the weighted residual.  Converged when the weighted
+   residual or the weighted update is at most IMEX_NEWTON_TOL (weights
+   1/(atol + rtol |Z|), the step controller's).  The reference's PETSc SNES
+   (newtonls, bt line search, rtol 1e-8 on the global residual 2-norm) is
+   replaced by this per-cell test; both leave stage errors far below the
+   step tolerance.  Returns the iteration count, or -1 on failure.  y holds
+   the initial guess (psi, tau) on entry and the stage state on exit, fi
+   the implicit stage derivative (Y - Z) / hg. */
+#define IMEX_NEWTON_TOL 1e-6
+#define IMEX_NEWTON_MAXIT 50
+int rsfo_imex_newton(const rsfo_problem *p, int i, const double *Z, double hg,
+                     double rtol, const double *atol, double *y, double *fi)
+{
+  double w0 = 1.0 / (atol[0] + rtol * fabs(Z[0])), w1 = 1.0 / (atol[1] + rtol * fabs(Z[1]));
+  double u[4] = {y[0], y[1], Z[2], Z[3]}, J[16], fp, ft, r0, r1, rn;
+  int it, conv = 0;
+  imex_local(p, i, u, &fp, &ft, J);
+  r0 = u[0] - Z[0] - hg * fp;
+  r1 = u[1] - Z[1] - hg * ft;
+  rn = fmax(fabs(r0) * w0, fabs(r1) * w1);
+  for (it = 0; it < IMEX_NEWTON_MAXIT; it++) {
+    double m00, m01, m10, m11, det, d0, d1, lam = 1.0, ut[4], rt0 = 0, rt1 = 0, rtn = 0;
+    int k, ok = 0;
+    if (!isfinite(rn)) return -1;
+    if (rn <= IMEX_NEWTON_TOL) { conv = 1; break; }
+    m00 = 1.0 - hg * J[0]; m01 = -hg * J[1];
+    m10 = -hg * J[4]; m11 = 1.0 - hg * J[5];
+    det = m00 * m11 - m01 * m10;
+    if (!(det != 0.0) || !isfinite(det)) return -1;
+    d0 = (-r0 * m11 + r1 * m01) / det;
+    d1 = (-r1 * m00 + r0 * m10) / det;
+    for (k = 0; k < 12; k++) {
+      ut[0] = u[0] + lam * d0; ut[1] = u[1] + lam * d1; ut[2] = u[2]; ut[3] = u[3];
+      imex_local(p, i, ut, &fp, &ft, J);
+      rt0 = ut[0] - Z[0] - hg * fp;
+      rt1 = ut[1] - Z[1] - hg * ft;
+      rtn = fmax(fabs(rt0) * w0, fabs(rt1) * w1);
+      if (isfinite(rtn) && rtn < (1.0 - 1e-4 * lam) * rn) { ok = 1; break; }
+      lam *= 0.5;
+    }
+    if (!ok) return -1;
+    u[0] = ut[0]; u[1] = ut[1];
+    r0 = rt0; r1 = rt1; rn = rtn;
+    if (fmax(fabs(lam * d0) * w0, fabs(lam * d1) * w1) <= IMEX_NEWTON_TOL) { conv = 1; it++; break; }
+  }
+  if (!conv) return -1;
+  y[0] = u[0]; y[1] = u[1]; y[2] = Z[2]; y[3] = Z[3];
+  fi[0] = (u[0] - Z[0]) / hg;
+  fi[1] = (u[1] - Z[1]) / hg;
+  fi[2] = 0.0;
+  fi[3] = 0.0;
+  return it;
+}
+
 static int is_normal(double v) { return isfinite(v) && (fpclassify(v) == FP_NORMAL); }
 
 /* rsf_domain_check, src/rsf_engine.c:309-362 */
@@ -204,14 +341,33 @@ int rsfo_domain_ok(const rsfo_problem *p, const double *X)
 
 /* ---------------- Runge-Kutta tableaus ---------------- */
 typedef struct {
-  int s, order;
+  int s, order, imex;
   double A[7][7], b[7], bh[7], c[7];
+  double At[7][7];      /* IMEX: implicit tableau (A is the explicit one) */
+  double d1[7], d2[7];  /* IMEX: dense output weights of theta and theta^2 */
 } rk_tab;
 
 static void tab_init(rk_tab *t, int method)
 {
   memset(t, 0, sizeof(*t));
-  if (method == 1) { /* Bogacki-Shampine 3(2), FSAL */
+  if (method == 2) {
+    /* PETSc 3.19.6 TSARKIMEXL2 (src/ts/impls/arkimex/arkimex.c, the
+       reference's -imex default, src/rsf_imex.c:283): L-stable second
+       order additive RK, explicit first stage, gamma = 1 - 1/sqrt 2;
+       b = bt = last implicit row, embedded (first order) bembedt, and the
+       quadratic dense output binterpt */
+    const double s2 = sqrt(2.0), g = 1.0 - 1.0 / s2;
+    t->s = 3; t->order = 2; t->imex = 1;
+    t->c[1] = 2.0 - s2; t->c[2] = 1.0;
+    t->A[1][0] = 2.0 - s2;
+    t->A[2][0] = 0.5; t->A[2][1] = 0.5;
+    t->At[1][0] = g; t->At[1][1] = g;
+    t->At[2][0] = 1.0 / (2.0 * s2); t->At[2][1] = 1.0 / (2.0 * s2); t->At[2][2] = g;
+    t->b[0] = t->At[2][0]; t->b[1] = t->At[2][1]; t->b[2] = t->At[2][2];
+    t->bh[0] = (4.0 - s2) / 8.0; t->bh[1] = (4.0 - s2) / 8.0; t->bh[2] = 1.0 / (2.0 * s2);
+    t->d1[0] = 1.0 / s2; t->d1[1] = 1.0 / s2; t->d1[2] = 1.0 - s2;
+    t->d2[0] = -1.0 / (2.0 * s2); t->d2[1] = -1.0 / (2.0 * s2); t->d2[2] = 1.0 / s2;
+  } else if (method == 1) { /* Bogacki-Shampine 3(2), FSAL */
     t->s = 4; t->order = 3;
     t->c[1] = 0.5; t->c[2] = 0.75; t->c[3] = 1.0;
     t->A[1][0] = 0.5;
@@ -240,6 +396,17 @@ static void dense_eval(const rk_tab *t, int m, double h, const double *X0,
                        const double *X1, double *const *K, double th, double *out)
 {
   int i;
+  if (t->imex) {
+    /* K[j] = G_j, K[3 + j] = Fimpl_j */
+    for (i = 0; i < m; i++) {
+      double acc = 0.0;
+      int j;
+      for (j = 0; j < 3; j++)
+        acc += (t->d1[j] * th + t->d2[j] * th * th) * (K[j][i] + K[3 + j][i]);
+      out[i] = X0[i] + h * acc;
+    }
+    return;
+  }
   if (t->s == 7) {
     static const double d1 = -12715105075.0 / 11282082432.0, d3 = 87487479700.0 / 32700410799.0,
                         d4 = -10690763975.0 / 1880347072.0, d5 = 701980252875.0 / 199316789632.0,
@@ -336,7 +503,15 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
   double fprev, peakv = 0.0;
   field_stats st;
   tab_init(&tb, o->method);
-  for (s = 0; s < tb.s; s++) K[s] = (double *)malloc(sizeof(double) * m);
+  if (tb.imex && nc != 1) {       /* the reference's IMEX is scalar-slip only */
+    r->steps = r->rejects = r->domain_rejects = r->rhs_evals = 0;
+    r->nevent = r->nmon = 0;
+    r->status = 4;
+    r->t_final = t;
+    r->dt_final = h;
+    return 4;
+  }
+  for (s = 0; s < 7; s++) K[s] = (double *)malloc(sizeof(double) * m);
   Y = (double *)malloc(sizeof(double) * m);
   Xn = (double *)malloc(sizeof(double) * m);
   Xmon = (double *)malloc(sizeof(double) * m);
@@ -350,7 +525,8 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
   r->nevent = 0;
   r->nmon = 0;
   hmax = (o->dt_max < o->dt_monitor) ? o->dt_max : o->dt_monitor;
-  rsfo_rhs(p, X, K[0], NULL, NULL);
+  if (tb.imex) rsfo_imex_parts(p, X, K[0], K[3]);
+  else rsfo_rhs(p, X, K[0], NULL, NULL);
   r->rhs_evals++;
   stats(p, X, &st);
   slipping = st.vmax > o->vel_event;
@@ -396,7 +572,42 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
       /* stage 0 is the accepted state; its check resets the storm counter */
       if (rsfo_domain_ok(p, X)) storm = 0;
       else { r->status = 3; goto done; }
-      for (s = 1; s < tb.s && stage_ok; s++) {
+      if (tb.imex) {
+        /* ARKIMEX stages 2 and 3: Z = X + h sum_j (At G-free implicit + A explicit),
+           per-cell Newton for Y = Z + h gamma Fimpl(Y) from the previous
+           stage's state (TSStep_ARKIMEX), then G at Y (one matvec) */
+        double *Yp = Xe;        /* previous stage state (initial guess) */
+        memcpy(Yp, X, sizeof(double) * m);
+        for (s = 1; s < tb.s && stage_ok; s++) {
+          const double hg = h * tb.At[s][s];
+          for (i = 0; i < n && stage_ok; i++) {
+            double Z[4], yv[4];
+            int f;
+            for (f = 0; f < 4; f++) {
+              size_t q = 4 * (size_t)i + f;
+              double acc = 0.0;
+              for (j = 0; j < s; j++) acc += tb.At[s][j] * K[3 + j][q] + tb.A[s][j] * K[j][q];
+              Z[f] = X[q] + h * acc;
+            }
+            yv[0] = Yp[4 * (size_t)i]; yv[1] = Yp[4 * (size_t)i + 1];
+            if (rsfo_imex_newton(p, i, Z, hg, o->rtol, o->atol, yv, K[3 + s] + 4 * (size_t)i) < 0)
+              stage_ok = 0;
+            for (f = 0; f < 4; f++) Y[4 * (size_t)i + f] = yv[f];
+          }
+          if (stage_ok && !rsfo_domain_ok(p, Y)) stage_ok = 0;
+          if (!stage_ok) break;
+          rsfo_imex_parts(p, Y, K[s], work);
+          r->rhs_evals++;
+          memcpy(Yp, Y, sizeof(double) * m);
+        }
+        if (stage_ok)
+          for (i = 0; i < m; i++) {
+            double acc = 0.0;
+            for (j = 0; j < tb.s; j++) acc += tb.b[j] * (K[j][i] + K[3 + j][i]);
+            Xn[i] = X[i] + h * acc;
+          }
+      }
+      for (s = 1; s < tb.s && stage_ok && !tb.imex; s++) {
         for (i = 0; i < m; i++) {
           double acc = 0.0;
           for (j = 0; j < s; j++) acc += tb.A[s][j] * K[j][i];
@@ -426,7 +637,7 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
       enorm = 0.0;
       for (i = 0; i < m; i++) {
         double a4 = 0.0, tol;
-        for (j = 0; j < tb.s; j++) a4 += tb.bh[j] * K[j][i];
+        for (j = 0; j < tb.s; j++) a4 += tb.bh[j] * (tb.imex ? K[j][i] + K[3 + j][i] : K[j][i]);
         a4 = X[i] + h * a4;
         tol = o->atol[i % dim] + o->rtol * fmax(fabs(Xn[i]), fabs(a4));
         if (tol > 0.0) enorm = fmax(enorm, fabs(Xn[i] - a4) / tol);
@@ -531,7 +742,12 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
         t += h;
         h = hnew;
         memcpy(X, Xn, sizeof(double) * m);
-        memcpy(K[0], K[tb.s - 1], sizeof(double) * m);   /* FSAL */
+        if (tb.imex) {                /* stage 1 of the next step: G and Fimpl at X */
+          rsfo_imex_parts(p, X, K[0], K[3]);
+          r->rhs_evals++;
+        } else {
+          memcpy(K[0], K[tb.s - 1], sizeof(double) * m);   /* FSAL */
+        }
       }
     }
     r->steps++;
@@ -549,7 +765,7 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
 done:
   r->t_final = t;
   r->dt_final = h;
-  for (s = 0; s < tb.s; s++) free(K[s]);
+  for (s = 0; s < 7; s++) free(K[s]);
   free(Y); free(Xn); free(Xmon); free(Xe); free(work);
   free(tau0); free(slip0); free(rupt);
   return r->status;

@@ -358,3 +358,123 @@ def test_bp5_1km_rk3bs_matches_reference_event_file():
     assert abs(on[0] / SEC_PER_YEAR - ref[2, 1]) < 1e-4
     assert abs(on[3] - ref[2, 4]) < 1e-3
     assert abs(on[4] - ref[2, 5]) < 2e-6
+
+
+# ------------------------------------------------------------ IMEX (src/rsf_imex.c)
+def _imex_cells(law, n=6, seed=1):
+    """A few coupled cells away from steady state, scalar layout."""
+    rng = np.random.default_rng(seed)
+    K = -np.abs(rng.normal(size=(n, n))) * 1e6
+    K = 0.5 * (K + K.T)
+    a, b, dc = np.full(n, 0.01), np.full(n, 0.015), np.full(n, 0.01)
+    P = oracle.RSFProblem(K, a, b, dc, K @ np.full(n, -1e-9), eta=32e9 / (2 * 3464.0),
+                          law=law, nthreads=1)
+    X = np.zeros(4 * n)
+    X[0::4] = 0.6 + 0.05 * rng.normal(size=n)
+    X[1::4] = 0.6 * 30e6 * (1 + 0.01 * rng.normal(size=n))
+    X[2::4] = 30e6
+    X[3::4] = 1.0
+    return P, X
+
+
+@pytest.mark.parametrize("law", [1, 2, 3, 4, 5])
+def test_imex_split_sums_to_rhs(law):
+    """G + Fimpl reproduces rsf_ODE_RHSFunction (src/rsf_imex.c:14-30)."""
+    P, X = _imex_cells(law)
+    G, Fi = P.imex_parts(X)
+    F = P.rhs(X)
+    assert np.all(G[0::4] == 0.0) and np.all(Fi[2::4] == 0.0) and np.all(Fi[3::4] == 0.0)
+    np.testing.assert_allclose(G + Fi, F, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("law", [1, 2, 3, 4, 5])
+def test_imex_block_jacobian_matches_finite_differences(law):
+    """The exact per-cell block (rsf_IMEX_IJacobian, src/rsf_imex.c:162-237)
+    against central differences of Fimpl in psi, tau and sigma."""
+    P, X = _imex_cells(law, seed=law)
+    for i in range(3):
+        x = X[4 * i:4 * i + 4].copy()
+        J = P.imex_block(i, x)
+        for k in range(3):
+            hk = 1e-6 * abs(x[k])
+            Xp, Xm = X.copy(), X.copy()
+            Xp[4 * i + k] += hk
+            Xm[4 * i + k] -= hk
+            fd = (P.imex_parts(Xp)[1][4 * i:4 * i + 2] - P.imex_parts(Xm)[1][4 * i:4 * i + 2]) / (2 * hk)
+            scale = np.abs(J[:2, :3]).max(axis=1)
+            assert np.all(np.abs(J[:2, k] - fd) <= 1e-6 * scale), (law, i, k, J[:2, k], fd)
+        assert np.all(J[2:] == 0.0) and np.all(J[:, 3] == 0.0)
+
+
+def test_imex_stage_newton_converges_and_detects_the_fold():
+    """Stage equation Y = Z + hg Fimpl(Y): a converged root satisfies the
+    equation to the weighted tolerance; on a nucleating cell at a stage of
+    hundreds of seconds the local problem has no root (the saddle-node the
+    reference describes for -imex, rsf_solve.md:905-916) and the solve
+    reports failure instead of returning a wrong state."""
+    P, X = _imex_cells(1)
+    x = X[:4].copy()
+    atol = np.array([1e-12, 1e-3, 1e-3, 1e-3])
+    for hg in (1.0, 10.0, 100.0):
+        Y, fi, it = P.imex_newton(0, x, hg, x)
+        assert it > 0
+        Xs = X.copy()
+        Xs[:4] = Y
+        R = Y[:2] - x[:2] - hg * P.imex_parts(Xs)[1][:2]
+        w = 1.0 / (atol[:2] + 1e-8 * np.abs(x[:2]))
+        assert np.max(np.abs(R) * w) < 1e-5
+        np.testing.assert_allclose(fi[:2], (Y[:2] - x[:2]) / hg, rtol=1e-12)
+        assert Y[2] == x[2] and Y[3] == x[3]
+    # a cell below steady state and accelerating: no root at a large stage
+    assert P.state_rate(0, x[0], oracle.vel(x[1], x[2], x[0], 0.01, 1e-6))[0] < 0
+    assert P.imex_newton(0, x, 1000.0, x)[2] == -1
+
+
+def _slider_problem(law):
+    G, cs = 3.204e10, 3464.0
+    g = oracle.read_patches(os.path.join(GOLDEN, "ref_results", "geom_slider10.in"))
+    K = oracle.traction_tensor(g, 0, 1)[:, :, 0, 0] * G / 1e4
+    vpl, sig = 1e-9, 50e6
+    P = oracle.RSFProblem(K, [0.01], [0.015], [0.008], K @ [-vpl], eta=G / (2 * cs),
+                          nthreads=1, law=law)
+    X0 = initial_state(0.6 * sig + G / (2 * cs) * vpl, sig, vpl, np.array([0.01]), 1e-6)
+    return P, X0
+
+
+def test_imex_l2_error_tracks_tolerance_on_the_slider():
+    """The l2 IMEX on the lab-parameter slider (slider/README.md:80-89):
+    delivered event-time error tracks rtol over decades, steps grow by
+    10^(1/2) per decade (second order), few rejections, and the event
+    count is the explicit method's."""
+    P, X0 = _slider_problem(1)
+    ref = P.solve(X0, 450 * SEC_PER_YEAR, method=0, rtol=1e-10, rdx_monitor=1e-5,
+                  rupture_vth=0.1)
+    on0 = ref["events"][ref["events"][:, 1] > 0, 0] / SEC_PER_YEAR
+    errs, steps = [], []
+    for rt in (1e-6, 1e-7, 1e-8):
+        r = P.solve(X0, 450 * SEC_PER_YEAR, method=2, rtol=rt, rdx_monitor=1e-5,
+                    rupture_vth=0.1)
+        on = r["events"][r["events"][:, 1] > 0, 0] / SEC_PER_YEAR
+        assert r["status"] == 0 and len(on) == len(on0) == 7
+        errs.append(np.sqrt(np.mean((on - on0) ** 2)))
+        steps.append(r["steps"])
+        assert r["rejects"] < 0.02 * r["steps"]
+    assert errs[0] < 0.01                               # reference: about 2e-3 yr at 1e-6
+    for k in range(2):
+        assert 5.0 < errs[k] / errs[k + 1] < 20.0       # error proportional to rtol
+        assert 2.6 < steps[k + 1] / steps[k] < 3.8      # 10^(1/2) per decade
+
+
+@pytest.mark.parametrize("law", [2, 3])
+def test_imex_slider_other_laws_keep_the_event_sequence(law):
+    """Slip and PRZ laws: same number of events as the explicit reference
+    solution, onsets within 0.01 yr at rtol 1e-6."""
+    P, X0 = _slider_problem(law)
+    ref = P.solve(X0, 450 * SEC_PER_YEAR, method=0, rtol=1e-9, rdx_monitor=1e-5,
+                  rupture_vth=0.1)
+    r = P.solve(X0, 450 * SEC_PER_YEAR, method=2, rtol=1e-6, rdx_monitor=1e-5,
+                rupture_vth=0.1)
+    on0 = ref["events"][ref["events"][:, 1] > 0, 0]
+    on = r["events"][r["events"][:, 1] > 0, 0]
+    assert r["status"] == 0 and len(on) == len(on0)
+    assert np.max(np.abs(on - on0)) / SEC_PER_YEAR < 0.01
