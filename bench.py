@@ -290,8 +290,32 @@ def main():
         K64 = fs.dense()
         out["cpu_baseline"] = cpu_baseline(g, f, K64, ref["rhs_evals"], args.years,
                                            args.cpu_budget, log)
+        del K64
     fs.close()
+    if args.config == "c3":
+        out["assembly_rough_triangles"] = rough_triangle_assembly(args.ds)
     print(json.dumps(out))
+
+
+def rough_triangle_assembly(ds_km):
+    """FP64 assembly throughput on the rough variant of the same outline:
+    two Nikkhoo-Walter triangles per cell (32000 at 0.5 km) on a seeded
+    Hurst 0.8 self-affine surface with amplitude-to-wavelength ratio 1e-3,
+    two slip components (64000 DOF, K 32.8 GB in FP64)."""
+    from interact_amd import faults, lib
+    from interact_amd.model import FaultSystem
+    nx, nz = int(round(100.0 / ds_km)), int(round(40.0 / ds_km))
+    g = faults.planar_tri_grid(ds_km, offset=faults.rough_offset(nx, nz, ds_km))
+    n = len(g["l"])
+    fs = FaultSystem(g)
+    fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9)
+    ms = fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9)
+    fs.close()
+    pairs = n * n
+    log(f"assembly, rough triangles: {n} triangles, {pairs} pairs in {ms:.1f} ms = "
+        f"{pairs / ms * 1e3:.3e} pairs/s")
+    return dict(triangles=n, pairs=pairs, ms=ms, pairs_per_s=pairs / ms * 1e3,
+                entries_per_s=4 * pairs / ms * 1e3, dof=2 * n)
 
 
 if __name__ == "__main__":
