@@ -224,6 +224,28 @@ int rsfg_solver_defaults(rsfg_solver_opts *o);
 int rsfg_set_state(rsfg_handle *h, double t, const double *X);
 int rsfg_get_state(rsfg_handle *h, double *t, double *X);
 
+/* ---- input tables and checkpoints ------------------------------------
+   Per-patch input tables of rsf_solve: n rows of ncols whitespace-separated
+   numbers, row-major into out: the rsf file "a b" (read_rsf,
+   src/interact/input.c:83-103), -rsf_ic_file "tau v", -rsf_dc_file "dc",
+   -rsf_sigma_file "sigma0" (src/rsf_solve.c:162-218), -rsf_vpl_file "vpl"
+   and -rsf_stress_rate_file "tau_dot sigma_dot" (:275-360).  A short or
+   malformed file returns RSFG_EINVAL naming the row (the reference exits). */
+int rsfg_read_table(const char *fname, int n, int ncols, double *out);
+
+/* rsf_solve checkpoint (rsf_write_checkpoint / rsf_read_checkpoint,
+   src/rsf_output.c:595-716): a PETSc binary file (big endian) holding a
+   16-entry metadata vector (magic 20260726, version 1, patches, state dim,
+   slip mode, state law, step, t, dt) and the state vector in the reference's
+   patch-major layout.  The writer goes through <file>.tmp and a rename and
+   keeps the previous file as <file>.prev.  The reader checks the magic, the
+   patch count and the state dim, and loads the state into the handle (as
+   rsfg_set_state); slip_mode, law, step and dt may be NULL. */
+int rsfg_write_checkpoint(rsfg_handle *h, const char *fname, int slip_mode, int law,
+                          long step, double dt);
+int rsfg_read_checkpoint(rsfg_handle *h, const char *fname, int *slip_mode, int *law,
+                         long *step, double *t, double *dt);
+
 typedef struct {
   long steps, rejects, domain_rejects, rhs_evals, launches;
   double t, dt;         /* time reached and next step size [s] */

@@ -969,6 +969,124 @@ int rsfg_get_state(rsfg_handle* h, double* t, double* X) {
   return RSFG_OK;
 }
 
+int rsfg_read_table(const char* fname, int n, int ncols, double* out) {
+  RSFG_CHECK(fname && out && n > 0 && ncols > 0, RSFG_EINVAL, "rsfg_read_table: bad arguments");
+  FILE* in = fopen(fname, "r");
+  RSFG_CHECK(in != nullptr, RSFG_EINVAL, std::string("rsfg_read_table: cannot open ") + fname);
+  for (int i = 0; i < n; ++i) {
+    for (int c = 0; c < ncols; ++c) {
+      if (fscanf(in, "%lf", out + (size_t)i * ncols + c) != 1) {
+        fclose(in);
+        set_error("rsfg_read_table: error reading row " + std::to_string(i) + " column " +
+                  std::to_string(c) + " of " + fname);
+        return RSFG_EINVAL;
+      }
+    }
+  }
+  fclose(in);
+  return RSFG_OK;
+}
+
+// PETSc binary vectors: int32 class id 1211214, int32 length, then the
+// values as IEEE doubles, all big endian
+constexpr int32_t kVecClassId = 1211214;
+constexpr double kCkptMagic = 20260726.0;
+constexpr int kCkptMeta = 16;
+
+static void put_be32(std::vector<unsigned char>& b, int32_t v) {
+  const uint32_t u = (uint32_t)v;
+  for (int k = 3; k >= 0; --k) b.push_back((unsigned char)(u >> (8 * k)));
+}
+static void put_be_double(std::vector<unsigned char>& b, double v) {
+  uint64_t u;
+  memcpy(&u, &v, 8);
+  for (int k = 7; k >= 0; --k) b.push_back((unsigned char)(u >> (8 * k)));
+}
+static bool get_be32(FILE* f, int32_t* v) {
+  unsigned char c[4];
+  if (fread(c, 1, 4, f) != 4) return false;
+  *v = (int32_t)(((uint32_t)c[0] << 24) | ((uint32_t)c[1] << 16) | ((uint32_t)c[2] << 8) | c[3]);
+  return true;
+}
+static bool get_be_vec(FILE* f, std::vector<double>& v) {
+  int32_t id, len;
+  if (!get_be32(f, &id) || !get_be32(f, &len) || id != kVecClassId || len < 0) return false;
+  std::vector<unsigned char> c((size_t)len * 8);
+  if (fread(c.data(), 1, c.size(), f) != c.size()) return false;
+  v.resize(len);
+  for (int32_t i = 0; i < len; ++i) {
+    uint64_t u = 0;
+    for (int k = 0; k < 8; ++k) u = (u << 8) | c[(size_t)i * 8 + k];
+    memcpy(&v[i], &u, 8);
+  }
+  return true;
+}
+
+int rsfg_write_checkpoint(rsfg_handle* h, const char* fname, int slip_mode, int law, long step,
+                          double dt) {
+  RSFG_CHECK(h && fname, RSFG_EINVAL, "rsfg_write_checkpoint: null argument");
+  const int n = h->n, D = h->dim;
+  std::vector<double> X((size_t)D * n);
+  double t = 0.0;
+  int rc = rsfg_get_state(h, &t, X.data());
+  if (rc) return rc;
+  RSFG_CHECK((size_t)D * n <= 0x7fffffff, RSFG_EINVAL, "rsfg_write_checkpoint: state too large");
+  double meta[kCkptMeta] = {0.0};
+  meta[0] = kCkptMagic;
+  meta[1] = 1.0;
+  meta[2] = n;
+  meta[3] = D;
+  meta[4] = slip_mode;
+  meta[5] = law;
+  meta[6] = (double)step;
+  meta[7] = t;
+  meta[8] = dt;
+  std::vector<unsigned char> b;
+  b.reserve(16 + 8 * (kCkptMeta + X.size()));
+  put_be32(b, kVecClassId);
+  put_be32(b, kCkptMeta);
+  for (double m : meta) put_be_double(b, m);
+  put_be32(b, kVecClassId);
+  put_be32(b, (int32_t)X.size());
+  for (double x : X) put_be_double(b, x);
+  const std::string tmp = std::string(fname) + ".tmp", prev = std::string(fname) + ".prev";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  RSFG_CHECK(f != nullptr, RSFG_EINVAL, "rsfg_write_checkpoint: cannot create " + tmp);
+  const bool ok = fwrite(b.data(), 1, b.size(), f) == b.size();
+  const bool closed = fclose(f) == 0;
+  RSFG_CHECK(ok && closed, RSFG_EINVAL, "rsfg_write_checkpoint: write to " + tmp + " failed");
+  rename(fname, prev.c_str());  // fails harmlessly on the first write
+  RSFG_CHECK(rename(tmp.c_str(), fname) == 0, RSFG_EINVAL,
+             std::string("rsfg_write_checkpoint: rename to ") + fname + " failed");
+  return RSFG_OK;
+}
+
+int rsfg_read_checkpoint(rsfg_handle* h, const char* fname, int* slip_mode, int* law,
+                         long* step, double* t, double* dt) {
+  RSFG_CHECK(h && fname, RSFG_EINVAL, "rsfg_read_checkpoint: null argument");
+  FILE* f = fopen(fname, "rb");
+  RSFG_CHECK(f != nullptr, RSFG_EINVAL, std::string("rsfg_read_checkpoint: cannot open ") + fname);
+  std::vector<double> meta, X;
+  const bool ok = get_be_vec(f, meta) && get_be_vec(f, X);
+  fclose(f);
+  RSFG_CHECK(ok && meta.size() == (size_t)kCkptMeta && meta[0] == kCkptMagic, RSFG_EINVAL,
+             std::string("rsfg_read_checkpoint: ") + fname + " is not an rsf_solve checkpoint");
+  RSFG_CHECK((int)meta[2] == h->n, RSFG_EINVAL,
+             "rsfg_read_checkpoint: checkpoint has " + std::to_string((long)meta[2]) +
+                 " patches, geometry has " + std::to_string(h->n));
+  RSFG_CHECK((int)meta[3] == h->dim && X.size() == (size_t)h->dim * h->n, RSFG_EINVAL,
+             "rsfg_read_checkpoint: checkpoint dim " + std::to_string((long)meta[3]) +
+                 ", run dim " + std::to_string(h->dim));
+  const int rc = rsfg_set_state(h, meta[7], X.data());
+  if (rc) return rc;
+  if (slip_mode) *slip_mode = (int)meta[4];
+  if (law) *law = (int)meta[5];
+  if (step) *step = (long)meta[6];
+  if (t) *t = meta[7];
+  if (dt) *dt = meta[8];
+  return RSFG_OK;
+}
+
 int rsfg_rhs(rsfg_handle* h, int prec, const double* X, double* F, int* domain_ok) {
   RSFG_CHECK(h && X && F, RSFG_EINVAL, "rsfg_rhs: null argument");
   RSFG_CHECK(h->have_friction && h->have_loading, RSFG_ESTATE,

@@ -34,6 +34,7 @@ EXPORTS = [
     "rsfg_set_friction", "rsfg_set_loading", "rsfg_rhs",
     "rsfg_solver_defaults", "rsfg_set_state", "rsfg_get_state", "rsfg_solve",
     "rsfg_events", "rsfg_monitor", "rsfg_stage_bench",
+    "rsfg_read_table", "rsfg_write_checkpoint", "rsfg_read_checkpoint",
 ]
 
 
@@ -125,6 +126,12 @@ def load(path=None):
     L.rsfg_solver_defaults.argtypes = [C.POINTER(SolverOpts)]
     L.rsfg_set_state.argtypes = [h, C.c_double, _dp]
     L.rsfg_get_state.argtypes = [h, C.POINTER(C.c_double), _dp]
+    L.rsfg_read_table.argtypes = [C.c_char_p, C.c_int, C.c_int, _dp]
+    L.rsfg_write_checkpoint.argtypes = [h, C.c_char_p, C.c_int, C.c_int, C.c_long,
+                                        C.c_double]
+    L.rsfg_read_checkpoint.argtypes = [h, C.c_char_p, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.POINTER(C.c_long),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.rsfg_solve.argtypes = [h, C.POINTER(SolverOpts), C.c_double,
                              C.POINTER(Stats)]
     L.rsfg_events.argtypes = [h, _dp, C.POINTER(C.c_int)]

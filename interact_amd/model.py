@@ -7,6 +7,7 @@ initial state -> adaptive integration with monitor and event output.
 All compute runs in librsfg on the GPU; this module only marshals arrays.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -170,6 +171,20 @@ class FaultSystem:
                                          np.ascontiguousarray(X, np.float64)),
                 "rsfg_set_state")
 
+    def write_checkpoint(self, fname, *, slip_mode=0, law=1, step=0, dt=0.0):
+        """rsf_solve checkpoint of the current state (PETSc binary)."""
+        L.check(self._lib.rsfg_write_checkpoint(self.h, os.fsencode(fname), slip_mode, law,
+                                                step, dt), "rsfg_write_checkpoint")
+
+    def read_checkpoint(self, fname):
+        """Loads the state of an rsf_solve checkpoint; returns its metadata."""
+        sm, law, step = C.c_int(), C.c_int(), C.c_long()
+        t, dt = C.c_double(), C.c_double()
+        L.check(self._lib.rsfg_read_checkpoint(self.h, os.fsencode(fname), C.byref(sm),
+                                               C.byref(law), C.byref(step), C.byref(t),
+                                               C.byref(dt)), "rsfg_read_checkpoint")
+        return dict(slip_mode=sm.value, law=law.value, step=step.value, t=t.value, dt=dt.value)
+
     def get_state(self):
         X = np.zeros(self.n * self.dim)
         t = C.c_double()
@@ -213,6 +228,14 @@ class FaultSystem:
         n = C.c_int(cap)
         L.check(self._lib.rsfg_monitor(self.h, rows, C.byref(n)), "monitor")
         return rows[:10 * n.value].reshape(-1, 10).copy()
+
+
+def read_table(fname, n, ncols):
+    """Per-patch input table of rsf_solve (rsf "a b", ic "tau v", dc,
+    sigma0, vpl, "tau_dot sigma_dot"): an (n, ncols) array."""
+    out = np.zeros(n * ncols)
+    L.check(L.load().rsfg_read_table(os.fsencode(fname), n, ncols, out), "rsfg_read_table")
+    return out.reshape(n, ncols)
 
 
 def initial_state(tau, sigma, vel, a, v0, ncomp=1):

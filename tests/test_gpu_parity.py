@@ -398,6 +398,56 @@ def test_imex_rejects_the_vector_layout():
     fs.close()
 
 
+def test_checkpoint_round_trip_and_restart(tmp_path):
+    """rsf_solve checkpoints (src/rsf_output.c:595-716): the PETSc binary
+    layout (big-endian class id 1211214 and length, 16 metadata values
+    starting with magic 20260726 and version 1, then the patch-major state);
+    a reload into a fresh handle restores the state bit for bit; a run
+    restarted from a mid-run checkpoint, with the checkpoint's dt as its
+    first step, ends exactly where the uninterrupted run does."""
+    from interact_amd import lib
+    g, f, fs, X0 = _bp5(2.0, prestress="make_bp5")
+    n = len(f["a"])
+    T = 0.02 * SEC_PER_YEAR
+    fs.set_state(X0)
+    stA = fs.solve(T, rtol=1e-6)
+    XA, tA = fs.get_state()
+    fs.set_state(X0)
+    st1 = fs.solve(T / 2, rtol=1e-6)
+    X1, t1 = fs.get_state()
+    ck = tmp_path / "rsf_checkpoint.bin"
+    fs.write_checkpoint(str(ck), slip_mode=0, law=1, step=st1["steps"], dt=st1["dt"])
+    raw = ck.read_bytes()
+    assert len(raw) == 8 + 16 * 8 + 8 + 4 * n * 8
+    assert list(np.frombuffer(raw[:8], ">i4")) == [1211214, 16]
+    meta = np.frombuffer(raw[8:136], ">f8")
+    assert list(meta[:7]) == [20260726.0, 1.0, n, 4, 0, 1, st1["steps"]]
+    assert meta[7] == t1 and meta[8] == st1["dt"] and not meta[9:].any()
+    assert list(np.frombuffer(raw[136:144], ">i4")) == [1211214, 4 * n]
+    assert np.array_equal(np.frombuffer(raw[144:], ">f8"), X1)
+    fs.write_checkpoint(str(ck), step=st1["steps"], dt=st1["dt"])
+    assert (tmp_path / "rsf_checkpoint.bin.prev").exists()
+    fs.close()
+    _, _, fs2, _ = _bp5(2.0, prestress="make_bp5")
+    md = fs2.read_checkpoint(str(ck))
+    X2, t2 = fs2.get_state()
+    assert np.array_equal(X2, X1) and t2 == t1 and md["step"] == st1["steps"]
+    st2 = fs2.solve(T, rtol=1e-6, dt_init=md["dt"])
+    XB, tB = fs2.get_state()
+    assert st1["steps"] + st2["steps"] == stA["steps"]
+    assert tB == tA
+    assert np.abs(XB - XA).max() <= 1e-12 * np.abs(XA).max()
+    fs2.close()
+    # a checkpoint of another mesh is refused
+    g3 = oracle.read_patches(os.path.join(GOLDEN, "ref_results", "geom_slider10.in"))
+    fs3 = _fs(g3)
+    fs3.assemble(lib.SLIP_STRIKE, shear_modulus=G_BP5)
+    fs3.set_friction([0.01], [0.015], [0.008], eta=1e6)
+    with pytest.raises(lib.RsfgError, match="patches"):
+        fs3.read_checkpoint(str(ck))
+    fs3.close()
+
+
 def test_mixed_precision_cycle_bp5_2km():
     """FP32 and bf16 hi+lo storage with FP64 state keep the recurrence
     within 0.05 yr of the FP64 path."""

@@ -49,3 +49,23 @@ int main(void){printf("%zu %zu %zu %zu %zu\n", sizeof(rsfg_patch),
         sizes = [int(v) for v in subprocess.run([exe], capture_output=True, text=True).stdout.split()]
     py = [ctypes.sizeof(t) for t in (lib.Patch, lib.Assembly, lib.Friction, lib.SolverOpts, lib.Stats)]
     assert sizes == py
+
+
+def test_read_table_matches_the_reference_inputs(tmp_path):
+    """rsf_solve's per-patch tables through the library's reader (host code,
+    no GPU): the BP5 2 km rsf ("a b"), ic ("tau v") and dc files as held by
+    the reference (seas_bp_tests/bp5); a short file names the failing row."""
+    import numpy as np
+    from interact_amd.model import read_table
+    from conftest import GOLDEN
+    d = os.path.join(GOLDEN, "bp5_2km")
+    for name, ncols in (("rsf_bp5_2km.dat", 2), ("ic_bp5_2km.in", 2), ("dc_bp5_2km.in", 1)):
+        ref = np.loadtxt(os.path.join(d, name)).reshape(1000, ncols)
+        got = read_table(os.path.join(d, name), 1000, ncols)
+        assert np.array_equal(got, ref), name
+    short = tmp_path / "short.in"
+    short.write_text("0.01 0.015\n0.01\n")
+    from interact_amd import lib
+    import pytest
+    with pytest.raises(lib.RsfgError, match="row 1 column 1"):
+        read_table(str(short), 2, 2)
