@@ -100,6 +100,13 @@ int rsfg_destroy(rsfg_handle *h);
    the capacity of patches; on return the number read. */
 int rsfg_read_patch_file(const char *fname, rsfg_patch *patches, int *n);
 
+/* A handle that shares geom's patches and assembled operators (read only)
+   and owns its own friction, loading, state and solver buffers: members of
+   an ensemble over one fault geometry (no counterpart in the reference,
+   which runs one parameter set per process).  Destroy members before their
+   geometry; a member cannot re-assemble. */
+int rsfg_create_member(rsfg_handle *geom, rsfg_handle **member);
+
 /* ---- operator assembly ----------------------------------------------
    Replaces the dense branch of calc_petsc_Isn_matrices
    (src/petsc_matrix_operations.c:379-503, MatScale at :735) and
@@ -257,6 +264,36 @@ typedef struct {
 /* integrates from the current state to t_end (STEPOVER semantics) */
 int rsfg_solve(rsfg_handle *h, const rsfg_solver_opts *o, double t_end,
                rsfg_stats *st);
+
+/* ---- ensembles ---------------------------------------------------------
+   Batched runs over shared operators (no counterpart in the reference,
+   which runs one parameter set per process): members_per_geom members per
+   geometry handle (16, 32 or 64), each a member handle (rsfg_create_member)
+   configured through the ordinary calls (rsfg_set_friction,
+   rsfg_set_loading, rsfg_set_state; rsfg_events, rsfg_monitor,
+   rsfg_get_state afterwards).  Every geometry needs an FP32 operator.  The
+   solve advances all members together, each with its own adaptive steps,
+   events and monitor; each stage of a geometry's members is one product
+   K [v_1 .. v_M] on the matrix cores (FP32 operands and accumulation,
+   flushed to FP64 every 256 columns).  Member m of geometry g is
+   g members_per_geom + k. */
+typedef struct rsfg_ensemble rsfg_ensemble;
+int rsfg_ensemble_create(rsfg_handle *const *geoms, int ngeom, int members_per_geom,
+                         rsfg_ensemble **e);
+int rsfg_ensemble_member(rsfg_ensemble *e, int member, rsfg_handle **h);
+/* o->prec must be RSFG_PREC_FP32, method rk5dp or rk3bs; st: one record
+   per member */
+int rsfg_ensemble_solve(rsfg_ensemble *e, const rsfg_solver_opts *o, double t_end,
+                        rsfg_stats *st);
+/* autotuner: times every candidate tile shape of the batched stage on this
+   ensemble (members must be configured), keeps the fastest; ms (may be
+   NULL) receives one time per candidate, -1 where a shape does not fit */
+int rsfg_ensemble_tune(rsfg_ensemble *e, const rsfg_solver_opts *o, int *choice, double *ms);
+/* current tile shape, number of candidates, and per candidate four ints
+   (rows per wave, row groups per workgroup, column ranges per row group,
+   chunks in flight per wave) */
+int rsfg_ensemble_config(rsfg_ensemble *e, int *cfg, int *ncfg, int *shape);
+int rsfg_ensemble_destroy(rsfg_ensemble *e);
 
 /* times nrep back-to-back launches of the fused stage kernel (K v with the
    RHS epilogue, as used inside rsfg_solve) at the current state with the

@@ -602,10 +602,69 @@ int rsfg_create(int device, const rsfg_patch* pt, int n, rsfg_handle** out) {
   return RSFG_OK;
 }
 
+int rsfg_create_member(rsfg_handle* geom, rsfg_handle** out) {
+  RSFG_CHECK(geom && out, RSFG_EINVAL, "rsfg_create_member: null argument");
+  RSFG_CHECK(!geom->borrowed, RSFG_EINVAL, "rsfg_create_member: geom is itself a member");
+  RSFG_CHECK(geom->assembled, RSFG_ESTATE, "rsfg_create_member: assemble the geometry first");
+  *out = nullptr;
+  RSFG_HIP(hipSetDevice(geom->device));
+  rsfg_ctx* h = new rsfg_ctx();
+  h->device = geom->device;
+  h->n = geom->n;
+  h->borrowed = true;
+  h->d_geo = geom->d_geo;
+  h->assembled = true;
+  h->has_tri = geom->has_tri;
+  h->slip_mode = geom->slip_mode;
+  h->ncomp = geom->ncomp;
+  h->dim = geom->dim;
+  h->with_normal = geom->with_normal;
+  h->full_space = geom->full_space;
+  h->store = geom->store;
+  h->G = geom->G;
+  h->rows_all = geom->rows_all;
+  h->cols = geom->cols;
+  h->ld = geom->ld;
+  h->scale = geom->scale;
+  h->K64 = geom->K64;
+  h->K32 = geom->K32;
+  h->Kbx2 = geom->Kbx2;
+  h->Kb16 = geom->Kb16;
+  h->kslack = geom->kslack;
+  h->Knf_lo = geom->Knf_lo;
+  h->nf_off = geom->nf_off;
+  h->nf_list = geom->nf_list;
+  h->nf_chunks = geom->nf_chunks;
+  h->nf_tol = geom->nf_tol;
+  hipError_t e = hipStreamCreate(&h->stream);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&h->ev1);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_ctl, sizeof(DevCtl), hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc(&h->d_ctl, sizeof(DevCtl));
+  if (e != hipSuccess) {
+    int rc = hip_fail(e, "rsfg_create_member", __FILE__, __LINE__);
+    rsfg_destroy(h);
+    return rc;
+  }
+  *out = h;
+  return RSFG_OK;
+}
+
 int rsfg_destroy(rsfg_handle* h) {
   if (!h) return RSFG_OK;
   hipSetDevice(h->device);
   if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->borrowed) {
+    // the geometry handle owns the patches and the operator
+    h->K64 = nullptr;
+    h->K32 = nullptr;
+    h->Kbx2 = nullptr;
+    h->Kb16 = nullptr;
+    h->Knf_lo = nullptr;
+    h->nf_off = nullptr;
+    h->nf_list = nullptr;
+    h->d_geo = nullptr;
+  }
   free_operator(h);
   free_state(h);
   hipFree(h->d_geo);
@@ -628,6 +687,8 @@ int rsfg_assemble(rsfg_handle* h, const rsfg_assembly* a, double* gpu_ms) {
   RSFG_CHECK(m == RSFG_SLIP_STRIKE || m == RSFG_SLIP_DIP || m == RSFG_SLIP_RAKE ||
                  m == RSFG_SLIP_VECTOR,
              RSFG_EINVAL, "rsfg_assemble: unknown slip mode");
+  RSFG_CHECK(!h->borrowed, RSFG_ESTATE,
+             "rsfg_assemble: a member handle shares its geometry's operator");
   RSFG_HIP(hipSetDevice(h->device));
   free_operator(h);
   h->slip_mode = m;

@@ -1,0 +1,791 @@
+// Batched ensemble of rate-and-state cycle runs over shared operators.
+//
+// Members are handles made by rsfg_create_member: each owns its friction,
+// loading, state and controller block, and borrows the assembled FP32
+// operator of its geometry.  All members advance together, one step
+// attempt at a time (each with its own step size, accept/reject, events
+// and monitor), so each stage of all members of a geometry is one product
+// K [v_1 .. v_M] with M = 16, 32 or 64 right-hand sides.  That product runs
+// on the matrix cores (v_mfma_f32_16x16x4_f32: FP32 operands, FP32
+// accumulation, flushed to FP64 every 256 columns) and its epilogue is the
+// per-(patch, member) stage update of the single-fault solver.  One attempt
+// of rk5dp for the whole ensemble is k_prep_b, six k_stage_b and k_ctrl_b;
+// the host captures batches of attempts in a hipGraph.  Tile shapes (rows
+// per wave, waves per workgroup, chunks in flight) are chosen at run time
+// by timing the candidates on the ensemble's own stage (rsfg_ensemble_tune).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+#include "stage_dev.h"
+
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+
+namespace rsfg {
+
+struct EnsGeo {        // one geometry of the ensemble
+  const float* K;      // FP32 operator, rows x ld (zero-padded columns)
+  long ld;             // leading dimension of K and of V
+  int rows, n;         // G n operator rows, n patches
+  int m0, nm;          // first member and member count
+  int tile0, ntiles;   // row tiles of this geometry in the tile list
+  float* V[2];         // product inputs by stage parity, nm x ld (member-major)
+};
+
+struct EnsArgs {
+  const StageArgs* args;  // per member
+  const EnsGeo* geo;      // per geometry
+  const int2* tiles;      // (geometry, first row) per workgroup, longest rows first
+  int ngeo;
+  int s;                  // stage
+};
+
+// members of a workgroup's geometry: controller fields read once
+struct MemberSnap {
+  int halted, xc;
+  int ks[kMaxStages];
+  double h;
+};
+
+__device__ __forceinline__ int geo_of_member(const EnsArgs& e, int m) {
+  int g = 0;
+  while (g + 1 < e.ngeo && m >= e.geo[g + 1].m0) ++g;
+  return g;
+}
+
+// v(X) of every member into V[0] (the input of the initial evaluation)
+template <int NC>
+__global__ __launch_bounds__(256) void k_init_b(EnsArgs e) {
+  const int m = blockIdx.y;
+  const StageArgs& a = e.args[m];
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  constexpr int D = Dim<NC>::v;
+  const int n = a.n;
+  const double* X = a.Xb[a.ctl->xcur];
+  const PatchCoef cf = load_coef(a.coef, n, p);
+  double x[D];
+#pragma unroll
+  for (int f = 0; f < D; ++f) x[f] = X[f * n + p];
+  write_vel<float, NC>(a.fp, cf.a, x, static_cast<float*>(const_cast<void*>(a.vin)), p, a.vpl);
+  if (!rsf_patch_ok<NC>(a.fp, cf, x)) atomicOr(&a.ctl->stage_fail, 1);
+}
+
+// first stage of an attempt for every member (product input into V[1])
+template <int NC>
+__global__ __launch_bounds__(256) void k_prep_b(EnsArgs e) {
+  const StageArgs& a = e.args[blockIdx.y];
+  prep_patch<float, NC>(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// step control, one workgroup per member (e.s: 0 after the initial
+// evaluation, S - 1 after an attempt's final stage)
+template <int NC>
+__global__ __launch_bounds__(kCtrlThreads) void k_ctrl_b(EnsArgs e) {
+  const int m = blockIdx.x;
+  ctrl_block<NC>(e.args[m], e.s, e.geo[geo_of_member(e, m)].ntiles);
+}
+
+// per member: halted flag and attempts made (steps + rejects), for the
+// host's termination and progress checks
+__global__ void k_halted_b(EnsArgs e, int M, long long* out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < M) {
+    const DevCtl* c = e.args[m].ctl;
+    out[2 * m] = c->halted;
+    out[2 * m + 1] = c->steps + c->rejects;
+  }
+}
+
+// The rows [rw, rw + 16 RF) of one wave against all 16 MF members over the
+// 16-column chunks [kc0, kc1).  Lane (j, i) = (lane / 16, lane % 16) loads
+// the 16-byte slice K[row i][k0 + 4j .. k0 + 4j + 3] and V[member i][same
+// columns]; the four MFMAs of a chunk take component q of both, so A[i][j]
+// and B[j][i] index the same column k0 + 4j + q.  FP32 partial sums are
+// added into FP64 every 16 chunks (256 columns).  Output lane layout
+// (checked on the device): acc[rf][mf][r] = Y[row 16 rf + 4 (lane / 16) + r]
+// [member 16 mf + lane % 16].
+template <int RF, int MF, int U>
+__device__ __forceinline__ void wave_product(const float* __restrict__ K, long ld, int rw,
+                                             int rows, const float* __restrict__ V, int kc0,
+                                             int kc1, double (&acc)[RF][MF][4]) {
+  const int lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
+  const float* ka[RF];
+  const float* vb[MF];
+#pragma unroll
+  for (int rf = 0; rf < RF; ++rf) ka[rf] = K + (size_t)min(rw + 16 * rf + i, rows - 1) * ld + 4 * j;
+#pragma unroll
+  for (int mf = 0; mf < MF; ++mf) vb[mf] = V + (size_t)(16 * mf + i) * ld + 4 * j;
+#pragma unroll
+  for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+    for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[rf][mf][r] = 0.0;
+  for (int kb = kc0; kb < kc1; kb += 16) {
+    v4f_t c[RF][MF];
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf) c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
+    const int ke = min(kb + 16, kc1);
+    for (int kc = kb; kc < ke; kc += U) {
+      v4f_t A[U][RF], B[U][MF];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+#pragma unroll
+        for (int rf = 0; rf < RF; ++rf)
+          A[u][rf] = __builtin_nontemporal_load(
+              reinterpret_cast<const v4f_t*>(ka[rf] + (size_t)(kc + u) * 16));
+#pragma unroll
+        for (int mf = 0; mf < MF; ++mf)
+          B[u][mf] = *reinterpret_cast<const v4f_t*>(vb[mf] + (size_t)(kc + u) * 16);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+            for (int mf = 0; mf < MF; ++mf)
+              c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u][rf][q], B[u][mf][q],
+                                                                c[rf][mf], 0, 0, 0);
+    }
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[rf][mf][r] += (double)c[rf][mf][r];
+  }
+}
+
+// Stage s update of patch p of one member from its elastic stressing rates
+// td (G rows): the epilogue of k_stage, written for one (patch, member)
+// pair with its operands fetched here.
+template <int NC, int G>
+__device__ __forceinline__ void member_patch_stage(const StageArgs& a, const MemberSnap& ms,
+                                                   int s, int p, const double* td, float* vnext,
+                                                   double* r9) {
+  constexpr int D = Dim<NC>::v;
+  constexpr int PD = PreDim<NC>::v;
+  const int n = a.n, S = a.tb.s;
+  const double* X = ms.xc ? a.Xb[1] : a.Xb[0];
+  double* Xn = ms.xc ? a.Xb[0] : a.Xb[1];
+  const size_t sst = a.Stg[1] - a.Stg[0];
+  const bool reduce = (s == 0) || (s == S - 1);
+  const bool mid = (s > 0) && (s < S - 1);
+  const PatchCoef cf = load_coef(a.coef, n, p);
+  double P[PD], y[D], kf[D], sn[NC];
+  if (reduce) {
+    const double* Yc = (s == 0) ? X : Xn;
+#pragma unroll
+    for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
+  }
+  if (s == 0) {
+    rsf_patch_pre<NC>(a.fp, cf, y, P);
+  } else {
+    const double* Ps = a.Pb[s & 1];
+#pragma unroll
+    for (int k = 0; k < PD; ++k) P[k] = Ps[k * n + p];
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) sn[k] = a.sinc[NC * p + k];
+  const double snn = a.sinc[NC * n + p];
+  const double sdn = (G > NC) ? td[NC] : 0.0;
+  rsf_patch_finish<NC>(a.fp, P, td, sdn, sn, snn, kf);
+  double* Ks = a.Stg[0] + (size_t)ms.ks[s] * sst;
+#pragma unroll
+  for (int f = 0; f < D; ++f) Ks[f * n + p] = kf[f];
+  if (mid) {
+    double ws[D];
+#pragma unroll
+    for (int f = 0; f < D; ++f) ws[f] = 0.0;
+    for (int j = 0; j < s; ++j) {
+      const double w = a.tb.A[s + 1][j];
+      if (w == 0.0) continue;
+      const double* Kj = a.Stg[0] + (size_t)ms.ks[j] * sst;
+#pragma unroll
+      for (int f = 0; f < D; ++f) ws[f] += w * Kj[f * n + p];
+    }
+    const double w = a.tb.A[s + 1][s];
+    double* Yn = (s + 1 == S - 1) ? Xn : a.Yb[(s + 1) & 1];
+    double yn[D];
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+      yn[f] = X[f * n + p] + ms.h * (ws[f] + w * kf[f]);
+      Yn[f * n + p] = yn[f];
+    }
+    double Pn[PD];
+    rsf_patch_pre<NC>(a.fp, cf, yn, Pn);
+    store_pre<NC>(Pn, a.Pb[(s + 1) & 1], n, p);
+    write_vel_pre<float, NC>(Pn, vnext, p, a.vpl);
+    if (!rsf_patch_ok<NC>(a.fp, cf, yn)) atomicOr(&a.ctl->stage_fail, 1);
+    return;
+  }
+  if (s == S - 1) {
+    double en = 0.0;
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+      double e = 0.0;
+      for (int j = 0; j < S - 1; ++j) {
+        const double w = a.tb.bh[j];
+        if (w != 0.0) e += w * a.Stg[0][(size_t)ms.ks[j] * sst + f * n + p];
+      }
+      e += a.tb.bh[S - 1] * kf[f];
+      const double ye = X[f * n + p] + ms.h * e;
+      const double tol = a.so.atol[f] + a.so.rtol * fmax(fabs(y[f]), fabs(ye));
+      if (tol > 0.0) en = fmax(en, fabs(y[f] - ye) / tol);
+      const double dm = a.Xmon[f * n + p] - y[f];
+      r9[RED_DX2] += dm * dm;
+    }
+    r9[RED_ERR] = en;
+  }
+  const double vm = pre_vmag<NC>(P);
+  r9[RED_VMAX] = vm;
+  r9[RED_VSUM] = vm;
+  r9[RED_SLIP] = y[NC + 2];
+  r9[RED_MU] = y[1] / y[NC + 1];
+  r9[RED_SMAX] = y[NC + 1];
+  r9[RED_SMIN] = y[NC + 1];
+#pragma unroll
+  for (int f = 0; f < D; ++f) r9[RED_XN2] += y[f] * y[f];
+}
+
+// One workgroup = one row tile of one geometry for all of its 16 MF
+// members.  Its WR x WK waves split the tile's 16 RF WR rows into WR row
+// groups and the columns into WK contiguous ranges, so that many short
+// streams run at once; the WK partial products of a row group are summed
+// in LDS in a fixed order.  Then every (patch, member) pair of the tile
+// gets its stage update.  The tile's patch count 16 RF WR / G is a power of
+// two of at most 64, so one member's pairs are consecutive lanes of one
+// wave and its reductions are shuffles.
+template <int NC, int G, int RF, int MF, int WR, int WK, int U>
+__global__ __launch_bounds__(64 * WR * WK) void k_stage_b(EnsArgs e) {
+  constexpr int W = WR * WK;
+  constexpr int TR = 16 * RF * WR;  // rows per tile
+  constexpr int NPT = TR / G;       // patches per tile
+  constexpr int NM = 16 * MF;       // members per geometry
+  static_assert(NPT == 8 || NPT == 16 || NPT == 32 || NPT == 64, "tile patches");
+  __shared__ double Yp[WK * TR * NM];
+  __shared__ MemberSnap snap[NM];
+  const int2 tile = e.tiles[blockIdx.x];
+  const EnsGeo& gg = e.geo[tile.x];
+  const int r0 = tile.y, s = e.s;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = wave / WK, wk = wave % WK;
+  for (int m = tid; m < NM; m += blockDim.x) {
+    const DevCtl* c = e.args[gg.m0 + m].ctl;
+    MemberSnap q;
+    q.halted = c->halted;
+    q.xc = c->xcur;
+    for (int k = 0; k < kMaxStages; ++k) q.ks[k] = c->kslot[k];
+    q.h = c->h;
+    snap[m] = q;
+  }
+  {
+    // column range of this wave, in whole groups of U chunks
+    const int nk = (int)(gg.ld / 16);
+    const int per = ((nk + WK - 1) / WK + U - 1) / U * U;
+    const int kc0 = min(nk, wk * per), kc1 = min(nk, kc0 + per);
+    double acc[RF][MF][4];
+    wave_product<RF, MF, U>(gg.K, gg.ld, r0 + wr * 16 * RF, gg.rows, gg.V[s & 1], kc0, kc1, acc);
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wr * 16 * RF + 16 * rf + 4 * (lane >> 4) + r;
+          Yp[(wk * TR + row) * NM + 16 * mf + (lane & 15)] = acc[rf][mf][r];
+        }
+  }
+  __syncthreads();
+  if (WK > 1) {
+    for (int i = tid; i < TR * NM; i += blockDim.x) {
+      double v = Yp[i];
+      for (int k = 1; k < WK; ++k) v += Yp[k * TR * NM + i];
+      Yp[i] = v;
+    }
+    __syncthreads();
+  }
+  const int S = e.args[gg.m0].tb.s;
+  const bool reduce = (s == 0) || (s == S - 1);
+  const int pt0 = r0 / G;
+  float* vnext = gg.V[(s + 1) & 1];
+  for (int q0 = 0; q0 < NPT * NM; q0 += 64 * W) {
+    const int q = q0 + tid;
+    const int m = q / NPT, pl = q % NPT, p = pt0 + pl;
+    double r9[kNRed];
+    r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
+    r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
+    r9[RED_DX2] = 0.0;
+    if (m < NM && p < gg.n && !snap[m].halted) {
+      double td[G];
+#pragma unroll
+      for (int k = 0; k < G; ++k) td[k] = Yp[(pl * G + k) * NM + m];
+      member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[m], s, p, td,
+                                vnext + (size_t)m * gg.ld, r9);
+    }
+    if (reduce) {
+      // the NPT pairs of member m are NPT consecutive lanes of one wave
+#pragma unroll
+      for (int k = 0; k < kNRed; ++k) {
+        double v = r9[k];
+        const int op = red_kind(k);
+#pragma unroll
+        for (int off = NPT / 2; off > 0; off >>= 1) v = red_op(op, v, __shfl_xor(v, off, 64));
+        if (pl == 0 && m < NM) {
+          const StageArgs& a = e.args[gg.m0 + m];
+          a.partials[(size_t)k * gg.ntiles + (blockIdx.x - gg.tile0)] = v;
+        }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------ host side
+struct EnsCfg {
+  int rf, wr, wk, u;
+};
+// candidate tile shapes: 16 RF rows per wave, WR row groups x WK column
+// ranges of waves per workgroup, U chunks in flight per wave
+static const EnsCfg kEnsCfgs[] = {{1, 1, 4, 2}, {1, 1, 4, 4}, {1, 1, 8, 2}, {1, 2, 4, 2},
+                                  {2, 1, 4, 2}, {1, 2, 2, 4}, {1, 4, 1, 2}, {1, 1, 2, 4}};
+constexpr int kEnsNCfg = sizeof(kEnsCfgs) / sizeof(kEnsCfgs[0]);
+
+static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].wr; }
+
+typedef void (*EnsStageFn)(const EnsArgs&, int nblocks, hipStream_t);
+
+template <int NC, int G, int RF, int MF, int WR, int WK, int U>
+static void launch_stage_b(const EnsArgs& e, int nblocks, hipStream_t st) {
+  // shapes whose tiles do not hold 8 to 64 patches are never selected
+  // (stage_fn returns no launcher for them) and are not instantiated
+  constexpr int npt = 16 * RF * WR / G;
+  if constexpr (npt == 8 || npt == 16 || npt == 32 || npt == 64) {
+    hipLaunchKernelGGL((k_stage_b<NC, G, RF, MF, WR, WK, U>), dim3(nblocks),
+                       dim3(64 * WR * WK), 0, st, e);
+  }
+}
+
+template <int NC, int G, int MF>
+static EnsStageFn stage_fn_mf(int cfg) {
+  switch (cfg) {
+    case 0: return launch_stage_b<NC, G, 1, MF, 1, 4, 2>;
+    case 1: return launch_stage_b<NC, G, 1, MF, 1, 4, 4>;
+    case 2: return launch_stage_b<NC, G, 1, MF, 1, 8, 2>;
+    case 3: return launch_stage_b<NC, G, 1, MF, 2, 4, 2>;
+    case 4: return launch_stage_b<NC, G, 2, MF, 1, 4, 2>;
+    case 5: return launch_stage_b<NC, G, 1, MF, 2, 2, 4>;
+    case 6: return launch_stage_b<NC, G, 1, MF, 4, 1, 2>;
+    default: return launch_stage_b<NC, G, 1, MF, 1, 2, 4>;
+  }
+}
+
+static EnsStageFn stage_fn(int ncomp, int G, int nm, int cfg) {
+  const int npt = cfg_rows(cfg) / G;
+  if (npt != 8 && npt != 16 && npt != 32 && npt != 64) return nullptr;
+  if (ncomp == 2 && G == 2) {
+    if (nm == 16) return stage_fn_mf<2, 2, 1>(cfg);
+    if (nm == 32) return stage_fn_mf<2, 2, 2>(cfg);
+    if (nm == 64) return stage_fn_mf<2, 2, 4>(cfg);
+  }
+  if (ncomp == 1 && G == 1) {
+    if (nm == 16) return stage_fn_mf<1, 1, 1>(cfg);
+    if (nm == 32) return stage_fn_mf<1, 1, 2>(cfg);
+    if (nm == 64) return stage_fn_mf<1, 1, 4>(cfg);
+  }
+  return nullptr;
+}
+
+}  // namespace rsfg
+
+using namespace rsfg;
+
+struct rsfg_ensemble {
+  int device = 0;
+  int ngeo = 0, nm = 0, M = 0, ncomp = 1, G = 1;
+  std::vector<rsfg_handle*> geoms;
+  std::vector<rsfg_handle*> members;
+  std::vector<EnsGeo> geo;
+  EnsGeo* d_geo = nullptr;
+  StageArgs* d_args = nullptr;
+  int2* d_tiles = nullptr;
+  int ntiles = 0;
+  int cfg = 0;
+  float* d_v = nullptr;
+  long long* d_halted = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// tile list for configuration cfg: per geometry ceil(rows / TR) row tiles,
+// ordered by decreasing row length (longest first: the dispatcher then
+// fills the CUs longest-job-first)
+static int build_tiles(rsfg_ensemble* e, int cfg) {
+  const int TR = cfg_rows(cfg);
+  std::vector<int> order(e->ngeo);
+  for (int g = 0; g < e->ngeo; ++g) order[g] = g;
+  std::sort(order.begin(), order.end(),
+            [&](int x, int y) { return e->geo[x].ld > e->geo[y].ld; });
+  std::vector<int2> tiles;
+  for (int g : order) {
+    e->geo[g].tile0 = (int)tiles.size();
+    e->geo[g].ntiles = (e->geo[g].rows + TR - 1) / TR;
+    for (int t = 0; t < e->geo[g].ntiles; ++t) tiles.push_back(make_int2(g, t * TR));
+  }
+  hipFree(e->d_tiles);
+  e->d_tiles = nullptr;
+  RSFG_HIP(hipMalloc(&e->d_tiles, sizeof(int2) * tiles.size()));
+  RSFG_HIP(hipMemcpy(e->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
+  RSFG_HIP(hipMemcpy(e->d_geo, e->geo.data(), sizeof(EnsGeo) * e->ngeo, hipMemcpyHostToDevice));
+  e->ntiles = (int)tiles.size();
+  e->cfg = cfg;
+  return RSFG_OK;
+}
+
+// per-member argument blocks for a solve with options o
+static int fill_member_args(rsfg_ensemble* e, const rsfg_solver_opts* o) {
+  std::vector<StageArgs> args(e->M);
+  for (int g = 0; g < e->ngeo; ++g) {
+    for (int k = 0; k < e->nm; ++k) {
+      const int m = g * e->nm + k;
+      rsfg_ctx* h = e->members[m];
+      RSFG_CHECK(h->have_friction && h->have_loading && h->have_state, RSFG_ESTATE,
+                 "rsfg_ensemble: member " + std::to_string(m) +
+                     " needs friction, loading and a state");
+      fill_stage_args(h, o, e->geo[g].K, args[m]);
+      args[m].vin = e->geo[g].V[0] + (size_t)k * e->geo[g].ld;   // initial evaluation
+      args[m].vout = e->geo[g].V[1] + (size_t)k * e->geo[g].ld;  // first stage of an attempt
+    }
+  }
+  RSFG_HIP(hipMemcpy(e->d_args, args.data(), sizeof(StageArgs) * e->M, hipMemcpyHostToDevice));
+  return RSFG_OK;
+}
+
+static EnsArgs ens_args(rsfg_ensemble* e, int s) {
+  EnsArgs a;
+  a.args = e->d_args;
+  a.geo = e->d_geo;
+  a.tiles = e->d_tiles;
+  a.ngeo = e->ngeo;
+  a.s = s;
+  return a;
+}
+
+static int max_patches(rsfg_ensemble* e) {
+  int nmax = 0;
+  for (auto& g : e->geo) nmax = std::max(nmax, g.n);
+  return nmax;
+}
+
+// controller blocks of every member at the start of a solve, then v(X),
+// the initial evaluation and the initial controller pass
+static int ens_start(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_end) {
+  for (int m = 0; m < e->M; ++m) {
+    rsfg_ctx* h = e->members[m];
+    DevCtl* c = h->h_ctl;
+    memset(c, 0, sizeof(*c));
+    c->t = h->t;
+    c->h = o->dt_init;
+    c->t_end = t_end;
+    c->hmax = (o->dt_max < o->dt_monitor) ? o->dt_max : o->dt_monitor;
+    c->max_steps = o->max_steps;
+    for (int i = 0; i < 8; ++i) c->kslot[i] = i;
+    c->max_event = kMaxEv;
+    c->max_mon = kMaxMon;
+    c->xcur = h->xcur;
+    RSFG_HIP(hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, e->stream));
+    RSFG_HIP(hipMemsetAsync(h->d_rupt, 0, sizeof(int) * h->n, e->stream));
+  }
+  const dim3 gp((max_patches(e) + 255) / 256, e->M);
+  EnsArgs a = ens_args(e, 0);
+  if (e->ncomp == 2) hipLaunchKernelGGL(k_init_b<2>, gp, dim3(256), 0, e->stream, a);
+  else hipLaunchKernelGGL(k_init_b<1>, gp, dim3(256), 0, e->stream, a);
+  stage_fn(e->ncomp, e->G, e->nm, e->cfg)(a, e->ntiles, e->stream);
+  if (e->ncomp == 2) hipLaunchKernelGGL(k_ctrl_b<2>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
+  else hipLaunchKernelGGL(k_ctrl_b<1>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
+  RSFG_HIP(hipGetLastError());
+  RSFG_HIP(hipStreamSynchronize(e->stream));
+  return RSFG_OK;
+}
+
+static void ens_attempt(rsfg_ensemble* e, int S) {
+  const dim3 gp((max_patches(e) + 255) / 256, e->M);
+  EnsStageFn sf = stage_fn(e->ncomp, e->G, e->nm, e->cfg);
+  EnsArgs a = ens_args(e, 0);
+  if (e->ncomp == 2) hipLaunchKernelGGL(k_prep_b<2>, gp, dim3(256), 0, e->stream, a);
+  else hipLaunchKernelGGL(k_prep_b<1>, gp, dim3(256), 0, e->stream, a);
+  for (int s = 1; s < S; ++s) {
+    a.s = s;
+    sf(a, e->ntiles, e->stream);
+  }
+  if (e->ncomp == 2) hipLaunchKernelGGL(k_ctrl_b<2>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
+  else hipLaunchKernelGGL(k_ctrl_b<1>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
+}
+
+extern "C" {
+
+int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_geom,
+                         rsfg_ensemble** out) {
+  RSFG_CHECK(geoms && out && ngeom > 0, RSFG_EINVAL, "rsfg_ensemble_create: bad arguments");
+  RSFG_CHECK(members_per_geom == 16 || members_per_geom == 32 || members_per_geom == 64,
+             RSFG_EINVAL, "rsfg_ensemble_create: members per geometry must be 16, 32 or 64");
+  *out = nullptr;
+  const int ncomp = geoms[0]->ncomp, G = geoms[0]->G;
+  for (int g = 0; g < ngeom; ++g) {
+    RSFG_CHECK(geoms[g] && !geoms[g]->borrowed && geoms[g]->assembled && geoms[g]->K32,
+               RSFG_ESTATE, "rsfg_ensemble_create: every geometry needs an FP32 operator");
+    RSFG_CHECK(geoms[g]->device == geoms[0]->device && geoms[g]->ncomp == ncomp &&
+                   geoms[g]->G == G,
+               RSFG_EINVAL, "rsfg_ensemble_create: geometries differ in device or layout");
+  }
+  RSFG_CHECK((ncomp == 2 && G == 2) || (ncomp == 1 && G == 1), RSFG_EINVAL,
+             "rsfg_ensemble_create: supported layouts are one or two shear components "
+             "without normal rows");
+  RSFG_HIP(hipSetDevice(geoms[0]->device));
+  rsfg_ensemble* e = new rsfg_ensemble();
+  e->device = geoms[0]->device;
+  e->ngeo = ngeom;
+  e->nm = members_per_geom;
+  e->M = ngeom * members_per_geom;
+  e->ncomp = ncomp;
+  e->G = G;
+  size_t vtot = 0;
+  for (int g = 0; g < ngeom; ++g) {
+    e->geoms.push_back(geoms[g]);
+    EnsGeo q;
+    memset(&q, 0, sizeof q);
+    q.K = geoms[g]->K32;
+    q.ld = geoms[g]->ld;
+    q.rows = geoms[g]->rows_all;
+    q.n = geoms[g]->n;
+    q.m0 = g * members_per_geom;
+    q.nm = members_per_geom;
+    e->geo.push_back(q);
+    vtot += 2 * (size_t)members_per_geom * q.ld;
+  }
+  int rc = RSFG_OK;
+  hipError_t he = hipStreamCreate(&e->stream);
+  if (he == hipSuccess) he = hipEventCreate(&e->ev0);
+  if (he == hipSuccess) he = hipEventCreate(&e->ev1);
+  if (he == hipSuccess) he = hipMalloc(&e->d_v, sizeof(float) * vtot);
+  if (he == hipSuccess) he = hipMemset(e->d_v, 0, sizeof(float) * vtot);
+  if (he == hipSuccess) he = hipMalloc(&e->d_geo, sizeof(EnsGeo) * ngeom);
+  if (he == hipSuccess) he = hipMalloc(&e->d_args, sizeof(StageArgs) * e->M);
+  if (he == hipSuccess) he = hipMalloc(&e->d_halted, 2 * sizeof(long long) * e->M);
+  if (he != hipSuccess) rc = hip_fail(he, "rsfg_ensemble_create", __FILE__, __LINE__);
+  if (rc == RSFG_OK) {
+    float* v = e->d_v;
+    for (auto& q : e->geo) {
+      q.V[0] = v;
+      v += (size_t)q.nm * q.ld;
+      q.V[1] = v;
+      v += (size_t)q.nm * q.ld;
+    }
+    for (int g = 0; g < ngeom && rc == RSFG_OK; ++g)
+      for (int k = 0; k < members_per_geom && rc == RSFG_OK; ++k) {
+        rsfg_handle* mh = nullptr;
+        rc = rsfg_create_member(geoms[g], &mh);
+        if (rc == RSFG_OK) e->members.push_back(mh);
+      }
+  }
+  if (rc == RSFG_OK) {
+    // default tile shape until rsfg_ensemble_tune picks one
+    int cfg = 0;
+    while (cfg < kEnsNCfg && !stage_fn(ncomp, G, e->nm, cfg)) ++cfg;
+    rc = build_tiles(e, cfg);
+  }
+  if (rc != RSFG_OK) {
+    rsfg_ensemble_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return RSFG_OK;
+}
+
+int rsfg_ensemble_member(rsfg_ensemble* e, int m, rsfg_handle** h) {
+  RSFG_CHECK(e && h && m >= 0 && m < e->M, RSFG_EINVAL, "rsfg_ensemble_member: bad member");
+  *h = e->members[m];
+  return RSFG_OK;
+}
+
+int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_end,
+                        rsfg_stats* st) {
+  RSFG_CHECK(e && o, RSFG_EINVAL, "rsfg_ensemble_solve: null argument");
+  RSFG_CHECK(o->method == RSFG_RK5DP || o->method == RSFG_RK3BS, RSFG_EINVAL,
+             "rsfg_ensemble_solve: method must be rk5dp or rk3bs");
+  RSFG_CHECK(o->prec == RSFG_PREC_FP32, RSFG_EINVAL,
+             "rsfg_ensemble_solve: the batched product uses the FP32 operator (prec FP32)");
+  RSFG_HIP(hipSetDevice(e->device));
+  const auto wall0 = std::chrono::steady_clock::now();
+  for (int m = 0; m < e->M; ++m) {
+    rsfg_ctx* h = e->members[m];
+    RSFG_CHECK((size_t)e->geo[m / e->nm].ntiles * kNRed <= h->partial_count, RSFG_ESTATE,
+               "rsfg_ensemble_solve: partial buffer too small");
+  }
+  int rc = fill_member_args(e, o);
+  if (rc) return rc;
+  rc = ens_start(e, o, t_end);
+  if (rc) return rc;
+  Tableau tb;
+  {
+    StageArgs a0;
+    fill_stage_args(e->members[0], o, e->geo[0].K, a0);
+    tb = a0.tb;
+  }
+  const int per = o->steps_per_launch > 0 ? o->steps_per_launch : 32;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  RSFG_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < per; ++k) ens_attempt(e, tb.s);
+  hipLaunchKernelGGL(k_halted_b, dim3((e->M + 255) / 256), dim3(256), 0, e->stream,
+                     ens_args(e, 0), e->M, e->d_halted);
+  RSFG_HIP(hipStreamEndCapture(e->stream, &graph));
+  RSFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  std::vector<long long> hs(2 * e->M, 0), prev(e->M, -1);
+  long launches = 0;
+  for (;;) {
+    hipError_t he = hipGraphLaunch(exec, e->stream);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(hs.data(), e->d_halted, 2 * sizeof(long long) * e->M,
+                          hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) {
+      rc = hip_fail(he, "rsfg_ensemble_solve batch launch", __FILE__, __LINE__);
+      break;
+    }
+    ++launches;
+    bool all = true, stuck = false;
+    for (int m = 0; m < e->M; ++m) {
+      if (!hs[2 * m]) {
+        all = false;
+        // a running member makes one attempt per captured attempt
+        if (hs[2 * m + 1] == prev[m]) stuck = true;
+      }
+      prev[m] = hs[2 * m + 1];
+    }
+    if (all) break;
+    if (stuck) {
+      set_error("rsfg_ensemble_solve: a running member made no step attempt in a batch");
+      rc = RSFG_ESTATE;
+      break;
+    }
+  }
+  hipGraphExecDestroy(exec);
+  hipGraphDestroy(graph);
+  if (rc) return rc;
+  const double wall =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+  for (int m = 0; m < e->M; ++m) {
+    rsfg_ctx* h = e->members[m];
+    DevCtl* c = h->h_ctl;
+    RSFG_HIP(hipMemcpy(c, h->d_ctl, sizeof(DevCtl), hipMemcpyDeviceToHost));
+    h->xcur = c->xcur;
+    h->t = c->t;
+    h->nevent = c->nevent;
+    h->nmon = c->nmon;
+    if (st) {
+      rsfg_stats& q = st[m];
+      memset(&q, 0, sizeof q);
+      q.steps = c->steps;
+      q.rejects = c->rejects;
+      q.domain_rejects = c->domain_rejects;
+      q.rhs_evals = c->rhs_evals;
+      q.launches = launches;
+      q.t = c->t;
+      q.dt = c->h;
+      q.status = c->status == 2 ? RSFG_OK : c->status;
+      q.nevent = std::min(c->nevent, c->max_event);
+      q.nmonitor = std::min(c->nmon, c->max_mon);
+      q.wall_s = wall;
+    }
+  }
+  return RSFG_OK;
+}
+
+int rsfg_ensemble_tune(rsfg_ensemble* e, const rsfg_solver_opts* o, int* choice, double* ms) {
+  RSFG_CHECK(e && o, RSFG_EINVAL, "rsfg_ensemble_tune: null argument");
+  RSFG_HIP(hipSetDevice(e->device));
+  rsfg_solver_opts oo = *o;
+  oo.prec = RSFG_PREC_FP32;
+  int rc = fill_member_args(e, &oo);
+  if (rc) return rc;
+  const double t_end = 1e300;
+  int best = -1;
+  float best_ms = 1e30f;
+  const dim3 gp((max_patches(e) + 255) / 256, e->M);
+  for (int cfg = 0; cfg < kEnsNCfg; ++cfg) {
+    if (ms) ms[cfg] = -1.0;
+    if (!stage_fn(e->ncomp, e->G, e->nm, cfg)) continue;
+    rc = build_tiles(e, cfg);
+    if (rc) return rc;
+    rc = ens_start(e, &oo, t_end);
+    if (rc) return rc;
+    EnsArgs a = ens_args(e, 0);
+    if (e->ncomp == 2) hipLaunchKernelGGL(k_prep_b<2>, gp, dim3(256), 0, e->stream, a);
+    else hipLaunchKernelGGL(k_prep_b<1>, gp, dim3(256), 0, e->stream, a);
+    // a middle stage, timed over repeated launches (it rewrites the same
+    // stage buffers; the solve starts again from the accepted state)
+    a.s = 1;
+    EnsStageFn sf = stage_fn(e->ncomp, e->G, e->nm, cfg);
+    sf(a, e->ntiles, e->stream);
+    RSFG_HIP(hipEventRecord(e->ev0, e->stream));
+    const int nrep = 20;
+    for (int r = 0; r < nrep; ++r) sf(a, e->ntiles, e->stream);
+    RSFG_HIP(hipEventRecord(e->ev1, e->stream));
+    RSFG_HIP(hipEventSynchronize(e->ev1));
+    float t = 0.f;
+    RSFG_HIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
+    t /= nrep;
+    if (ms) ms[cfg] = t;
+    if (t < best_ms) {
+      best_ms = t;
+      best = cfg;
+    }
+  }
+  RSFG_CHECK(best >= 0, RSFG_EINVAL, "rsfg_ensemble_tune: no tile shape fits this layout");
+  rc = build_tiles(e, best);
+  if (rc) return rc;
+  if (choice) *choice = best;
+  return RSFG_OK;
+}
+
+int rsfg_ensemble_config(rsfg_ensemble* e, int* cfg, int* ncfg, int* shape) {
+  RSFG_CHECK(e, RSFG_EINVAL, "rsfg_ensemble_config: null argument");
+  if (cfg) *cfg = e->cfg;
+  if (ncfg) *ncfg = kEnsNCfg;
+  if (shape) {
+    for (int c = 0; c < kEnsNCfg; ++c) {
+      shape[4 * c] = 16 * kEnsCfgs[c].rf;
+      shape[4 * c + 1] = kEnsCfgs[c].wr;
+      shape[4 * c + 2] = kEnsCfgs[c].wk;
+      shape[4 * c + 3] = kEnsCfgs[c].u;
+    }
+  }
+  return RSFG_OK;
+}
+
+int rsfg_ensemble_destroy(rsfg_ensemble* e) {
+  if (!e) return RSFG_OK;
+  hipSetDevice(e->device);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  for (rsfg_handle* h : e->members) rsfg_destroy(h);
+  hipFree(e->d_v);
+  hipFree(e->d_geo);
+  hipFree(e->d_args);
+  hipFree(e->d_tiles);
+  hipFree(e->d_halted);
+  if (e->ev0) hipEventDestroy(e->ev0);
+  if (e->ev1) hipEventDestroy(e->ev1);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
+  return RSFG_OK;
+}
+
+}  // extern "C"

@@ -94,6 +94,7 @@ struct SolveOpts {
 struct rsfg_ctx {
   int device = 0;
   int n = 0;                  // patches
+  bool borrowed = false;      // geometry and operator belong to another handle
   hipStream_t stream = nullptr;
   // geometry (device, SoA)
   double* d_geo = nullptr;    // kGeoFields x n, see kGeo*

@@ -149,3 +149,24 @@ def cycle_fields(g, seed=20261202, a_vw=0.004, b0=0.03, amb_excess=0.04,
     tau = sigma * a * np.arcsinh(0.5 * v / v0 * np.exp(psi_ss / a)) + eta * v
     return dict(a=a, b=b, dc=dc, tau=tau, vel=v, sigma=np.full_like(x, sigma),
                 f0=f0, v0=v0, vpl=vpl, eta=eta, shear_modulus=shear_modulus, cs=cs)
+
+
+def ensemble_fields(g, length_km, depth_km, member, geom_index=0, seed=20261202):
+    """Friction fields of one ensemble member on a planar outline of
+    length_km x depth_km: the benchmark fields with the seismogenic box,
+    taper and nucleation square scaled to the outline (box |along| <=
+    0.3 L, depth 0.25 W +- 0.15 W; nucleation square of half-size 0.06 L at
+    along = -0.24 L) and three seeded draws per member: a - b in the box
+    uniform in [-0.03, -0.02] (a = 0.004), D_c log-uniform in [0.05, 0.3] m
+    (0.93 D_c in the nucleation square), sigma_n uniform in [20, 40] MPa.
+    The draws come from a PCG64 stream keyed by (seed, geometry, member)."""
+    rng = np.random.Generator(np.random.PCG64([seed, geom_index, member]))
+    amb = rng.uniform(-0.03, -0.02)
+    dc = float(np.exp(rng.uniform(np.log(0.05), np.log(0.3))))
+    sigma = rng.uniform(20e6, 40e6)
+    L, W = float(length_km), float(depth_km)
+    f = cycle_fields(g, seed=seed + 1000 * geom_index + member, a_vw=0.004, b0=0.004 - amb,
+                     dc0=dc, dc_nuc=0.93 * dc, sigma=sigma,
+                     box=(0.3 * L, 0.25 * W, 0.15 * W), nuc=(-0.24 * L, 0.25 * W, 0.06 * L))
+    f.update(amb=amb, dc_draw=dc, sigma_draw=sigma)
+    return f

@@ -35,6 +35,9 @@ EXPORTS = [
     "rsfg_solver_defaults", "rsfg_set_state", "rsfg_get_state", "rsfg_solve",
     "rsfg_events", "rsfg_monitor", "rsfg_stage_bench",
     "rsfg_read_table", "rsfg_write_checkpoint", "rsfg_read_checkpoint",
+    "rsfg_create_member", "rsfg_ensemble_create", "rsfg_ensemble_member",
+    "rsfg_ensemble_solve", "rsfg_ensemble_tune", "rsfg_ensemble_config",
+    "rsfg_ensemble_destroy",
 ]
 
 
@@ -106,6 +109,17 @@ def load(path=None):
     L.rsfg_create.argtypes = [C.c_int, C.POINTER(Patch), C.c_int,
                               C.POINTER(C.c_void_p)]
     L.rsfg_destroy.argtypes = [h]
+    L.rsfg_create_member.argtypes = [h, C.POINTER(C.c_void_p)]
+    L.rsfg_ensemble_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int,
+                                       C.POINTER(C.c_void_p)]
+    L.rsfg_ensemble_member.argtypes = [h, C.c_int, C.POINTER(C.c_void_p)]
+    L.rsfg_ensemble_solve.argtypes = [h, C.POINTER(SolverOpts), C.c_double,
+                                      C.POINTER(Stats)]
+    L.rsfg_ensemble_tune.argtypes = [h, C.POINTER(SolverOpts), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]
+    L.rsfg_ensemble_config.argtypes = [h, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]
+    L.rsfg_ensemble_destroy.argtypes = [h]
     L.rsfg_read_patch_file.argtypes = [C.c_char_p, C.POINTER(Patch),
                                        C.POINTER(C.c_int)]
     L.rsfg_assemble.argtypes = [h, C.POINTER(Assembly), C.POINTER(C.c_double)]

@@ -38,11 +38,29 @@ class FaultSystem:
         self.G = 1
         self.precisions = 0
         self.assembly_ms = None
+        self._members = []
+
+    def member(self):
+        """A member system sharing these patches and the assembled operators
+        (rsfg_create_member): own friction, loading and state."""
+        m = FaultSystem.__new__(FaultSystem)
+        m._lib, m.n, m._patches = self._lib, self.n, self._patches
+        hh = C.c_void_p()
+        L.check(self._lib.rsfg_create_member(self.h, C.byref(hh)), "rsfg_create_member")
+        m.h = hh
+        m.ncomp, m.G, m.precisions, m.assembly_ms = self.ncomp, self.G, self.precisions, None
+        m.dim = getattr(self, "dim", 2 * self.ncomp + 2)
+        m._members = []
+        m._parent = self
+        self._members.append(m)
+        return m
 
     def close(self):
-        if self.h:
+        for m in getattr(self, "_members", []):
+            m.close()
+        if self.h and getattr(self, "_owned", True):
             self._lib.rsfg_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -228,6 +246,80 @@ class FaultSystem:
         n = C.c_int(cap)
         L.check(self._lib.rsfg_monitor(self.h, rows, C.byref(n)), "monitor")
         return rows[:10 * n.value].reshape(-1, 10).copy()
+
+
+class Ensemble:
+    """Batched runs over shared operators (rsfg_ensemble_*): members_per_geom
+    members for each geometry (FaultSystems with an FP32 operator).  Members
+    are configured like single systems (set_friction, set_loading,
+    set_state) and read out the same way (events, get_state)."""
+
+    def __init__(self, geoms, members_per_geom=32):
+        self._lib = L.load()
+        self.geoms = list(geoms)
+        arr = (C.c_void_p * len(self.geoms))(*[g.h for g in self.geoms])
+        e = C.c_void_p()
+        L.check(self._lib.rsfg_ensemble_create(arr, len(self.geoms), members_per_geom,
+                                               C.byref(e)), "rsfg_ensemble_create")
+        self.e = e
+        self.nm = members_per_geom
+        self.members = []
+        for gi, g in enumerate(self.geoms):
+            for k in range(members_per_geom):
+                hh = C.c_void_p()
+                L.check(self._lib.rsfg_ensemble_member(self.e, gi * members_per_geom + k,
+                                                       C.byref(hh)), "rsfg_ensemble_member")
+                m = FaultSystem.__new__(FaultSystem)
+                m._lib, m.n, m._patches = self._lib, g.n, g._patches
+                m.h, m._owned, m._members = hh, False, []
+                m.ncomp, m.G, m.precisions, m.assembly_ms = g.ncomp, g.G, g.precisions, None
+                m.dim = getattr(g, "dim", 2 * g.ncomp + 2)
+                self.members.append(m)
+
+    def solve(self, t_end, **kw):
+        kw.setdefault("prec", L.PREC_FP32)
+        o = FaultSystem.solver_options(**kw)
+        st = (L.Stats * len(self.members))()
+        L.check(self._lib.rsfg_ensemble_solve(self.e, C.byref(o), t_end, st),
+                "rsfg_ensemble_solve")
+        return [{k: getattr(s, k) for k, _ in L.Stats._fields_} for s in st]
+
+    def tune(self, **kw):
+        """Times every tile shape on this ensemble's stage and keeps the
+        fastest: returns (choice, per-shape ms, shapes)."""
+        o = FaultSystem.solver_options(**kw)
+        ch = C.c_int()
+        ncfg = C.c_int()
+        self._lib.rsfg_ensemble_config(self.e, None, C.byref(ncfg), None)
+        ms = (C.c_double * ncfg.value)()
+        L.check(self._lib.rsfg_ensemble_tune(self.e, C.byref(o), C.byref(ch), ms),
+                "rsfg_ensemble_tune")
+        return ch.value, list(ms), self.shapes()
+
+    def shapes(self):
+        ncfg, cur = C.c_int(), C.c_int()
+        self._lib.rsfg_ensemble_config(self.e, C.byref(cur), C.byref(ncfg), None)
+        sh = (C.c_int * (4 * ncfg.value))()
+        self._lib.rsfg_ensemble_config(self.e, None, None, sh)
+        return [tuple(sh[4 * i:4 * i + 4]) for i in range(ncfg.value)]
+
+    def config(self):
+        cur = C.c_int()
+        self._lib.rsfg_ensemble_config(self.e, C.byref(cur), None, None)
+        return cur.value
+
+    def close(self):
+        for m in self.members:
+            m.h = None
+        if self.e:
+            self._lib.rsfg_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def read_table(fname, n, ncols):
