@@ -19,7 +19,14 @@ slip rate) and in K v; the roofline of the headline fused stage kernel
 assembly throughput; and the CPU oracle timed on the host cores on a
 bounded sample.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--config c2|c3]
+--config c4 runs the batched ensemble instead: four planar outlines (40 x 20,
+60 x 24, 80 x 32 and 100 x 40 km at 1 km patches, 1600 to 8000 DOF) with 32
+seeded friction draws each, 100 yr per member, all 128 members in one
+ensemble (K [v_1 .. v_32] per geometry on the matrix cores, FP32 operator),
+against the same 128 members run one after another through the single-fault
+solver.
+
+    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--config c2|c3|c4]
 """
 import argparse
 import json
@@ -57,6 +64,7 @@ DTYPES = {"fp64": "f64 K.v + f64 state",
 # patches), counted on the oracle by oracle/opcount.cpp
 F_PAIR = 6205
 FP64_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
 METRIC = "sim-years/hour + K·v HBM GB/s, mixed-prec vs FP64 GPU at matched event times"
 
 
@@ -151,11 +159,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--config", choices=["c2", "c3"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4"], default="c2",
                     help="c2: 1 km patches, 8000 DOF, 400 yr (default); c3: 0.5 km patches, "
-                         "32000 DOF, 200 yr")
+                         "32000 DOF, 200 yr; c4: ensemble of 4 geometries x 32 friction "
+                         "draws, 100 yr each")
     ap.add_argument("--years", type=float, default=None)
     ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--singles", type=int, default=None,
+                    help="c4: members also run one after another through the single-fault "
+                         "solver (default all 128)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default=None,
@@ -166,16 +178,19 @@ def main():
                          "per precision and a 256 MiB device copy of known size, for "
                          "hardware-counter runs; no cycle, no JSON bench line")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "runs on one GPU only: the row-block sharding over "
+                          "several GPUs is not built", "n_gpus": args.gpus}))
+        sys.exit(2)
+    if args.config == "c4":
+        print(json.dumps(ensemble_bench(args)))
+        return
     args.ds = {"c2": 1.0, "c3": 0.5}[args.config]
     if args.years is None:
         args.years = {"c2": 400.0, "c3": 200.0}[args.config]
     if args.precisions is None:
         args.precisions = ("fp64,bf16nf,fp32,bf16x2,bf16" if args.config == "c2"
                            else "fp64,bf16nf,fp32")
-    if args.gpus != 1:
-        print(json.dumps({"error": "runs on one GPU only: the row-block sharding over "
-                          "several GPUs is not built", "n_gpus": args.gpus}))
-        sys.exit(2)
 
     from interact_amd import faults, lib
     from interact_amd.model import FaultSystem, initial_state
@@ -295,6 +310,162 @@ def main():
     if args.config == "c3":
         out["assembly_rough_triangles"] = rough_triangle_assembly(args.ds)
     print(json.dumps(out))
+
+
+ENS_OUTLINES = [(40, 20), (60, 24), (80, 32), (100, 40)]   # km
+ENS_MEMBERS = 32
+
+
+def ensemble_bench(args):
+    """c4: one ensemble of 4 geometries x 32 members against the same
+    members run one at a time through the single-fault FP32 solver."""
+    from interact_amd import faults, lib
+    from interact_amd.model import Ensemble, FaultSystem, initial_state
+    import oracle
+    years = 100.0 if args.years is None else args.years
+    t_end = years * SEC_PER_YEAR
+    nm = ENS_MEMBERS
+    geoms, gdat = [], []
+    for (L, W) in ENS_OUTLINES:
+        g = faults.planar_grid(1.0, L, W)
+        fs = FaultSystem(g)
+        fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9, store=lib.PREC_FP32)
+        geoms.append(fs)
+        gdat.append((g, L, W))
+    npat = [len(g["l"]) for g, _, _ in gdat]
+    log(f"ensemble: geometries {npat} patches, {nm} members each")
+    ens = Ensemble(geoms, nm)
+    members = []
+    for gi, (g, L, W) in enumerate(gdat):
+        for k in range(nm):
+            f = faults.ensemble_fields(g, L, W, k, gi)
+            X0 = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
+            members.append((gi, f, X0))
+
+    def configure(m, f, X0):
+        m.set_friction(f["a"], f["b"], f["dc"], v0=f["v0"], f0=f["f0"], eta=f["eta"])
+        m.set_loading(f["vpl"])
+        m.set_state(X0)
+
+    for m, (gi, f, X0) in zip(ens.members, members):
+        configure(m, f, X0)
+    t0 = time.perf_counter()
+    choice, ms, shapes = ens.tune(rtol=args.rtol)
+    tune_s = time.perf_counter() - t0
+    log("tuner: " + ", ".join(f"{shapes[i]} {ms[i]*1e3:.1f} us" for i in range(len(ms))
+                              if ms[i] > 0) + f" -> {shapes[choice]} ({tune_s:.2f} s)")
+    for _ in range(args.warmup):
+        for m, (gi, f, X0) in zip(ens.members, members):
+            m.set_state(X0)
+        ens.solve(t_end, rtol=args.rtol, steps_per_launch=64)
+    walls = []
+    for _ in range(args.steps):
+        for m, (gi, f, X0) in zip(ens.members, members):
+            m.set_state(X0)
+        t0 = time.perf_counter()
+        st = ens.solve(t_end, rtol=args.rtol, steps_per_launch=64)
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.mean(walls))
+    M = len(members)
+    ens_rate = M * years / (wall / 3600.0)
+    steps = [s["steps"] + s["rejects"] for s in st]
+    log(f"ensemble: {M} members x {years:g} yr in {wall:.2f} s ({ens_rate:.3e} sim-yr/h); "
+        f"attempts per member min {min(steps)} max {max(steps)}")
+    # the same members one after another through the single-fault solver
+    nsing = M if args.singles is None else min(args.singles, M)
+    single_s, worst_on, n_on = 0.0, 0.0, 0
+    for i in range(nsing):
+        gi, f, X0 = members[i]
+        s = geoms[gi].member()
+        configure(s, f, X0)
+        t0 = time.perf_counter()
+        s.solve(t_end, rtol=args.rtol, prec=lib.PREC_FP32, steps_per_launch=64)
+        single_s += time.perf_counter() - t0
+        e1, e2 = s.events(), ens.members[i].events()
+        o1 = e1[e1[:, 1] > 0, 0] / SEC_PER_YEAR if len(e1) else np.zeros(0)
+        o2 = e2[e2[:, 1] > 0, 0] / SEC_PER_YEAR if len(e2) else np.zeros(0)
+        k = min(len(o1), len(o2))
+        if k:
+            worst_on = max(worst_on, float(np.abs(o1[:k] - o2[:k]).max()))
+        n_on += int(len(o1) != len(o2))
+        s.close()
+        if (i + 1) % nm == 0:
+            log(f"singles: {i + 1} of {nsing} done, {single_s:.1f} s")
+    seq_rate = nsing * years / (single_s / 3600.0)
+    log(f"singles: {nsing} runs one after another in {single_s:.2f} s ({seq_rate:.3e} sim-yr/h); "
+        f"onsets vs ensemble: worst {worst_on:.2e} yr, {n_on} members with another count")
+    # roofline of the batched stage kernel at the tuned shape (HIP events
+    # around repeated launches of a middle stage, on the ensemble's stream)
+    launch_s = ms[choice] * 1e-3
+    flops = sum(2.0 * (2 * n) * (2 * n) * nm for n in npat)
+    k_bytes = sum((2 * n) * (2 * n) * 4.0 for n in npat)
+    roof = dict(kernel=f"k_stage_b (K [v_1..v_{nm}] on v_mfma_f32_16x16x4_f32 + per-member "
+                       f"stage epilogue), shape {shapes[choice]}",
+                bound="mfma", achieved=flops / launch_s / 1e12, peak=FP32_MFMA_PEAK_TFLOPS,
+                unit="TFLOP/s", frac=flops / launch_s / 1e12 / FP32_MFMA_PEAK_TFLOPS,
+                traffic=None, flops_per_launch=flops, launch_us=launch_s * 1e6,
+                operator_bytes_per_launch=k_bytes,
+                hbm_frac_operator=k_bytes / launch_s / 1e9 / HBM_PEAK_GBS)
+    out = {
+        "metric": "ensemble sim-years/hour vs 128 sequential single runs",
+        "value": ens_rate,
+        "unit": "sim-years/hour",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 K (f32 MFMA, f64 accumulation every 256 columns) + f64 state",
+        "data": "synthetic (seeded friction draws)",
+        "config": {"workload": f"c4: {len(npat)} planar strike-slip outlines "
+                               f"{[f'{L}x{W} km' for L, W in ENS_OUTLINES]} at 1 km patches "
+                               f"({[2 * n for n in npat]} DOF) x {nm} friction draws, "
+                               f"{years:g} yr each, rk5dp rtol {args.rtol:g}",
+                   "parallelism": "1 GPU"},
+        "sequential_single_runs": dict(runs=nsing, wall_s=single_s, sim_years_per_hour=seq_rate),
+        "speedup_vs_sequential": ens_rate / seq_rate,
+        "parity_vs_single": dict(worst_onset_diff_yr=worst_on, members_with_other_count=n_on),
+        "tuner": dict(choice=list(shapes[choice]), ms={str(tuple(shapes[i])): ms[i]
+                                                        for i in range(len(ms))}, wall_s=tune_s),
+        "attempts_per_member": dict(min=min(steps), max=max(steps)),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        # FP64 oracle right-hand sides on each geometry, scaled by every
+        # member's RHS count from the ensemble run
+        threads = len(os.sched_getaffinity(0))
+        t_rhs = []
+        for gi, (g, L, W) in enumerate(gdat):
+            K = geoms[gi].dense()
+            f = faults.ensemble_fields(g, L, W, 0, gi)
+            xpl = np.zeros(K.shape[0])
+            xpl[0::2] = -f["vpl"]
+            P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ xpl, ncomp=2, eta=f["eta"],
+                                  v0=f["v0"], f0=f["f0"], nthreads=threads)
+            X = members[gi * nm][2]
+            P.rhs(X)
+            nev, t0 = 0, time.perf_counter()
+            while time.perf_counter() - t0 < args.cpu_budget / len(gdat):
+                P.rhs(X)
+                nev += 1
+            t_rhs.append((time.perf_counter() - t0) / nev)
+            del K, P
+        cpu_s = sum(st[i]["rhs_evals"] * t_rhs[members[i][0]] for i in range(M))
+        log("cpu oracle: " + ", ".join(f"{t*1e3:.2f} ms/RHS" for t in t_rhs) +
+            f" -> {cpu_s:.0f} s for the ensemble's RHS evaluations")
+        out["cpu_baseline"] = dict(
+            value=M * years / (cpu_s / 3600.0), unit="sim-years/hour", cores=threads,
+            kind="port",
+            sample=(f"FP64 right-hand sides (dense K v + RHS, OpenMP over rows, {threads} "
+                    f"threads) timed for {args.cpu_budget / len(gdat):.1f} s on each geometry "
+                    f"({', '.join(f'{t*1e3:.2f}' for t in t_rhs)} ms each), scaled by every "
+                    f"member's RHS count in the ensemble run"))
+    ens.close()
+    for g in geoms:
+        g.close()
+    return out
 
 
 def rough_triangle_assembly(ds_km):

@@ -624,13 +624,13 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
              "rsfg_ensemble_solve: the batched product uses the FP32 operator (prec FP32)");
   RSFG_HIP(hipSetDevice(e->device));
   const auto wall0 = std::chrono::steady_clock::now();
+  int rc = fill_member_args(e, o);
+  if (rc) return rc;
   for (int m = 0; m < e->M; ++m) {
     rsfg_ctx* h = e->members[m];
     RSFG_CHECK((size_t)e->geo[m / e->nm].ntiles * kNRed <= h->partial_count, RSFG_ESTATE,
                "rsfg_ensemble_solve: partial buffer too small");
   }
-  int rc = fill_member_args(e, o);
-  if (rc) return rc;
   rc = ens_start(e, o, t_end);
   if (rc) return rc;
   Tableau tb;
