@@ -312,6 +312,49 @@ int rsfg_events(rsfg_handle *h, double *rows, int *n);
    max sigma, min sigma, log10 mean|v| */
 int rsfg_monitor(rsfg_handle *h, double *rows, int *n);
 
+/* ---- sensitivities: discrete adjoint of the explicit RK integration ----
+   No counterpart in the reference, which has no adjoint (its nearest code
+   is the analytic per-cell Jacobian of the IMEX path, src/rsf_imex.c:165-237).
+   The differentiated map is a replay of rsf_ODE_RHSFunction
+   (src/rsf_engine.c:186-302) under the rk5dp / rk3bs tableau over a fixed
+   list of step sizes, normally the accepted steps of an rsfg_solve
+   (rsfg_set_step_log); the objective is the misfit of log slip rates at
+   receiver patches,
+     J = 1/2 sum_n weight[n] sum_r (ln|v_{recv[r]}(X_n)| - target[n][r])^2,
+   over the states X_0 (the handle's current state) .. X_nsteps.  Operators
+   with normal-stress rows are not supported. */
+
+/* keep the accepted step sizes of subsequent solves (capacity steps; 0
+   turns the log off) */
+int rsfg_set_step_log(rsfg_handle *h, long long capacity);
+/* the accepted step sizes of the last solve: *nsteps = their number, the
+   first min(*nsteps, capacity) are copied to steps (may be NULL) */
+int rsfg_get_step_log(rsfg_handle *h, double *steps, long long capacity, long long *nsteps);
+
+typedef struct {
+  int method;            /* RSFG_RK5DP or RSFG_RK3BS */
+  int prec;              /* operator of K v and K^T w: RSFG_PREC_FP64 or RSFG_PREC_FP32 */
+  int checkpoint_every;  /* steps between stored states of the adjoint (<= 0: 256) */
+  int nrecv;             /* receiver patches */
+  const int *recv;       /* [nrecv] patch indices */
+  const double *target;  /* [(nsteps + 1) x nrecv] target ln|v|, or NULL (replay only) */
+  const double *weight;  /* [nsteps + 1] weight of each state, or NULL (replay only) */
+} rsfg_adjoint_problem;
+
+/* replays nsteps steps from the handle's current state (which is left
+   unchanged).  J (if not NULL) the objective; trace (if not NULL)
+   [(nsteps + 1) x nrecv] ln|v| at the receivers; X_end (if not NULL) the
+   final state in rsfg_set_state's layout */
+int rsfg_replay(rsfg_handle *h, const double *steps, long long nsteps,
+                const rsfg_adjoint_problem *pb, double *J, double *trace, double *X_end);
+/* J and its gradient with respect to the per-patch friction coefficients
+   a, b and D_c ([n] each, any may be NULL): forward replay with
+   checkpoints every checkpoint_every steps, then the backward sweep over
+   the steps with each segment recomputed from its checkpoint */
+int rsfg_adjoint_gradient(rsfg_handle *h, const double *steps, long long nsteps,
+                          const rsfg_adjoint_problem *pb, double *J, double *grad_a,
+                          double *grad_b, double *grad_dc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,7 +8,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["assemble.hip", "solver.hip", "ensemble.hip", "api.hip"]
+SOURCES = ["assemble.hip", "solver.hip", "ensemble.hip", "adjoint.hip", "api.hip"]
 LIB = os.path.join(HERE, "librsfg.so")
 ARCH = os.environ.get("RSFG_OFFLOAD_ARCH", "gfx950")
 

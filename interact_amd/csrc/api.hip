@@ -96,7 +96,7 @@ const void* operator_ptr(rsfg_ctx* h, int prec) {
   }
 }
 
-static size_t vin_elems_for(int cols) {
+size_t vin_elems_for(int cols) {
   return ((size_t)cols + 2 * kChunkMax + kChunkMax - 1) / kChunkMax * kChunkMax;
 }
 
@@ -672,6 +672,7 @@ int rsfg_destroy(rsfg_handle* h) {
   hipFree(h->d_sinc);
   hipFree(h->d_ctl);
   hipFree(h->d_tmp);
+  hipFree(h->d_hlog);
   if (h->h_ctl) hipHostFree(h->h_ctl);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
@@ -1027,6 +1028,34 @@ int rsfg_get_state(rsfg_handle* h, double* t, double* X) {
   for (int p = 0; p < n; ++p)
     for (int f = 0; f < D; ++f) X[(size_t)p * D + f] = soa[(size_t)f * n + p];
   if (t) *t = h->t;
+  return RSFG_OK;
+}
+
+int rsfg_set_step_log(rsfg_handle* h, long long capacity) {
+  RSFG_CHECK(h && capacity >= 0, RSFG_EINVAL, "rsfg_set_step_log: bad arguments");
+  RSFG_HIP(hipSetDevice(h->device));
+  if (h->stream) RSFG_HIP(hipStreamSynchronize(h->stream));
+  hipFree(h->d_hlog);
+  h->d_hlog = nullptr;
+  h->hlog_cap = 0;
+  if (capacity > 0) {
+    RSFG_HIP(hipMalloc(&h->d_hlog, sizeof(double) * capacity));
+    h->hlog_cap = capacity;
+  }
+  return RSFG_OK;
+}
+
+int rsfg_get_step_log(rsfg_handle* h, double* steps, long long capacity, long long* nsteps) {
+  RSFG_CHECK(h && nsteps && capacity >= 0, RSFG_EINVAL, "rsfg_get_step_log: bad arguments");
+  RSFG_CHECK(h->d_hlog != nullptr, RSFG_ESTATE, "rsfg_get_step_log: no step log (rsfg_set_step_log)");
+  RSFG_HIP(hipSetDevice(h->device));
+  const long long done = h->h_ctl ? h->h_ctl->steps : 0;
+  RSFG_CHECK(done <= h->hlog_cap, RSFG_ESTATE,
+             "rsfg_get_step_log: the last solve took more steps than the log holds");
+  *nsteps = done;
+  const long long k = std::min(done, capacity);
+  if (steps && k > 0)
+    RSFG_HIP(hipMemcpy(steps, h->d_hlog, sizeof(double) * k, hipMemcpyDeviceToHost));
   return RSFG_OK;
 }
 

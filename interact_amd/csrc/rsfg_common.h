@@ -70,6 +70,8 @@ struct DevCtl {
   int imex_g1;          // additive pair: the explicit part at X is stale
   int nevent, nmon;
   int max_event, max_mon;
+  double* hlog;         // accepted step sizes, in order (optional)
+  long long hlog_cap;
 };
 
 // Butcher tableau and integrator settings (kernel argument, by value)
@@ -147,6 +149,9 @@ struct rsfg_ctx {
   rsfg::DevCtl* d_ctl = nullptr;
   rsfg::DevCtl* h_ctl = nullptr;  // pinned
   int nevent = 0, nmon = 0;
+  // accepted step sizes of the last solve (rsfg_set_step_log)
+  double* d_hlog = nullptr;
+  long long hlog_cap = 0;
   // scratch for API calls
   double* d_tmp = nullptr;
   size_t tmp_bytes = 0;

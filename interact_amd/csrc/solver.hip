@@ -405,7 +405,7 @@ __global__ __launch_bounds__(kCtrlThreads) void k_ctrl(StageArgs a, int nb) {
 
 // ------------------------------------------------------------ host side
 
-static void fill_tableau(Tableau& tb, int method) {
+void fill_tableau(Tableau& tb, int method) {
   memset(&tb, 0, sizeof(tb));
   if (method == RSFG_ARKIMEX_L2) {
     // PETSc 3.19.6 TSARKIMEXL2 (src/ts/impls/arkimex/arkimex.c; the
@@ -449,8 +449,6 @@ static void fill_tableau(Tableau& tb, int method) {
     tb.bh[4] = -92097.0 / 339200.0; tb.bh[5] = 187.0 / 2100.0; tb.bh[6] = 1.0 / 40.0;
   }
 }
-
-const void* operator_ptr(rsfg_ctx* h, int prec);
 
 struct Launcher {
   virtual ~Launcher() {}
@@ -571,6 +569,8 @@ static int start_solve(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, Sta
   c->max_event = kMaxEv;
   c->max_mon = kMaxMon;
   c->xcur = h->xcur;
+  c->hlog = h->d_hlog;
+  c->hlog_cap = h->hlog_cap;
   RSFG_HIP(hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, h->stream));
   // the GEMV reads the input vector up to a whole chunk past the last
   // column: its padding must be zero in this precision's element type

@@ -396,6 +396,7 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
   __syncthreads();
   // ---- advance
   if (tid == 0) {
+    if (c->hlog && c->steps < c->hlog_cap) c->hlog[c->steps] = c->h;
     c->t += c->h;
     c->h = s_par[0];
     c->xcur = xo ^ 1;
@@ -548,5 +549,13 @@ __device__ __forceinline__ void ctrl_block(const StageArgs& a, int s, int nb) {
 
 // host: the argument block of a solve of handle h with operator K
 void fill_stage_args(rsfg_ctx* h, const rsfg_solver_opts* o, const void* K, StageArgs& a);
+// Butcher tableau of an integration method (RSFG_RK5DP, RSFG_RK3BS,
+// RSFG_ARKIMEX_L2)
+void fill_tableau(Tableau& tb, int method);
+// stored operator of precision prec (nullptr if not assembled)
+const void* operator_ptr(rsfg_ctx* h, int prec);
+// elements of a product input vector for cols columns, zero-padded past the
+// last column far enough for every product kernel's chunked reads
+size_t vin_elems_for(int cols);
 
 }  // namespace rsfg

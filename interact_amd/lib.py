@@ -37,7 +37,8 @@ EXPORTS = [
     "rsfg_read_table", "rsfg_write_checkpoint", "rsfg_read_checkpoint",
     "rsfg_create_member", "rsfg_ensemble_create", "rsfg_ensemble_member",
     "rsfg_ensemble_solve", "rsfg_ensemble_tune", "rsfg_ensemble_config",
-    "rsfg_ensemble_destroy",
+    "rsfg_ensemble_destroy", "rsfg_set_step_log", "rsfg_get_step_log", "rsfg_replay",
+    "rsfg_adjoint_gradient",
 ]
 
 
@@ -72,6 +73,12 @@ class SolverOpts(C.Structure):
                 ("rupture_vth", C.c_double), ("max_steps", C.c_long),
                 ("max_domain_reject", C.c_int),
                 ("steps_per_launch", C.c_int)]
+
+
+class AdjointProblem(C.Structure):
+    _fields_ = [("method", C.c_int), ("prec", C.c_int), ("checkpoint_every", C.c_int),
+                ("nrecv", C.c_int), ("recv", C.c_void_p), ("target", C.c_void_p),
+                ("weight", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -120,6 +127,12 @@ def load(path=None):
     L.rsfg_ensemble_config.argtypes = [h, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]
     L.rsfg_ensemble_destroy.argtypes = [h]
+    L.rsfg_set_step_log.argtypes = [h, C.c_longlong]
+    L.rsfg_get_step_log.argtypes = [h, C.c_void_p, C.c_longlong, C.POINTER(C.c_longlong)]
+    L.rsfg_replay.argtypes = [h, _dp, C.c_longlong, C.POINTER(AdjointProblem), C.c_void_p,
+                              C.c_void_p, C.c_void_p]
+    L.rsfg_adjoint_gradient.argtypes = [h, _dp, C.c_longlong, C.POINTER(AdjointProblem),
+                                        C.POINTER(C.c_double), _dp, _dp, _dp]
     L.rsfg_read_patch_file.argtypes = [C.c_char_p, C.POINTER(Patch),
                                        C.POINTER(C.c_int)]
     L.rsfg_assemble.argtypes = [h, C.POINTER(Assembly), C.POINTER(C.c_double)]

@@ -203,6 +203,65 @@ class FaultSystem:
                                                C.byref(dt)), "rsfg_read_checkpoint")
         return dict(slip_mode=sm.value, law=law.value, step=step.value, t=t.value, dt=dt.value)
 
+    # ---- sensitivities (discrete adjoint of the replayed integration)
+    def set_step_log(self, capacity):
+        """Keep the accepted step sizes of the following solves (0: off)."""
+        L.check(self._lib.rsfg_set_step_log(self.h, int(capacity)), "rsfg_set_step_log")
+
+    def step_log(self):
+        """Accepted step sizes of the last solve [s]."""
+        k = C.c_longlong()
+        L.check(self._lib.rsfg_get_step_log(self.h, None, 0, C.byref(k)), "rsfg_get_step_log")
+        out = np.zeros(k.value)
+        if k.value:
+            L.check(self._lib.rsfg_get_step_log(self.h, out.ctypes.data, k.value, C.byref(k)),
+                    "rsfg_get_step_log")
+        return out
+
+    def _adjoint_problem(self, nsteps, recv, target, weight, method, prec, checkpoint_every):
+        keep = []
+        recv = np.ascontiguousarray(recv, np.int32)
+        keep.append(recv)
+        pb = L.AdjointProblem(method, prec, checkpoint_every, len(recv),
+                              recv.ctypes.data if len(recv) else None, None, None)
+        if target is not None:
+            t = np.ascontiguousarray(target, np.float64).reshape(nsteps + 1, len(recv))
+            w = np.ascontiguousarray(weight, np.float64).reshape(nsteps + 1)
+            keep += [t, w]
+            pb.target, pb.weight = t.ctypes.data, w.ctypes.data
+        return pb, keep
+
+    def replay(self, steps, recv=(), target=None, weight=None, *, method=L.RK5DP,
+               prec=L.PREC_FP64):
+        """Replays the steps from the current state (left unchanged):
+        dict(J (None without a target), trace [(N+1) x nrecv] ln|v| at the
+        receivers, X_end)."""
+        steps = np.ascontiguousarray(steps, np.float64)
+        N = len(steps)
+        pb, keep = self._adjoint_problem(N, recv, target, weight, method, prec, 0)
+        trace = np.zeros((N + 1) * max(pb.nrecv, 1))
+        Xe = np.zeros(self.n * self.dim)
+        J = C.c_double()
+        L.check(self._lib.rsfg_replay(self.h, steps, N, C.byref(pb),
+                                      C.byref(J) if target is not None else None,
+                                      trace.ctypes.data, Xe.ctypes.data), "rsfg_replay")
+        return dict(J=J.value if target is not None else None,
+                    trace=trace.reshape(N + 1, -1)[:, :pb.nrecv].copy(), X_end=Xe)
+
+    def adjoint_gradient(self, steps, recv, target, weight, *, method=L.RK5DP,
+                         prec=L.PREC_FP64, checkpoint_every=256):
+        """J and dJ/da, dJ/db, dJ/dD_c per patch of the replay misfit
+        (rsfg_adjoint_gradient)."""
+        steps = np.ascontiguousarray(steps, np.float64)
+        N = len(steps)
+        pb, keep = self._adjoint_problem(N, recv, target, weight, method, prec,
+                                         checkpoint_every)
+        ga, gb, gd = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        J = C.c_double()
+        L.check(self._lib.rsfg_adjoint_gradient(self.h, steps, N, C.byref(pb), C.byref(J),
+                                                ga, gb, gd), "rsfg_adjoint_gradient")
+        return dict(J=J.value, grad_a=ga, grad_b=gb, grad_dc=gd)
+
     def get_state(self):
         X = np.zeros(self.n * self.dim)
         t = C.c_double()

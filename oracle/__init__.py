@@ -74,7 +74,8 @@ class Result(C.Structure):
                 ("domain_rejects", C.c_long), ("rhs_evals", C.c_long),
                 ("t_final", C.c_double), ("dt_final", C.c_double),
                 ("status", C.c_int), ("nevent", C.c_int),
-                ("ev", C.c_void_p), ("nmon", C.c_int), ("mon", C.c_void_p)]
+                ("ev", C.c_void_p), ("nmon", C.c_int), ("mon", C.c_void_p),
+                ("hlog", C.c_void_p), ("hlog_cap", C.c_long)]
 
 
 def _setup(L):
@@ -107,6 +108,17 @@ def _setup(L):
                            C.c_void_p]
     L.rsfo_domain_ok.restype = C.c_int
     L.rsfo_domain_ok.argtypes = [C.POINTER(Problem), _dp]
+    L.rsfo_patch_rhs_c.restype = None
+    L.rsfo_patch_rhs_c.argtypes = [C.POINTER(Problem), C.c_int, _dp, _dp, _dp, _dp]
+    L.rsfo_patch_jacobian.restype = None
+    L.rsfo_patch_jacobian.argtypes = [C.POINTER(Problem), C.c_int, _dp, _dp] + [_dp] * 6 + \
+        [C.POINTER(C.c_double)]
+    L.rsfo_replay.restype = None
+    L.rsfo_replay.argtypes = [C.POINTER(Problem), C.c_int, _dp, _dp, C.c_long, _ip, C.c_int,
+                              C.c_void_p, C.c_void_p]
+    L.rsfo_adjoint.restype = C.c_double
+    L.rsfo_adjoint.argtypes = [C.POINTER(Problem), C.c_int, _dp, _dp, C.c_long, _ip, C.c_int,
+                               _dp, _dp, _dp, _dp, _dp]
     L.rsfo_solve.restype = C.c_int
     L.rsfo_solve.argtypes = [C.POINTER(Problem), C.POINTER(Options), _dp,
                              C.POINTER(Result)]
@@ -317,7 +329,8 @@ class RSFProblem:
               dt_init=1.0, dt_max=1e10, dt_monitor=5 * SEC_PER_YEAR,
               rdx_monitor=1e-4, adx_monitor=0.0, vel_event=1e-3,
               vel_event_hyst=0.5, rupture_vth=None, max_steps=10_000_000,
-              max_domain_reject=5000, max_events=4096, max_monitor=200000):
+              max_domain_reject=5000, max_events=4096, max_monitor=200000,
+              log_steps=0):
         X = np.array(X0, np.float64, copy=True)
         at = (C.c_double * 6)(*(list(atol) if atol is not None else
                                 [1e-12, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3][:self.dim]
@@ -331,12 +344,60 @@ class RSFProblem:
         r = Result()
         r.ev = ev.ctypes.data
         r.mon = mon.ctypes.data
+        hlog = np.zeros(max(int(log_steps), 1))
+        if log_steps:
+            r.hlog = hlog.ctypes.data
+            r.hlog_cap = int(log_steps)
         lib().rsfo_solve(C.byref(self.p), C.byref(o), X, C.byref(r))
-        return dict(X=X, steps=r.steps, rejects=r.rejects,
-                    domain_rejects=r.domain_rejects, rhs_evals=r.rhs_evals,
-                    t_final=r.t_final, dt_final=r.dt_final, status=r.status,
-                    events=ev[:8 * r.nevent].reshape(-1, 8).copy(),
-                    monitor=mon[:10 * r.nmon].reshape(-1, 10).copy())
+        out = dict(X=X, steps=r.steps, rejects=r.rejects,
+                   domain_rejects=r.domain_rejects, rhs_evals=r.rhs_evals,
+                   t_final=r.t_final, dt_final=r.dt_final, status=r.status,
+                   events=ev[:8 * r.nevent].reshape(-1, 8).copy(),
+                   monitor=mon[:10 * r.nmon].reshape(-1, 10).copy())
+        if log_steps:
+            out["hlog"] = hlog[:min(r.steps, int(log_steps))].copy()
+        return out
+
+    # ---- discrete adjoint of the replayed integration (rsfo_adj.c)
+    def patch_rhs_c(self, i, x, td):
+        """f, v of patch i by the complex restatement on the real axis."""
+        f, v = np.zeros(self.dim), np.zeros(self.ncomp)
+        lib().rsfo_patch_rhs_c(C.byref(self.p), i, np.ascontiguousarray(x, np.float64),
+                               np.ascontiguousarray(td, np.float64), f, v)
+        return f, v
+
+    def patch_jacobian(self, i, x, td):
+        """Complex-step Jacobians of patch i: dict Jx, Jtd, Jp (a, b, dc),
+        Vx, Vp, Lx, La."""
+        d, nc = self.dim, self.ncomp
+        Jx, Jtd, Jp = np.zeros(d * d), np.zeros(d * nc), np.zeros(d * 3)
+        Vx, Vp, Lx, La = np.zeros(nc * d), np.zeros(nc), np.zeros(d), C.c_double()
+        lib().rsfo_patch_jacobian(C.byref(self.p), i, np.ascontiguousarray(x, np.float64),
+                                  np.ascontiguousarray(td, np.float64), Jx, Jtd, Jp, Vx, Vp,
+                                  Lx, C.byref(La))
+        return dict(Jx=Jx.reshape(d, d), Jtd=Jtd.reshape(d, nc), Jp=Jp.reshape(d, 3),
+                    Vx=Vx.reshape(nc, d), Vp=Vp, Lx=Lx, La=La.value)
+
+    def replay(self, X0, steps, recv=(), method=0):
+        """Replay the steps from X0: (trace [(N+1) x nrecv] of ln|v|, X_end)."""
+        steps = np.ascontiguousarray(steps, np.float64)
+        recv = np.ascontiguousarray(recv, np.int32)
+        trace = np.zeros((len(steps) + 1) * max(len(recv), 1))
+        Xe = np.zeros(len(X0))
+        lib().rsfo_replay(C.byref(self.p), method, np.ascontiguousarray(X0, np.float64), steps,
+                          len(steps), recv, len(recv), trace.ctypes.data, Xe.ctypes.data)
+        return trace.reshape(len(steps) + 1, -1)[:, :len(recv)].copy(), Xe
+
+    def adjoint(self, X0, steps, recv, target, weight, method=0):
+        """J and dJ/da, dJ/db, dJ/ddc of the replay misfit."""
+        steps = np.ascontiguousarray(steps, np.float64)
+        recv = np.ascontiguousarray(recv, np.int32)
+        ga, gb, gd = np.zeros(self.n), np.zeros(self.n), np.zeros(self.n)
+        J = lib().rsfo_adjoint(C.byref(self.p), method, np.ascontiguousarray(X0, np.float64),
+                               steps, len(steps), recv, len(recv),
+                               np.ascontiguousarray(target, np.float64).ravel(),
+                               np.ascontiguousarray(weight, np.float64), ga, gb, gd)
+        return J, ga, gb, gd
 
 
 def vel(tau, sigma, psi, a, v0):

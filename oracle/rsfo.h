@@ -153,6 +153,8 @@ typedef struct {
                                  ruptured cell count */
   int nmon;
   double *mon;                /* max_monitor x 10 monitor columns */
+  double *hlog;               /* accepted step sizes in order, or NULL */
+  long hlog_cap;
 } rsfo_result;
 
 /* integrates X in place; X is AoS as for rsfo_rhs */
@@ -165,6 +167,30 @@ int rsfo_rk_attempt(const rsfo_problem *p, int method, double rtol,
                     const double *atol, double h, const double *X,
                     const double *F0, double *Xnew, double *Fnew,
                     double *errnorm);
+
+/* ---- discrete adjoint of the replayed integration, rsfo_adj.c ----
+   (no reference counterpart; the map is rsfo_rk_attempt stepping over a
+   fixed step list, the objective
+   J = 1/2 sum_n weight[n] sum_r (ln|v_recv[r](X_n)| - target[n][r])^2) */
+
+/* right-hand side f[dim] and slip rate v[nc] of patch i from its state x and
+   stressing rates td[nc] (complex restatement evaluated on the real axis) */
+void rsfo_patch_rhs_c(const rsfo_problem *p, int i, const double *x, const double *td,
+                      double *f, double *v);
+/* complex-step Jacobians of patch i (row-major, any output may be NULL):
+   Jx[dim][dim] df/dx, Jtd[dim][nc] df/dtd, Jp[dim][3] df/d(a, b, dc),
+   Vx[nc][dim] dv/dx, Vp[nc] dv/da, Lx[dim] dln|v|/dx, *La dln|v|/da */
+void rsfo_patch_jacobian(const rsfo_problem *p, int i, const double *x, const double *td,
+                         double *Jx, double *Jtd, double *Jp, double *Vx, double *Vp,
+                         double *Lx, double *La);
+/* replay nsteps steps (method 0 rk5dp, 1 rk3bs) from X0 (AoS); trace
+   [(nsteps+1) x nrecv] ln|v| at the receivers and Xend may be NULL */
+void rsfo_replay(const rsfo_problem *p, int method, const double *X0, const double *steps,
+                 long nsteps, const int *recv, int nrecv, double *trace, double *Xend);
+/* J (returned) and dJ/da, dJ/db, dJ/ddc per patch; stores every stage */
+double rsfo_adjoint(const rsfo_problem *p, int method, const double *X0, const double *steps,
+                    long nsteps, const int *recv, int nrecv, const double *target,
+                    const double *weight, double *ga, double *gb, double *gdc);
 
 #ifdef __cplusplus
 }

@@ -739,6 +739,7 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
             }
           }
         }
+        if (r->hlog && r->steps < r->hlog_cap) r->hlog[r->steps] = h;
         t += h;
         h = hnew;
         memcpy(X, Xn, sizeof(double) * m);

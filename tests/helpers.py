@@ -96,3 +96,58 @@ def first_recurrence(events, t_min_yr=1.0):
     on = events[events[:, 1] > 0, 0] / SEC_PER_YEAR
     on = on[on > t_min_yr]
     return float(on[0]) if len(on) else float("nan")
+
+
+class AdjointCase:
+    """The small sensitivity problem shared by the oracle and device adjoint
+    tests: the 40 x 20 km outline at 2 km (200 patches), friction draw 2 of
+    the ensemble generator, started just after its first event (13 yr, from
+    an oracle solve) and run over an interseismic window; 16 seeded
+    receivers; the target is the ln|v| trace of a run with perturbed b and
+    D_c (its own adaptive steps), interpolated in time onto the states of
+    the model's steps; weights are the step lengths over the window."""
+
+    def __init__(self, ncomp=2, window_yr=40.0, nthreads=8, seed=5):
+        import oracle
+        from interact_amd import faults
+        g = faults.planar_grid(2.0, 40, 20)
+        T = oracle.traction_tensor(g, 0, nthreads)
+        self.ncomp = ncomp
+        self.K = vector_operator(T, 32e9 / 1e4) if ncomp == 2 else strike_operator(T, 32e9 / 1e4)
+        self.g = g
+        self.n = n = len(g["l"])
+        self.f = f = faults.ensemble_fields(g, 40, 20, 2, 0)
+        X0 = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=ncomp)
+        P = self.problem(f["a"], f["b"], f["dc"], nthreads)
+        r0 = P.solve(X0, 13.0 * SEC_PER_YEAR, rtol=1e-6)
+        self.X_start, self.t_start = r0["X"], r0["t_final"]
+        ro = P.solve(self.X_start, self.t_start + window_yr * SEC_PER_YEAR, t0=self.t_start,
+                     rtol=1e-6, log_steps=100000)
+        self.steps, self.X_solve = ro["hlog"], ro["X"]
+        rng = np.random.default_rng(seed)
+        self.recv = np.sort(rng.choice(n, 16, replace=False)).astype(np.int32)
+        bt = f["b"] * (1 + 0.02 * rng.standard_normal(n))
+        dt = f["dc"] * (1 + 0.05 * rng.standard_normal(n))
+        Pt = self.problem(f["a"], bt, dt, nthreads)
+        rt = Pt.solve(self.X_start, self.t_start + 1.01 * window_yr * SEC_PER_YEAR,
+                      t0=self.t_start, rtol=1e-6, log_steps=100000)
+        trt, _ = Pt.replay(self.X_start, rt["hlog"], self.recv, 0)
+        tt = np.concatenate([[0.0], np.cumsum(rt["hlog"])])
+        tm = np.concatenate([[0.0], np.cumsum(self.steps)])
+        self.target = np.stack([np.interp(tm, tt, trt[:, r]) for r in range(len(self.recv))], 1)
+        self.weight = np.zeros(len(self.steps) + 1)
+        self.weight[1:] = self.steps / self.steps.sum()
+
+    def problem(self, a, b, dc, nthreads=8):
+        import oracle
+        f, n, nc = self.f, self.n, self.ncomp
+        xpl = np.zeros(nc * n)
+        xpl[0::nc] = -f["vpl"]
+        return oracle.RSFProblem(self.K, a, b, dc, self.K @ xpl, ncomp=nc, eta=f["eta"],
+                                 v0=f["v0"], f0=f["f0"], nthreads=nthreads)
+
+    def J_oracle(self, a, b, dc, steps=None, method=0):
+        tr, _ = self.problem(a, b, dc).replay(self.X_start,
+                                              self.steps if steps is None else steps,
+                                              self.recv, method)
+        return 0.5 * np.sum(self.weight[:, None] * (tr - self.target) ** 2)
