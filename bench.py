@@ -19,14 +19,20 @@ slip rate) and in K v; the roofline of the headline fused stage kernel
 assembly throughput; and the CPU oracle timed on the host cores on a
 bounded sample.
 
---config c4 runs the batched ensemble instead: four planar outlines (40 x 20,
+--config c5 runs the sensitivity workload: on the 1 km fault (8000 DOF), the
+gradient of the log slip-rate misfit at 64 seeded receivers over a 50 yr
+interseismic window (after the first event) with respect to a - b and D_c of
+every patch (discrete adjoint of the accepted rk5dp steps, FP64 operator,
+checkpoints every 256 steps), checked against central differences on 32
+random directions and the oracle's FP64 reference adjoint on the window's
+first steps.  --config c4 runs the batched ensemble instead: four planar outlines (40 x 20,
 60 x 24, 80 x 32 and 100 x 40 km at 1 km patches, 1600 to 8000 DOF) with 32
 seeded friction draws each, 100 yr per member, all 128 members in one
 ensemble (K [v_1 .. v_32] per geometry on the matrix cores, FP32 operator),
 against the same 128 members run one after another through the single-fault
 solver.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--config c2|c3|c4]
+    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--config c2|c3|c4|c5]
 """
 import argparse
 import json
@@ -159,12 +165,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--config", choices=["c2", "c3", "c4"], default="c2",
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
                     help="c2: 1 km patches, 8000 DOF, 400 yr (default); c3: 0.5 km patches, "
                          "32000 DOF, 200 yr; c4: ensemble of 4 geometries x 32 friction "
-                         "draws, 100 yr each")
+                         "draws, 100 yr each; c5: adjoint gradient on the 1 km fault")
     ap.add_argument("--years", type=float, default=None)
     ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--fd-directions", type=int, default=32,
+                    help="c5: random directions of the central-difference check")
     ap.add_argument("--singles", type=int, default=None,
                     help="c4: members also run one after another through the single-fault "
                          "solver (default all 128)")
@@ -184,6 +192,9 @@ def main():
         sys.exit(2)
     if args.config == "c4":
         print(json.dumps(ensemble_bench(args)))
+        return
+    if args.config == "c5":
+        print(json.dumps(adjoint_bench(args)))
         return
     args.ds = {"c2": 1.0, "c3": 0.5}[args.config]
     if args.years is None:
@@ -466,6 +477,166 @@ def ensemble_bench(args):
     for g in geoms:
         g.close()
     return out
+
+
+def adjoint_bench(args):
+    """c5: gradient of the receiver misfit on the 1 km fault."""
+    from interact_amd import faults, lib
+    from interact_amd.model import FaultSystem, initial_state
+    import oracle
+    window = 50.0
+    rtol = args.rtol
+    g = faults.planar_grid(1.0)
+    f = faults.cycle_fields(g)
+    n = len(g["l"])
+    D = 2 * n
+    fs = FaultSystem(g)
+    fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"], store=lib.PREC_FP32)
+    fric = dict(v0=f["v0"], f0=f["f0"], eta=f["eta"])
+    fs.set_friction(f["a"], f["b"], f["dc"], **fric)
+    fs.set_loading(f["vpl"])
+    X0 = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
+    fs.set_state(X0)
+    # the window starts after the first event (its arrest)
+    fs.solve(20.0 * SEC_PER_YEAR, rtol=rtol, prec=lib.PREC_FP64, steps_per_launch=64)
+    t_first = float("nan")
+    for _ in range(20):
+        fs.solve(fs.get_state()[1] + 2.0 * SEC_PER_YEAR, rtol=rtol, prec=lib.PREC_FP64,
+                 steps_per_launch=64)
+        ev = fs.events()
+        arr = ev[ev[:, 1] < 0, 0] if len(ev) else []
+        if len(arr):
+            t_first = first_onset(ev)
+            break
+    fs.solve(fs.get_state()[1] + 1.0 * SEC_PER_YEAR, rtol=rtol, prec=lib.PREC_FP64,
+             steps_per_launch=64)
+    Xs, ts = fs.get_state()
+    log(f"adjoint: first event {t_first:.3f} yr, window starts {ts / SEC_PER_YEAR:.3f} yr")
+    fs.set_step_log(1_000_000)
+    fs.solve(ts + window * SEC_PER_YEAR, rtol=rtol, prec=lib.PREC_FP64, steps_per_launch=64)
+    steps = fs.step_log()
+    ev = fs.events()
+    n_on = int(np.sum(ev[:, 1] > 0)) if len(ev) else 0
+    N = len(steps)
+    rng = np.random.default_rng(20261202)
+    recv = np.sort(rng.choice(n, 64, replace=False)).astype(np.int32)
+    # target: the same window with perturbed b and D_c, on its own steps
+    bt = f["b"] * (1 + 0.02 * rng.standard_normal(n))
+    dt = f["dc"] * (1 + 0.05 * rng.standard_normal(n))
+    fs.set_friction(f["a"], bt, dt, **fric)
+    fs.set_state(Xs, ts)
+    fs.solve(ts + 1.01 * window * SEC_PER_YEAR, rtol=rtol, prec=lib.PREC_FP64, steps_per_launch=64)
+    st_t = fs.step_log()
+    fs.set_state(Xs, ts)
+    tr_t = fs.replay(st_t, recv)["trace"]
+    tt = np.concatenate([[0.0], np.cumsum(st_t)])
+    tm = np.concatenate([[0.0], np.cumsum(steps)])
+    target = np.stack([np.interp(tm, tt, tr_t[:, r]) for r in range(len(recv))], 1)
+    weight = np.zeros(N + 1)
+    weight[1:] = steps / steps.sum()
+    fs.set_friction(f["a"], f["b"], f["dc"], **fric)
+    fs.set_state(Xs, ts)
+    log(f"adjoint: {N} steps over {window:g} yr, {n_on} onsets in the window, 64 receivers")
+
+    def grad():
+        return fs.adjoint_gradient(steps, recv, target, weight, prec=lib.PREC_FP64,
+                                   checkpoint_every=256)
+    for _ in range(args.warmup):
+        grad()
+    walls = []
+    for _ in range(max(args.steps, 1)):
+        t0 = time.perf_counter()
+        r = grad()
+        walls.append(time.perf_counter() - t0)
+    wall = float(np.mean(walls))
+    t0 = time.perf_counter()
+    J_rep = fs.replay(steps, recv, target, weight)["J"]
+    replay_s = time.perf_counter() - t0
+    g_amb, g_dc = -r["grad_b"], r["grad_dc"]
+    log(f"adjoint: J {r['J']:.6e} (replay {J_rep:.6e}); gradient in {wall:.2f} s, "
+        f"replay alone {replay_s:.2f} s; |dJ/d(a-b)| {np.linalg.norm(g_amb):.3e} "
+        f"|dJ/dDc| {np.linalg.norm(g_dc):.3e}")
+    # central differences on 32 random directions of (a - b at fixed a, D_c)
+    amb0 = f["a"] - f["b"]
+    eps = 1e-5
+    rels = []
+    nd = max(0, args.fd_directions)
+    for k in range(nd):
+        d1 = rng.standard_normal(n) * np.abs(amb0)
+        d2 = rng.standard_normal(n) * f["dc"]
+        Js = []
+        for sgn in (1.0, -1.0):
+            fs.set_friction(f["a"], f["a"] - (amb0 + sgn * eps * d1), f["dc"] + sgn * eps * d2,
+                            **fric)
+            fs.set_state(Xs, ts)
+            Js.append(fs.replay(steps, recv, target, weight)["J"])
+        fdv = (Js[0] - Js[1]) / (2 * eps)
+        ad = g_amb @ d1 + g_dc @ d2
+        rels.append(abs(fdv - ad) / abs(ad))
+        if k % 8 == 7 or k == nd - 1:
+            log(f"adjoint: {k + 1} of {nd} directions, worst relative difference {max(rels):.2e}")
+    fs.set_friction(f["a"], f["b"], f["dc"], **fric)
+    fs.set_state(Xs, ts)
+    # the oracle's FP64 reference adjoint on the window's first steps
+    Np = min(16, N)
+    K64 = fs.dense()
+    xpl = np.zeros(D)
+    xpl[0::2] = -f["vpl"]
+    threads = len(os.sched_getaffinity(0))
+    P = oracle.RSFProblem(K64, f["a"], f["b"], f["dc"], K64 @ xpl, ncomp=2, eta=f["eta"],
+                          v0=f["v0"], f0=f["f0"], nthreads=threads)
+    tp, wp = target[:Np + 1], weight[:Np + 1]
+    t0 = time.perf_counter()
+    Jo, gao, gbo, gdo = P.adjoint(Xs, steps[:Np], recv, tp, wp, 0)
+    cpu_s = time.perf_counter() - t0
+    rp = fs.adjoint_gradient(steps[:Np], recv, tp, wp, prec=lib.PREC_FP64)
+    ref_err = dict(J=abs(rp["J"] - Jo) / abs(Jo),
+                   grad_a=float(np.linalg.norm(rp["grad_a"] - gao) / np.linalg.norm(gao)),
+                   grad_b=float(np.linalg.norm(rp["grad_b"] - gbo) / np.linalg.norm(gbo)),
+                   grad_dc=float(np.linalg.norm(rp["grad_dc"] - gdo) / np.linalg.norm(gdo)))
+    log(f"adjoint: oracle FP64 reference adjoint on {Np} steps in {cpu_s:.1f} s on {threads} "
+        f"threads; relative differences {ref_err}")
+    del K64, P
+    # roofline of the backward sweep's K^T w (HIP events on the handle's stream)
+    kt_ms = fs.adjoint_bench(lib.PREC_FP64, 20)
+    kt_bytes = float(D) * D * 8 + 8.0 * D * 2
+    fs.close()
+    rate = window / (wall / 3600.0)
+    cpu_rate = window / (cpu_s / Np * N / 3600.0)
+    return {
+        "metric": "adjoint gradient: simulated years/hour of replay + backward sweep",
+        "value": rate,
+        "unit": "sim-years/hour",
+        "n_gpus": 1,
+        "steps": max(args.steps, 1),
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 K v, K^T w and state",
+        "data": "synthetic (seeded fault, friction, receivers and target)",
+        "config": {"workload": f"c5: planar fault 100x40 km, 1 km patches, {D} DOF; "
+                               f"ln|v| misfit at 64 receivers over a {window:g} yr window "
+                               f"after the first event ({N} rk5dp steps at rtol {rtol:g}); "
+                               f"gradient in a - b and D_c of {n} patches",
+                   "parallelism": "1 GPU"},
+        "adjoint": dict(J=r["J"], steps=N, onsets_in_window=n_on, gradient_s=wall,
+                        replay_s=replay_s, fd_directions=nd, fd_eps=eps,
+                        fd_worst_rel=float(max(rels)) if rels else None,
+                        fd_median_rel=float(np.median(rels)) if rels else None,
+                        oracle_steps=Np, oracle_rel=ref_err, first_event_yr=t_first,
+                        window_start_yr=ts / SEC_PER_YEAR),
+        "roofline": dict(kernel="k_adj_kt<double> (K^T w of the backward sweep, row-split "
+                                "partials)", bound="hbm", achieved=kt_bytes / kt_ms / 1e6,
+                         peak=HBM_PEAK_GBS, unit="GB/s", frac=kt_bytes / kt_ms / 1e6 / HBM_PEAK_GBS,
+                         traffic=None, bytes_per_launch=kt_bytes, launch_us=kt_ms * 1e3),
+        "cpu_baseline": dict(value=cpu_rate, unit="sim-years/hour", cores=threads, kind="port",
+                             sample=(f"the oracle's FP64 reference adjoint (replay + backward, "
+                                     f"OpenMP products on {threads} threads) on the window's "
+                                     f"first {Np} steps in {cpu_s:.1f} s, scaled to the "
+                                     f"{N} steps of the window")),
+    }
 
 
 def rough_triangle_assembly(ds_km):

@@ -354,6 +354,10 @@ int rsfg_replay(rsfg_handle *h, const double *steps, long long nsteps,
 int rsfg_adjoint_gradient(rsfg_handle *h, const double *steps, long long nsteps,
                           const rsfg_adjoint_problem *pb, double *J, double *grad_a,
                           double *grad_b, double *grad_dc);
+/* times nrep launches of the backward sweep's transposed product K^T w
+   (row-split partial sums, FP64 or FP32 operator): *ms is the mean kernel
+   duration by HIP events on the handle's stream */
+int rsfg_adjoint_bench(rsfg_handle *h, int prec, int nrep, double *ms);
 
 #ifdef __cplusplus
 }

@@ -869,4 +869,38 @@ int rsfg_adjoint_gradient(rsfg_handle* h, const double* steps, long long nsteps,
   return RSFG_OK;
 }
 
+int rsfg_adjoint_bench(rsfg_handle* h, int prec, int nrep, double* ms) {
+  RSFG_CHECK(h && ms && nrep > 0, RSFG_EINVAL, "rsfg_adjoint_bench: bad arguments");
+  RSFG_CHECK(h->assembled && h->G == h->ncomp, RSFG_ESTATE,
+             "rsfg_adjoint_bench: needs an assembled operator without normal rows");
+  RSFG_CHECK(prec == RSFG_PREC_FP64 || prec == RSFG_PREC_FP32, RSFG_EINVAL,
+             "rsfg_adjoint_bench: prec must be FP64 or FP32");
+  RSFG_CHECK(operator_ptr(h, prec) != nullptr, RSFG_ESTATE,
+             "rsfg_adjoint_bench: operator precision not assembled");
+  RSFG_HIP(hipSetDevice(h->device));
+  rsfg_adjoint_problem pb;
+  memset(&pb, 0, sizeof pb);
+  pb.method = RSFG_RK5DP;
+  pb.prec = prec;
+  const double one = 1.0;
+  AdjSetup s;
+  adj_setup(h, &one, 1, &pb, s);
+  AdjWork W;
+  RSFG_HIP(hipMalloc(&W.base, sizeof(double) * (h->cols + (size_t)s.nsplit * h->cols)));
+  W.w = W.base;
+  W.zpart = W.base + h->cols;
+  RSFG_HIP(hipMemsetAsync(W.w, 0, sizeof(double) * h->cols, h->stream));
+  const bool f64 = prec == RSFG_PREC_FP64;
+  f64 ? launch_kt<double>(s, W) : launch_kt<float>(s, W);
+  RSFG_HIP(hipEventRecord(h->ev0, h->stream));
+  for (int r = 0; r < nrep; ++r) f64 ? launch_kt<double>(s, W) : launch_kt<float>(s, W);
+  RSFG_HIP(hipEventRecord(h->ev1, h->stream));
+  RSFG_HIP(hipEventSynchronize(h->ev1));
+  RSFG_HIP(hipGetLastError());
+  float t = 0.f;
+  RSFG_HIP(hipEventElapsedTime(&t, h->ev0, h->ev1));
+  *ms = t / nrep;
+  return RSFG_OK;
+}
+
 }  // extern "C"

@@ -38,7 +38,7 @@ EXPORTS = [
     "rsfg_create_member", "rsfg_ensemble_create", "rsfg_ensemble_member",
     "rsfg_ensemble_solve", "rsfg_ensemble_tune", "rsfg_ensemble_config",
     "rsfg_ensemble_destroy", "rsfg_set_step_log", "rsfg_get_step_log", "rsfg_replay",
-    "rsfg_adjoint_gradient",
+    "rsfg_adjoint_gradient", "rsfg_adjoint_bench",
 ]
 
 
@@ -133,6 +133,7 @@ def load(path=None):
                               C.c_void_p, C.c_void_p]
     L.rsfg_adjoint_gradient.argtypes = [h, _dp, C.c_longlong, C.POINTER(AdjointProblem),
                                         C.POINTER(C.c_double), _dp, _dp, _dp]
+    L.rsfg_adjoint_bench.argtypes = [h, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.rsfg_read_patch_file.argtypes = [C.c_char_p, C.POINTER(Patch),
                                        C.POINTER(C.c_int)]
     L.rsfg_assemble.argtypes = [h, C.POINTER(Assembly), C.POINTER(C.c_double)]

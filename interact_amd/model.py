@@ -262,6 +262,13 @@ class FaultSystem:
                                                 ga, gb, gd), "rsfg_adjoint_gradient")
         return dict(J=J.value, grad_a=ga, grad_b=gb, grad_dc=gd)
 
+    def adjoint_bench(self, prec=L.PREC_FP64, nrep=20):
+        """mean duration [ms] of the backward sweep's K^T w product"""
+        ms = C.c_double()
+        L.check(self._lib.rsfg_adjoint_bench(self.h, prec, nrep, C.byref(ms)),
+                "rsfg_adjoint_bench")
+        return ms.value
+
     def get_state(self):
         X = np.zeros(self.n * self.dim)
         t = C.c_double()
