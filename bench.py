@@ -184,16 +184,24 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="one assembly, a few launches of the stage and matvec kernels "
                          "per precision and a 256 MiB device copy of known size, for "
-                         "hardware-counter runs; no cycle, no JSON bench line")
+                         "hardware-counter runs; no cycle, no JSON bench line (c4: the "
+                         "tuner's launches of every tile shape; c5: one gradient over 64 "
+                         "fixed steps and 20 K^T w launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "runs on one GPU only: the row-block sharding over "
                           "several GPUs is not built", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.config == "c4":
+        if args.kernels_only:
+            ensemble_kernels_only(args)
+            return
         print(json.dumps(ensemble_bench(args)))
         return
     if args.config == "c5":
+        if args.kernels_only:
+            adjoint_kernels_only(args)
+            return
         print(json.dumps(adjoint_bench(args)))
         return
     args.ds = {"c2": 1.0, "c3": 0.5}[args.config]
@@ -477,6 +485,59 @@ def ensemble_bench(args):
     for g in geoms:
         g.close()
     return out
+
+
+def ensemble_kernels_only(args):
+    """c4 kernels for counter and trace runs: the ensemble's setup (no
+    solve) and the tuner, which launches the batched stage kernel 21 times
+    per tile shape."""
+    from interact_amd import faults, lib
+    from interact_amd.model import Ensemble, FaultSystem, initial_state
+    geoms, gdat = [], []
+    for (L, W) in ENS_OUTLINES:
+        g = faults.planar_grid(1.0, L, W)
+        fs = FaultSystem(g)
+        fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9, store=lib.PREC_FP32)
+        geoms.append(fs)
+        gdat.append((g, L, W))
+    ens = Ensemble(geoms, ENS_MEMBERS)
+    for gi, (g, L, W) in enumerate(gdat):
+        for k in range(ENS_MEMBERS):
+            f = faults.ensemble_fields(g, L, W, k, gi)
+            m = ens.members[gi * ENS_MEMBERS + k]
+            m.set_friction(f["a"], f["b"], f["dc"], v0=f["v0"], f0=f["f0"], eta=f["eta"])
+            m.set_loading(f["vpl"])
+            m.set_state(initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2))
+    choice, ms, shapes = ens.tune(rtol=args.rtol)
+    log("kernels-only c4: " + ", ".join(f"{shapes[i]} {ms[i]*1e3:.1f} us" for i in range(len(ms))
+                                         if ms[i] > 0))
+    ens.close()
+    for g in geoms:
+        g.close()
+
+
+def adjoint_kernels_only(args):
+    """c5 kernels for counter and trace runs: one gradient over 64 steps of
+    0.05 yr from the initial (interseismic) state of the 1 km fault and 20
+    launches of the backward sweep's K^T w."""
+    from interact_amd import faults, lib
+    from interact_amd.model import FaultSystem, initial_state
+    g = faults.planar_grid(1.0)
+    f = faults.cycle_fields(g)
+    n = len(g["l"])
+    fs = FaultSystem(g)
+    fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"], store=lib.PREC_FP32)
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=f["v0"], f0=f["f0"], eta=f["eta"])
+    fs.set_loading(f["vpl"])
+    fs.set_state(initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2))
+    steps = np.full(64, 0.05 * SEC_PER_YEAR)
+    recv = np.sort(np.random.default_rng(20261202).choice(n, 64, replace=False)).astype(np.int32)
+    tr = fs.replay(steps, recv)["trace"]
+    weight = np.full(65, 1.0 / 65)
+    r = fs.adjoint_gradient(steps, recv, tr + 0.01, weight, prec=lib.PREC_FP64)
+    ms = fs.adjoint_bench(lib.PREC_FP64, 20)
+    log(f"kernels-only c5: J {r['J']:.6e}, K^T w {ms * 1e3:.1f} us")
+    fs.close()
 
 
 def adjoint_bench(args):
