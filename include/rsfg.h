@@ -289,9 +289,9 @@ int rsfg_ensemble_solve(rsfg_ensemble *e, const rsfg_solver_opts *o, double t_en
    ensemble (members must be configured), keeps the fastest; ms (may be
    NULL) receives one time per candidate, -1 where a shape does not fit */
 int rsfg_ensemble_tune(rsfg_ensemble *e, const rsfg_solver_opts *o, int *choice, double *ms);
-/* current tile shape, number of candidates, and per candidate four ints
-   (rows per wave, row groups per workgroup, column ranges per row group,
-   chunks in flight per wave) */
+/* current unit shape, number of candidates, and per candidate four ints
+   (operator rows per wave, waves per product unit, 16-column chunks in
+   flight per wave, columns per product unit) */
 int rsfg_ensemble_config(rsfg_ensemble *e, int *cfg, int *ncfg, int *shape);
 int rsfg_ensemble_destroy(rsfg_ensemble *e);
 

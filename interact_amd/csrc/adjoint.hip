@@ -287,117 +287,140 @@ __global__ __launch_bounds__(kAdjThreads) void k_adj_begin(AdjArgs A, StageCoef 
   for (int c = 0; c < NC; ++c) A.w[NC * p + c] = w[c];
 }
 
-// stage i of a step, given z_i = K^T w_i in row-split partials
+// stage i of a step, given z_i = K^T w_i in row-split partials.  Eight
+// consecutive lanes per patch, one input direction each: lanes 0 .. NC + 1
+// the state components psi, the shear stresses and sigma (slip does not
+// enter the law), lanes NC + 2 .. NC + 4 the parameters a, b, D_c.  The
+// row splits of z are summed over the eight lanes and reduced in a fixed
+// order, and the next stage's adjoint vector is exchanged by shuffles.
+constexpr int kAdjLanes = 8;
+
 template <int NC>
 __global__ __launch_bounds__(kAdjThreads) void k_adj_stage(AdjArgs A, StageCoef ha, int i,
                                                          const double* recY,
-                            const double* recT, const double* prevY, const double* prevT,
-                            long state) {
+                                                         const double* recT, const double* prevY,
+                                                         const double* prevT, long state) {
   using adj::Dual;
-  constexpr int D = 2 * NC + 2;
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= A.n) return;
+  constexpr int D = 2 * NC + 2, NQ = NC + 2;
+  static_assert(NQ + 3 <= kAdjLanes, "directions per patch");
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = gid / kAdjLanes, q = gid % kAdjLanes;
+  const int base = (threadIdx.x & 63) - q;   // first lane of this patch's group
   const int n = A.n;
+  const bool valid = p < n;
+  const int pc = valid ? p : n - 1;
+  // z: the row splits of K^T w, divided over the lanes, summed in order
   double z[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     double s = 0.0;
-    for (int k = 0; k < A.nsplit; ++k) s += A.zpart[(size_t)k * A.cols + NC * p + c];
-    z[c] = s;
+    for (int k = q; k < A.nsplit; k += kAdjLanes) s += A.zpart[(size_t)k * A.cols + NC * pc + c];
+#pragma unroll
+    for (int off = kAdjLanes / 2; off > 0; off >>= 1) s += __shfl_down(s, off, kAdjLanes);
+    z[c] = __shfl(s, base, 64);
   }
-  double y[D], td[NC], mu[D];
   const double* Mi = nullptr;
 #pragma unroll
   for (int j = 0; j < kMaxStages - 1; ++j)
     if (j == i) Mi = A.M[j];
+  double y[D], td[NC], mu[D];
 #pragma unroll
   for (int f = 0; f < D; ++f) {
-    y[f] = recY[(size_t)f * n + p];
-    mu[f] = Mi[(size_t)f * n + p];
+    y[f] = recY[(size_t)f * n + pc];
+    mu[f] = Mi[(size_t)f * n + pc];
   }
 #pragma unroll
-  for (int c = 0; c < NC; ++c) td[c] = recT[(size_t)c * n + p];
-  const double a0 = A.coef[p], b0 = A.coef[n + p], d0 = A.coef[2 * n + p];
-  const double* sn = A.sinc + NC * p;
-  const double snn = A.sinc[NC * n + p];
-  Dual t[NC];
-#pragma unroll
-  for (int c = 0; c < NC; ++c) t[c] = Dual{td[c], 0.0};
-  // state directions: psi, the shear stresses and sigma (slip does not
-  // enter the right-hand side)
-  double nu[D];
-#pragma unroll
-  for (int f = 0; f < D; ++f) nu[f] = 0.0;
-#pragma unroll
-  for (int q = 0; q < NC + 2; ++q) {
-    Dual s[D], out[D], v[2];
+  for (int c = 0; c < NC; ++c) td[c] = recT[(size_t)c * n + pc];
+  const double a0 = A.coef[pc], b0 = A.coef[n + pc], d0 = A.coef[2 * n + pc];
+  const double* sn = A.sinc + NC * pc;
+  const double snn = A.sinc[NC * n + pc];
+  // this lane's direction
+  double g = 0.0;
+  if (q < NQ + 3) {
+    Dual s[D], t[NC], out[D], v[2];
 #pragma unroll
     for (int f = 0; f < D; ++f) s[f] = Dual{y[f], (f == q) ? 1.0 : 0.0};
-    const Dual ca{a0, 0.0}, cb{b0, 0.0}, cd{d0, 0.0};
+#pragma unroll
+    for (int c = 0; c < NC; ++c) t[c] = Dual{td[c], 0.0};
+    const Dual ca{a0, q == NQ ? 1.0 : 0.0}, cb{b0, q == NQ + 1 ? 1.0 : 0.0},
+        cd{d0, q == NQ + 2 ? 1.0 : 0.0};
     adj::rhs<Dual, NC>(A.fp, ca, cb, cd, s, t, sn, snn, out);
-    adj::vel<Dual, NC>(A.fp, ca, s, v);
-    double acc = 0.0;
 #pragma unroll
-    for (int f = 0; f < D; ++f) acc += out[f].d * mu[f];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc += v[c].d * z[c];
-    nu[q] = acc;
-  }
-  // parameter directions a, b, dc
-  double gp[3];
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    Dual s[D], out[D], v[2];
-#pragma unroll
-    for (int f = 0; f < D; ++f) s[f] = Dual{y[f], 0.0};
-    const Dual ca{a0, q == 0 ? 1.0 : 0.0}, cb{b0, q == 1 ? 1.0 : 0.0}, cd{d0, q == 2 ? 1.0 : 0.0};
-    adj::rhs<Dual, NC>(A.fp, ca, cb, cd, s, t, sn, snn, out);
-    double acc = 0.0;
-#pragma unroll
-    for (int f = 0; f < D; ++f) acc += out[f].d * mu[f];
-    if (q == 0) {
+    for (int f = 0; f < D; ++f) g += out[f].d * mu[f];
+    if (q <= NQ) {   // the slip rates depend on the state and on a
       adj::vel<Dual, NC>(A.fp, ca, s, v);
 #pragma unroll
-      for (int c = 0; c < NC; ++c) acc += v[c].d * z[c];
-    }
-    gp[q] = acc;
-  }
-  // (the misfit of state X_n adds its own d/da at stage 0 below)
-  // earlier stages: M_j += h a_ij nu_i (ha.c[j] = h a_ij)
-#pragma unroll
-  for (int j = 0; j < kMaxStages - 1; ++j) {
-    if (j < i && ha.c[j] != 0.0) {
-#pragma unroll
-      for (int f = 0; f < D; ++f) A.M[j][(size_t)f * n + p] += ha.c[j] * nu[f];
+      for (int c = 0; c < NC; ++c) g += v[c].d * z[c];
     }
   }
-  double lam[D];
+  // lanes q < NQ: nu_q; earlier stages M_j += h a_ij nu_i; lambda += nu_i
+  double upd_prev = 0.0;
+  if (q < NQ && valid) {
 #pragma unroll
-  for (int f = 0; f < D; ++f) lam[f] = A.lam[(size_t)f * n + p] + nu[f];
+    for (int j = 0; j < kMaxStages - 1; ++j) {
+      if (j < i && ha.c[j] != 0.0) A.M[j][(size_t)q * n + p] += ha.c[j] * g;
+    }
+  }
+  if (q < NQ) {
+    // the updated component q of M_{i-1} (what the lane just stored)
+#pragma unroll
+    for (int j = 0; j < kMaxStages - 1; ++j)
+      if (i > 0 && j == i - 1) upd_prev = A.M[j][(size_t)q * n + pc];
+  }
+  double lam_q = 0.0;
+  if (q < NQ) lam_q = A.lam[(size_t)q * n + pc] + g;
+  double ga_add = (q == NQ) ? g : 0.0;
   if (i > 0) {
-    double y1[D], t1[NC], mu1[D], w[NC];
+    // w of stage i - 1: mu' components from the lanes that own them (slip
+    // components are unchanged by this stage and are loaded directly)
+    double y1[D], t1[NC], mu1[D];
     const double* Mp = nullptr;
 #pragma unroll
     for (int j = 0; j < kMaxStages - 1; ++j)
       if (j == i - 1) Mp = A.M[j];
 #pragma unroll
     for (int f = 0; f < D; ++f) {
-      y1[f] = prevY[(size_t)f * n + p];
-      mu1[f] = Mp[(size_t)f * n + p];
+      y1[f] = prevY[(size_t)f * n + pc];
+      const double own = __shfl(upd_prev, base + (f < NQ ? f : 0), 64);
+      mu1[f] = (f < NQ) ? own : Mp[(size_t)f * n + pc];
     }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) t1[c] = prevT[(size_t)c * n + p];
-    adj_w<NC>(A, p, y1, t1, mu1, w);
+    for (int c = 0; c < NC; ++c) t1[c] = prevT[(size_t)c * n + pc];
+    if (q < NC) {
+      Dual s[D], t[NC], out[D];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) A.w[NC * p + c] = w[c];
-  } else {
-    adj_misfit<NC>(A, p, state, y, lam, &gp[0]);
+      for (int f = 0; f < D; ++f) s[f] = Dual{y1[f], 0.0};
+#pragma unroll
+      for (int c = 0; c < NC; ++c) t[c] = Dual{t1[c], (c == q) ? 1.0 : 0.0};
+      const Dual ca{a0, 0.0}, cb{b0, 0.0}, cd{d0, 0.0};
+      adj::rhs<Dual, NC>(A.fp, ca, cb, cd, s, t, sn, snn, out);
+      double w = 0.0;
+#pragma unroll
+      for (int f = 0; f < D; ++f) w += out[f].d * mu1[f];
+      if (valid) A.w[NC * p + q] = w;
+    }
+  } else if (valid && A.recv_of[p] >= 0 && A.weight[state] != 0.0 && q <= NQ) {
+    // the misfit of state X_n: d ln|v| / ds_q (lanes q < NQ) and / da (lane NQ)
+    const double wt = A.weight[state];
+    double v0v[2];
+    adj::vel<double, NC>(A.fp, a0, y, v0v);
+    const double res = log(vmag_of<NC>(v0v)) - A.target[state * A.nrecv + A.recv_of[p]];
+    Dual s[D], v[2];
+#pragma unroll
+    for (int f = 0; f < D; ++f) s[f] = Dual{y[f], (f == q) ? 1.0 : 0.0};
+    const Dual ca{a0, q == NQ ? 1.0 : 0.0};
+    adj::vel<Dual, NC>(A.fp, ca, s, v);
+    const Dual vm = (NC == 1) ? adj::fabs(v[0]) : adj::sqrt(v[0] * v[0] + v[1] * v[1]);
+    const double dl = wt * res * (vm.d / vm.v);
+    if (q < NQ) lam_q += dl;
+    else ga_add += dl;
   }
-#pragma unroll
-  for (int f = 0; f < D; ++f) A.lam[(size_t)f * n + p] = lam[f];
-  A.ga[p] += gp[0];
-  A.gb[p] += gp[1];
-  A.gdc[p] += gp[2];
+  if (valid) {
+    if (q < NQ) A.lam[(size_t)q * n + p] = lam_q;
+    if (q == NQ) A.ga[p] += ga_add;
+    if (q == NQ + 1) A.gb[p] += g;
+    if (q == NQ + 2) A.gdc[p] += g;
+  }
 }
 
 // zpart[split][col] = sum over the rows of this split of K[row][col] w[row];
@@ -761,6 +784,7 @@ static int backward_pass(AdjSetup& s, AdjWork& W, const double* steps) {
   A.target = W.target;
   A.weight = W.weight;
   const dim3 gp((n + kAdjThreads - 1) / kAdjThreads);
+  const dim3 gp8(((size_t)n * kAdjLanes + kAdjThreads - 1) / kAdjThreads);
   RSFG_HIP(hipMemsetAsync(W.g, 0, 3 * (size_t)n * sizeof(double), h->stream));
   // lambda_N from the final state (Ys[0] after the forward pass)
   hipLaunchKernelGGL(k_adj_init<NC>, gp, dim3(kAdjThreads), 0, h->stream, A, W.Ys[0], (long)s.N);
@@ -791,8 +815,8 @@ static int backward_pass(AdjSetup& s, AdjWork& W, const double* steps) {
         StageCoef ha;
         memset(&ha, 0, sizeof ha);
         for (int j = 0; j < i; ++j) ha.c[j] = hm * s.tb.A[i][j];
-        hipLaunchKernelGGL(k_adj_stage<NC>, gp, dim3(kAdjThreads), 0, h->stream, A, ha, i, recY(i),
-                           recT(i), py, pt, (long)m);
+        hipLaunchKernelGGL(k_adj_stage<NC>, gp8, dim3(kAdjThreads), 0, h->stream, A, ha, i,
+                           recY(i), recT(i), py, pt, (long)m);
       }
     }
     RSFG_HIP(hipGetLastError());

@@ -8,11 +8,16 @@
 // K [v_1 .. v_M] with M = 16, 32 or 64 right-hand sides.  That product runs
 // on the matrix cores (v_mfma_f32_16x16x4_f32: FP32 operands, FP32
 // accumulation, flushed to FP64 every 256 columns) and its epilogue is the
-// per-(patch, member) stage update of the single-fault solver.  One attempt
-// of rk5dp for the whole ensemble is k_prep_b, six k_stage_b and k_ctrl_b;
-// the host captures batches of attempts in a hipGraph.  Tile shapes (rows
-// per wave, waves per workgroup, chunks in flight) are chosen at run time
-// by timing the candidates on the ensemble's own stage (rsfg_ensemble_tune).
+// per-(patch, member) stage update of the single-fault solver.  The product
+// is split into equal units (a row group of one geometry x a block of its
+// columns, for all members), so that geometries of very different size
+// fill the chip evenly; each unit writes FP64 partial sums, and the stage
+// epilogue adds a row's partials over its column blocks in a fixed order.
+// One attempt of rk5dp for the whole ensemble is k_prep_b, six times
+// (k_prod_b, k_epi_b), and k_ctrl_b; the host captures batches of attempts
+// in a hipGraph.  Unit shapes (rows per wave, waves per unit, chunks in
+// flight, columns per block) are chosen at run time by timing the candidates
+// on the ensemble's own stage (rsfg_ensemble_tune).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,18 +33,28 @@ namespace rsfg {
 
 struct EnsGeo {        // one geometry of the ensemble
   const float* K;      // FP32 operator, rows x ld (zero-padded columns)
+  float* Kt;           // the same in MFMA tiles: rows_pad / 16 x ld / 16 tiles of
+                       // 16 x 16, tile (b, c) at Kt + (b ld / 16 + c) 256, lane
+                       // l = 16 j + i holding row 16 b + i, columns 16 c + 4 j .. + 3
   long ld;             // leading dimension of K and of V
   int rows, n;         // G n operator rows, n patches
   int m0, nm;          // first member and member count
-  int tile0, ntiles;   // row tiles of this geometry in the tile list
+  int tile0, ntiles;   // epilogue tiles (16 rows) of this geometry in the tile list
   float* V[2];         // product inputs by stage parity, nm x ld (member-major)
+  double* P;           // product partials [column block][rows_pad][nm]
+  int rows_pad;        // rows rounded up to a whole number of product row groups
 };
+
+constexpr int kEpiRows = 16;      // operator rows per epilogue tile
+constexpr int kRowPad = 256;      // rows_pad granularity (largest product row group)
 
 struct EnsArgs {
   const StageArgs* args;  // per member
   const EnsGeo* geo;      // per geometry
-  const int2* tiles;      // (geometry, first row) per workgroup, longest rows first
+  const int2* tiles;      // epilogue: (geometry, first row) per workgroup
+  const int4* units;      // product: (geometry, first row, first chunk, end chunk)
   int ngeo;
+  int cb_chunks;          // 16-column chunks per column block
   int s;                  // stage
 };
 
@@ -100,23 +115,43 @@ __global__ void k_halted_b(EnsArgs e, int M, long long* out) {
   }
 }
 
-// The rows [rw, rw + 16 RF) of one wave against all 16 MF members over the
-// 16-column chunks [kc0, kc1).  Lane (j, i) = (lane / 16, lane % 16) loads
-// the 16-byte slice K[row i][k0 + 4j .. k0 + 4j + 3] and V[member i][same
-// columns]; the four MFMAs of a chunk take component q of both, so A[i][j]
-// and B[j][i] index the same column k0 + 4j + q.  FP32 partial sums are
-// added into FP64 every 16 chunks (256 columns).  Output lane layout
-// (checked on the device): acc[rf][mf][r] = Y[row 16 rf + 4 (lane / 16) + r]
-// [member 16 mf + lane % 16].
-template <int RF, int MF, int U>
-__device__ __forceinline__ void wave_product(const float* __restrict__ K, long ld, int rw,
-                                             int rows, const float* __restrict__ V, int kc0,
-                                             int kc1, double (&acc)[RF][MF][4]) {
+// the operator in MFMA tiles (rows past the last are zero)
+__global__ void k_tile_k(const float* __restrict__ K, long ld, int rows, int nrb,
+                         float* __restrict__ Kt) {
+  const long nk = ld / 16;
+  const long t = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);  // tile
+  if (t >= (long)nrb * nk) return;
+  const int l = threadIdx.x & 63, i = l & 15, j = l >> 4;
+  const long b = t / nk, c = t % nk;
+  const long row = 16 * b + i;
+  v4f_t v = v4f_t{0.f, 0.f, 0.f, 0.f};
+  if (row < rows) v = *reinterpret_cast<const v4f_t*>(K + row * ld + 16 * c + 4 * j);
+  *reinterpret_cast<v4f_t*>(Kt + t * 256 + 4 * l) = v;
+}
+
+// The rows [rw, rw + 16 RF) of one wave (rw a multiple of 16) against all
+// 16 MF members over the 16-column chunks [kc0, kc1).  Lane (j, i) = (lane /
+// 16, lane % 16) holds row i's and member i's columns k0 + 4j .. k0 + 4j + 3:
+// one contiguous kilobyte of tiled K per (row block, chunk), and 16-byte
+// slices of the member-major product input; the four MFMAs of a chunk take
+// component q of both, so A[i][j] and B[j][i] index the same column
+// k0 + 4j + q.  FP32 partial sums are added into FP64 every 16 chunks (256
+// columns).  Output lane layout (checked on the device): acc[rf][mf][r] =
+// Y[row 16 rf + 4 (lane / 16) + r][member 16 mf + lane % 16].
+// PIPE: the loads of the next U chunks are issued before the MFMAs of the
+// current ones (two register sets), so that a wave keeps its own loads in
+// flight while its matrix instructions run; the products and the order of
+// summation are the same.
+template <int RF, int MF, int U, int PIPE>
+__device__ __forceinline__ void wave_product(const float* __restrict__ Kt, long ld, int rw,
+                                             const float* __restrict__ V, int kc0, int kc1,
+                                             double (&acc)[RF][MF][4]) {
   const int lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
+  const long nk = ld / 16;
   const float* ka[RF];
   const float* vb[MF];
 #pragma unroll
-  for (int rf = 0; rf < RF; ++rf) ka[rf] = K + (size_t)min(rw + 16 * rf + i, rows - 1) * ld + 4 * j;
+  for (int rf = 0; rf < RF; ++rf) ka[rf] = Kt + ((long)(rw / 16 + rf) * nk) * 256 + 4 * lane;
 #pragma unroll
   for (int mf = 0; mf < MF; ++mf) vb[mf] = V + (size_t)(16 * mf + i) * ld + 4 * j;
 #pragma unroll
@@ -125,42 +160,64 @@ __device__ __forceinline__ void wave_product(const float* __restrict__ K, long l
     for (int mf = 0; mf < MF; ++mf)
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[rf][mf][r] = 0.0;
-  for (int kb = kc0; kb < kc1; kb += 16) {
-    v4f_t c[RF][MF];
+  if (kc0 >= kc1) return;
+  v4f_t A[U][RF], B[U][MF];
+  auto load = [&](int kc, v4f_t (&a)[U][RF], v4f_t (&b)[U][MF]) {
 #pragma unroll
-    for (int rf = 0; rf < RF; ++rf)
+    for (int u = 0; u < U; ++u) {
 #pragma unroll
-      for (int mf = 0; mf < MF; ++mf) c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
-    const int ke = min(kb + 16, kc1);
-    for (int kc = kb; kc < ke; kc += U) {
-      v4f_t A[U][RF], B[U][MF];
+      for (int rf = 0; rf < RF; ++rf)
+        a[u][rf] = __builtin_nontemporal_load(
+            reinterpret_cast<const v4f_t*>(ka[rf] + (size_t)(kc + u) * 256));
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf)
+        b[u][mf] = *reinterpret_cast<const v4f_t*>(vb[mf] + (size_t)(kc + u) * 16);
+    }
+  };
+  v4f_t c[RF][MF];
+#pragma unroll
+  for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+    for (int mf = 0; mf < MF; ++mf) c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
+  if (PIPE) load(kc0, A, B);
+  for (int kc = kc0; kc < kc1; kc += U) {
+    const bool more = kc + U < kc1;
+    v4f_t An[U][RF], Bn[U][MF];
+    if (PIPE) {
+      if (more) load(kc + U, An, Bn);
+    } else {
+      load(kc, A, B);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+          for (int mf = 0; mf < MF; ++mf)
+            c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u][rf][q], B[u][mf][q],
+                                                              c[rf][mf], 0, 0, 0);
+    // FP64 flush every 16 chunks from the start of the range, and at its end
+    if (((kc + U - kc0) & 15) == 0 || !more) {
+#pragma unroll
+      for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+        for (int mf = 0; mf < MF; ++mf) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[rf][mf][r] += (double)c[rf][mf][r];
+          c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
+        }
+    }
+    if (PIPE && more) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
 #pragma unroll
-        for (int rf = 0; rf < RF; ++rf)
-          A[u][rf] = __builtin_nontemporal_load(
-              reinterpret_cast<const v4f_t*>(ka[rf] + (size_t)(kc + u) * 16));
+        for (int rf = 0; rf < RF; ++rf) A[u][rf] = An[u][rf];
 #pragma unroll
-        for (int mf = 0; mf < MF; ++mf)
-          B[u][mf] = *reinterpret_cast<const v4f_t*>(vb[mf] + (size_t)(kc + u) * 16);
+        for (int mf = 0; mf < MF; ++mf) B[u][mf] = Bn[u][mf];
       }
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-          for (int rf = 0; rf < RF; ++rf)
-#pragma unroll
-            for (int mf = 0; mf < MF; ++mf)
-              c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u][rf][q], B[u][mf][q],
-                                                                c[rf][mf], 0, 0, 0);
     }
-#pragma unroll
-    for (int rf = 0; rf < RF; ++rf)
-#pragma unroll
-      for (int mf = 0; mf < MF; ++mf)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[rf][mf][r] += (double)c[rf][mf][r];
   }
 }
 
@@ -256,28 +313,54 @@ __device__ __forceinline__ void member_patch_stage(const StageArgs& a, const Mem
   for (int f = 0; f < D; ++f) r9[RED_XN2] += y[f] * y[f];
 }
 
-// One workgroup = one row tile of one geometry for all of its 16 MF
-// members.  Its WR x WK waves split the tile's 16 RF WR rows into WR row
-// groups and the columns into WK contiguous ranges, so that many short
-// streams run at once; the WK partial products of a row group are summed
-// in LDS in a fixed order.  Then every (patch, member) pair of the tile
-// gets its stage update.  The tile's patch count 16 RF WR / G is a power of
-// two of at most 64, so one member's pairs are consecutive lanes of one
-// wave and its reductions are shuffles.
-template <int NC, int G, int RF, int MF, int WR, int WK, int U>
-__global__ __launch_bounds__(64 * WR * WK) void k_stage_b(EnsArgs e) {
-  constexpr int W = WR * WK;
-  constexpr int TR = 16 * RF * WR;  // rows per tile
-  constexpr int NPT = TR / G;       // patches per tile
-  constexpr int NM = 16 * MF;       // members per geometry
-  static_assert(NPT == 8 || NPT == 16 || NPT == 32 || NPT == 64, "tile patches");
-  __shared__ double Yp[WK * TR * NM];
+// true if any member of geometry gg is still running (every thread of
+// the workgroup gets the answer)
+__device__ __forceinline__ bool geometry_running(const EnsArgs& e, const EnsGeo& gg) {
+  int run = 0;
+  for (int m = threadIdx.x; m < gg.nm; m += blockDim.x) run |= !e.args[gg.m0 + m].ctl->halted;
+  return __syncthreads_or(run) != 0;
+}
+
+// One product unit per workgroup: rows [r0, r0 + 16 RF W) of one geometry
+// (16 RF per wave) over the 16-column chunks [kc0, kc1), for all 16 MF
+// members, into that column block's partials.
+template <int RF, int MF, int W, int U, int PIPE>
+__global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
+  constexpr int NM = 16 * MF;
+  const int4 u = e.units[blockIdx.x];
+  const EnsGeo& gg = e.geo[u.x];
+  if (!geometry_running(e, gg)) return;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int rw = u.y + wave * 16 * RF;
+  double acc[RF][MF][4];
+  wave_product<RF, MF, U, PIPE>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], u.z, u.w, acc);
+  double* P = gg.P + (size_t)(u.z / e.cb_chunks) * gg.rows_pad * NM;
+#pragma unroll
+  for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+    for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = rw + 16 * rf + 4 * (lane >> 4) + r;
+        P[(size_t)row * NM + 16 * mf + (lane & 15)] = acc[rf][mf][r];
+      }
+}
+
+// Stage epilogue: one workgroup per 16 operator rows (16 / G patches) of one
+// geometry, one thread per (patch, member).  The stressing rates of a row
+// are its partials summed over the column blocks in order; then the
+// single-fault stage update, and each member's reductions over the
+// consecutive lanes holding its patches.
+template <int NC, int G, int MF>
+__global__ __launch_bounds__(64 * (kEpiRows / G) * MF / 4) void k_epi_b(EnsArgs e) {
+  constexpr int NPT = kEpiRows / G;   // patches per tile
+  constexpr int NM = 16 * MF;
+  static_assert(NPT == 8 || NPT == 16, "tile patches");
   __shared__ MemberSnap snap[NM];
   const int2 tile = e.tiles[blockIdx.x];
   const EnsGeo& gg = e.geo[tile.x];
   const int r0 = tile.y, s = e.s;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int wr = wave / WK, wk = wave % WK;
+  const int tid = threadIdx.x;
   for (int m = tid; m < NM; m += blockDim.x) {
     const DevCtl* c = e.args[gg.m0 + m].ctl;
     MemberSnap q;
@@ -287,62 +370,39 @@ __global__ __launch_bounds__(64 * WR * WK) void k_stage_b(EnsArgs e) {
     q.h = c->h;
     snap[m] = q;
   }
-  {
-    // column range of this wave, in whole groups of U chunks
-    const int nk = (int)(gg.ld / 16);
-    const int per = ((nk + WK - 1) / WK + U - 1) / U * U;
-    const int kc0 = min(nk, wk * per), kc1 = min(nk, kc0 + per);
-    double acc[RF][MF][4];
-    wave_product<RF, MF, U>(gg.K, gg.ld, r0 + wr * 16 * RF, gg.rows, gg.V[s & 1], kc0, kc1, acc);
-#pragma unroll
-    for (int rf = 0; rf < RF; ++rf)
-#pragma unroll
-      for (int mf = 0; mf < MF; ++mf)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = wr * 16 * RF + 16 * rf + 4 * (lane >> 4) + r;
-          Yp[(wk * TR + row) * NM + 16 * mf + (lane & 15)] = acc[rf][mf][r];
-        }
-  }
   __syncthreads();
-  if (WK > 1) {
-    for (int i = tid; i < TR * NM; i += blockDim.x) {
-      double v = Yp[i];
-      for (int k = 1; k < WK; ++k) v += Yp[k * TR * NM + i];
-      Yp[i] = v;
-    }
-    __syncthreads();
-  }
   const int S = e.args[gg.m0].tb.s;
   const bool reduce = (s == 0) || (s == S - 1);
-  const int pt0 = r0 / G;
-  float* vnext = gg.V[(s + 1) & 1];
-  for (int q0 = 0; q0 < NPT * NM; q0 += 64 * W) {
-    const int q = q0 + tid;
-    const int m = q / NPT, pl = q % NPT, p = pt0 + pl;
-    double r9[kNRed];
-    r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
-    r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
-    r9[RED_DX2] = 0.0;
-    if (m < NM && p < gg.n && !snap[m].halted) {
-      double td[G];
+  const int ncb = (int)((gg.ld / 16 + e.cb_chunks - 1) / e.cb_chunks);
+  const size_t cbs = (size_t)gg.rows_pad * NM;
+  const int m = tid / NPT, pl = tid % NPT, p = r0 / G + pl;
+  double r9[kNRed];
+  r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
+  r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
+  r9[RED_DX2] = 0.0;
+  if (p < gg.n && !snap[m].halted) {
+    double td[G];
 #pragma unroll
-      for (int k = 0; k < G; ++k) td[k] = Yp[(pl * G + k) * NM + m];
-      member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[m], s, p, td,
-                                vnext + (size_t)m * gg.ld, r9);
+    for (int k = 0; k < G; ++k) {
+      const double* q = gg.P + (size_t)(p * G + k) * NM + m;
+      double v = 0.0;
+      for (int cb = 0; cb < ncb; ++cb) v += q[cb * cbs];
+      td[k] = v;
     }
-    if (reduce) {
-      // the NPT pairs of member m are NPT consecutive lanes of one wave
+    member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[m], s, p, td,
+                              gg.V[(s + 1) & 1] + (size_t)m * gg.ld, r9);
+  }
+  if (reduce) {
+    // the NPT pairs of member m are NPT consecutive lanes of one wave
 #pragma unroll
-      for (int k = 0; k < kNRed; ++k) {
-        double v = r9[k];
-        const int op = red_kind(k);
+    for (int k = 0; k < kNRed; ++k) {
+      double v = r9[k];
+      const int op = red_kind(k);
 #pragma unroll
-        for (int off = NPT / 2; off > 0; off >>= 1) v = red_op(op, v, __shfl_xor(v, off, 64));
-        if (pl == 0 && m < NM) {
-          const StageArgs& a = e.args[gg.m0 + m];
-          a.partials[(size_t)k * gg.ntiles + (blockIdx.x - gg.tile0)] = v;
-        }
+      for (int off = NPT / 2; off > 0; off >>= 1) v = red_op(op, v, __shfl_xor(v, off, 64));
+      if (pl == 0) {
+        const StageArgs& a = e.args[gg.m0 + m];
+        a.partials[(size_t)k * gg.ntiles + (blockIdx.x - gg.tile0)] = v;
       }
     }
   }
@@ -350,55 +410,66 @@ __global__ __launch_bounds__(64 * WR * WK) void k_stage_b(EnsArgs e) {
 
 // ------------------------------------------------------------ host side
 struct EnsCfg {
-  int rf, wr, wk, u;
+  int rf, w, u, cb, pipe;  // 16 RF rows per wave, W waves per unit, U chunks per load
+                           // group, CB columns per unit, loads issued one group ahead
 };
-// candidate tile shapes: 16 RF rows per wave, WR row groups x WK column
-// ranges of waves per workgroup, U chunks in flight per wave
-static const EnsCfg kEnsCfgs[] = {{1, 1, 4, 2}, {1, 1, 4, 4}, {1, 1, 8, 2}, {1, 2, 4, 2},
-                                  {2, 1, 4, 2}, {1, 2, 2, 4}, {1, 4, 1, 2}, {1, 1, 2, 4}};
+static const EnsCfg kEnsCfgs[] = {
+    {4, 2, 1, 1024, 1}, {4, 4, 1, 1024, 1}, {4, 2, 1, 2048, 1}, {4, 1, 1, 1024, 1},
+    {2, 4, 1, 1024, 1}, {2, 2, 1, 1024, 1}, {4, 2, 2, 1024, 1}, {8, 1, 1, 1024, 1},
+    {8, 2, 1, 1024, 1}, {4, 2, 1, 512, 1}};
 constexpr int kEnsNCfg = sizeof(kEnsCfgs) / sizeof(kEnsCfgs[0]);
 
-static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].wr; }
+static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].w; }
 
-typedef void (*EnsStageFn)(const EnsArgs&, int nblocks, hipStream_t);
+typedef void (*EnsProdFn)(const EnsArgs&, int nblocks, hipStream_t);
+typedef void (*EnsEpiFn)(const EnsArgs&, int nblocks, hipStream_t);
 
-template <int NC, int G, int RF, int MF, int WR, int WK, int U>
-static void launch_stage_b(const EnsArgs& e, int nblocks, hipStream_t st) {
-  // shapes whose tiles do not hold 8 to 64 patches are never selected
-  // (stage_fn returns no launcher for them) and are not instantiated
-  constexpr int npt = 16 * RF * WR / G;
-  if constexpr (npt == 8 || npt == 16 || npt == 32 || npt == 64) {
-    hipLaunchKernelGGL((k_stage_b<NC, G, RF, MF, WR, WK, U>), dim3(nblocks),
-                       dim3(64 * WR * WK), 0, st, e);
-  }
+template <int RF, int MF, int W, int U, int PIPE>
+static void launch_prod_b(const EnsArgs& e, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_prod_b<RF, MF, W, U, PIPE>), dim3(nblocks), dim3(64 * W), 0, st, e);
 }
 
 template <int NC, int G, int MF>
-static EnsStageFn stage_fn_mf(int cfg) {
+static void launch_epi_b(const EnsArgs& e, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_epi_b<NC, G, MF>), dim3(nblocks), dim3((kEpiRows / G) * 16 * MF), 0,
+                     st, e);
+}
+
+// shapes with four or more 16-row blocks per wave only with up to two
+// member blocks, eight only with one or two (register budget)
+template <int MF>
+static EnsProdFn prod_fn_mf(int cfg) {
   switch (cfg) {
-    case 0: return launch_stage_b<NC, G, 1, MF, 1, 4, 2>;
-    case 1: return launch_stage_b<NC, G, 1, MF, 1, 4, 4>;
-    case 2: return launch_stage_b<NC, G, 1, MF, 1, 8, 2>;
-    case 3: return launch_stage_b<NC, G, 1, MF, 2, 4, 2>;
-    case 4: return launch_stage_b<NC, G, 2, MF, 1, 4, 2>;
-    case 5: return launch_stage_b<NC, G, 1, MF, 2, 2, 4>;
-    case 6: return launch_stage_b<NC, G, 1, MF, 4, 1, 2>;
-    default: return launch_stage_b<NC, G, 1, MF, 1, 2, 4>;
+    case 0: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 1>; else return nullptr;
+    case 1: if constexpr (MF <= 2) return launch_prod_b<4, MF, 4, 1, 1>; else return nullptr;
+    case 2: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 1>; else return nullptr;
+    case 3: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 1>; else return nullptr;
+    case 4: return launch_prod_b<2, MF, 4, 1, 1>;
+    case 5: return launch_prod_b<2, MF, 2, 1, 1>;
+    case 6: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 2, 1>; else return nullptr;
+    case 7: if constexpr (MF <= 2) return launch_prod_b<8, MF, 1, 1, 1>; else return nullptr;
+    case 8: if constexpr (MF <= 2) return launch_prod_b<8, MF, 2, 1, 1>; else return nullptr;
+    default: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 1>; else return nullptr;
   }
 }
 
-static EnsStageFn stage_fn(int ncomp, int G, int nm, int cfg) {
-  const int npt = cfg_rows(cfg) / G;
-  if (npt != 8 && npt != 16 && npt != 32 && npt != 64) return nullptr;
+static EnsProdFn prod_fn(int nm, int cfg) {
+  if (nm == 16) return prod_fn_mf<1>(cfg);
+  if (nm == 32) return prod_fn_mf<2>(cfg);
+  if (nm == 64) return prod_fn_mf<4>(cfg);
+  return nullptr;
+}
+
+static EnsEpiFn epi_fn(int ncomp, int G, int nm) {
   if (ncomp == 2 && G == 2) {
-    if (nm == 16) return stage_fn_mf<2, 2, 1>(cfg);
-    if (nm == 32) return stage_fn_mf<2, 2, 2>(cfg);
-    if (nm == 64) return stage_fn_mf<2, 2, 4>(cfg);
+    if (nm == 16) return launch_epi_b<2, 2, 1>;
+    if (nm == 32) return launch_epi_b<2, 2, 2>;
+    if (nm == 64) return launch_epi_b<2, 2, 4>;
   }
   if (ncomp == 1 && G == 1) {
-    if (nm == 16) return stage_fn_mf<1, 1, 1>(cfg);
-    if (nm == 32) return stage_fn_mf<1, 1, 2>(cfg);
-    if (nm == 64) return stage_fn_mf<1, 1, 4>(cfg);
+    if (nm == 16) return launch_epi_b<1, 1, 1>;
+    if (nm == 32) return launch_epi_b<1, 1, 2>;
+    if (nm == 64) return launch_epi_b<1, 1, 4>;
   }
   return nullptr;
 }
@@ -417,34 +488,49 @@ struct rsfg_ensemble {
   StageArgs* d_args = nullptr;
   int2* d_tiles = nullptr;
   int ntiles = 0;
+  int4* d_units = nullptr;
+  int nunits = 0;
   int cfg = 0;
   float* d_v = nullptr;
+  double* d_P = nullptr;
+  float* d_Kt = nullptr;
   long long* d_halted = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-// tile list for configuration cfg: per geometry ceil(rows / TR) row tiles,
-// ordered by decreasing row length (longest first: the dispatcher then
-// fills the CUs longest-job-first)
-static int build_tiles(rsfg_ensemble* e, int cfg) {
-  const int TR = cfg_rows(cfg);
+// Work lists for configuration cfg.  Epilogue: per geometry ceil(rows /
+// 16) tiles of 16 rows.  Product: per geometry and row group of cfg_rows
+// rows, one unit per block of CB columns; geometries with the longest rows
+// first.
+static int build_plan(rsfg_ensemble* e, int cfg) {
+  const int TR = cfg_rows(cfg), CBK = kEnsCfgs[cfg].cb / 16;
   std::vector<int> order(e->ngeo);
   for (int g = 0; g < e->ngeo; ++g) order[g] = g;
   std::sort(order.begin(), order.end(),
             [&](int x, int y) { return e->geo[x].ld > e->geo[y].ld; });
   std::vector<int2> tiles;
+  std::vector<int4> units;
   for (int g : order) {
-    e->geo[g].tile0 = (int)tiles.size();
-    e->geo[g].ntiles = (e->geo[g].rows + TR - 1) / TR;
-    for (int t = 0; t < e->geo[g].ntiles; ++t) tiles.push_back(make_int2(g, t * TR));
+    EnsGeo& q = e->geo[g];
+    q.tile0 = (int)tiles.size();
+    q.ntiles = (q.rows + kEpiRows - 1) / kEpiRows;
+    for (int t = 0; t < q.ntiles; ++t) tiles.push_back(make_int2(g, t * kEpiRows));
+    const int nk = (int)(q.ld / 16);
+    for (int r = 0; r < q.rows; r += TR)
+      for (int k = 0; k < nk; k += CBK) units.push_back(make_int4(g, r, k, std::min(nk, k + CBK)));
   }
   hipFree(e->d_tiles);
+  hipFree(e->d_units);
   e->d_tiles = nullptr;
+  e->d_units = nullptr;
   RSFG_HIP(hipMalloc(&e->d_tiles, sizeof(int2) * tiles.size()));
   RSFG_HIP(hipMemcpy(e->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
+  RSFG_HIP(hipMalloc(&e->d_units, sizeof(int4) * units.size()));
+  RSFG_HIP(hipMemcpy(e->d_units, units.data(), sizeof(int4) * units.size(), hipMemcpyHostToDevice));
   RSFG_HIP(hipMemcpy(e->d_geo, e->geo.data(), sizeof(EnsGeo) * e->ngeo, hipMemcpyHostToDevice));
   e->ntiles = (int)tiles.size();
+  e->nunits = (int)units.size();
   e->cfg = cfg;
   return RSFG_OK;
 }
@@ -473,9 +559,17 @@ static EnsArgs ens_args(rsfg_ensemble* e, int s) {
   a.args = e->d_args;
   a.geo = e->d_geo;
   a.tiles = e->d_tiles;
+  a.units = e->d_units;
   a.ngeo = e->ngeo;
+  a.cb_chunks = kEnsCfgs[e->cfg].cb / 16;
   a.s = s;
   return a;
+}
+
+// the stage product and epilogue of stage a.s for every member
+static void ens_stage(rsfg_ensemble* e, const EnsArgs& a) {
+  prod_fn(e->nm, e->cfg)(a, e->nunits, e->stream);
+  epi_fn(e->ncomp, e->G, e->nm)(a, e->ntiles, e->stream);
 }
 
 static int max_patches(rsfg_ensemble* e) {
@@ -507,7 +601,7 @@ static int ens_start(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_end) 
   EnsArgs a = ens_args(e, 0);
   if (e->ncomp == 2) hipLaunchKernelGGL(k_init_b<2>, gp, dim3(256), 0, e->stream, a);
   else hipLaunchKernelGGL(k_init_b<1>, gp, dim3(256), 0, e->stream, a);
-  stage_fn(e->ncomp, e->G, e->nm, e->cfg)(a, e->ntiles, e->stream);
+  ens_stage(e, a);
   if (e->ncomp == 2) hipLaunchKernelGGL(k_ctrl_b<2>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
   else hipLaunchKernelGGL(k_ctrl_b<1>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
   RSFG_HIP(hipGetLastError());
@@ -517,13 +611,12 @@ static int ens_start(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_end) 
 
 static void ens_attempt(rsfg_ensemble* e, int S) {
   const dim3 gp((max_patches(e) + 255) / 256, e->M);
-  EnsStageFn sf = stage_fn(e->ncomp, e->G, e->nm, e->cfg);
   EnsArgs a = ens_args(e, 0);
   if (e->ncomp == 2) hipLaunchKernelGGL(k_prep_b<2>, gp, dim3(256), 0, e->stream, a);
   else hipLaunchKernelGGL(k_prep_b<1>, gp, dim3(256), 0, e->stream, a);
   for (int s = 1; s < S; ++s) {
     a.s = s;
-    sf(a, e->ntiles, e->stream);
+    ens_stage(e, a);
   }
   if (e->ncomp == 2) hipLaunchKernelGGL(k_ctrl_b<2>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
   else hipLaunchKernelGGL(k_ctrl_b<1>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
@@ -556,7 +649,9 @@ int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_g
   e->M = ngeom * members_per_geom;
   e->ncomp = ncomp;
   e->G = G;
-  size_t vtot = 0;
+  size_t vtot = 0, ptot = 0, ktot = 0;
+  int cb_min = kEnsCfgs[0].cb;
+  for (int c = 0; c < kEnsNCfg; ++c) cb_min = std::min(cb_min, kEnsCfgs[c].cb);
   for (int g = 0; g < ngeom; ++g) {
     e->geoms.push_back(geoms[g]);
     EnsGeo q;
@@ -567,8 +662,11 @@ int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_g
     q.n = geoms[g]->n;
     q.m0 = g * members_per_geom;
     q.nm = members_per_geom;
+    q.rows_pad = (q.rows + kRowPad - 1) / kRowPad * kRowPad;
     e->geo.push_back(q);
     vtot += 2 * (size_t)members_per_geom * q.ld;
+    ptot += (size_t)((q.ld + cb_min - 1) / cb_min) * q.rows_pad * members_per_geom;
+    ktot += (size_t)q.rows_pad * q.ld;
   }
   int rc = RSFG_OK;
   hipError_t he = hipStreamCreate(&e->stream);
@@ -576,18 +674,36 @@ int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_g
   if (he == hipSuccess) he = hipEventCreate(&e->ev1);
   if (he == hipSuccess) he = hipMalloc(&e->d_v, sizeof(float) * vtot);
   if (he == hipSuccess) he = hipMemset(e->d_v, 0, sizeof(float) * vtot);
+  if (he == hipSuccess) he = hipMalloc(&e->d_P, sizeof(double) * ptot);
+  if (he == hipSuccess) he = hipMalloc(&e->d_Kt, sizeof(float) * ktot);
   if (he == hipSuccess) he = hipMalloc(&e->d_geo, sizeof(EnsGeo) * ngeom);
   if (he == hipSuccess) he = hipMalloc(&e->d_args, sizeof(StageArgs) * e->M);
   if (he == hipSuccess) he = hipMalloc(&e->d_halted, 2 * sizeof(long long) * e->M);
   if (he != hipSuccess) rc = hip_fail(he, "rsfg_ensemble_create", __FILE__, __LINE__);
   if (rc == RSFG_OK) {
     float* v = e->d_v;
+    double* pp = e->d_P;
     for (auto& q : e->geo) {
       q.V[0] = v;
       v += (size_t)q.nm * q.ld;
       q.V[1] = v;
       v += (size_t)q.nm * q.ld;
+      q.P = pp;
+      pp += (size_t)((q.ld + cb_min - 1) / cb_min) * q.rows_pad * q.nm;
     }
+    // the operators in MFMA tiles
+    float* kt = e->d_Kt;
+    for (auto& q : e->geo) {
+      q.Kt = kt;
+      const int nrb = q.rows_pad / 16;
+      const long ntile = (long)nrb * (q.ld / 16);
+      hipLaunchKernelGGL(k_tile_k, dim3((unsigned)((ntile + 3) / 4)), dim3(256), 0, e->stream,
+                         q.K, q.ld, q.rows, nrb, q.Kt);
+      kt += (size_t)q.rows_pad * q.ld;
+    }
+    he = hipGetLastError();
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    if (he != hipSuccess) rc = hip_fail(he, "rsfg_ensemble_create: tiling", __FILE__, __LINE__);
     for (int g = 0; g < ngeom && rc == RSFG_OK; ++g)
       for (int k = 0; k < members_per_geom && rc == RSFG_OK; ++k) {
         rsfg_handle* mh = nullptr;
@@ -596,10 +712,12 @@ int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_g
       }
   }
   if (rc == RSFG_OK) {
-    // default tile shape until rsfg_ensemble_tune picks one
+    // default unit shape until rsfg_ensemble_tune picks one
     int cfg = 0;
-    while (cfg < kEnsNCfg && !stage_fn(ncomp, G, e->nm, cfg)) ++cfg;
-    rc = build_tiles(e, cfg);
+    while (cfg < kEnsNCfg && !prod_fn(e->nm, cfg)) ++cfg;
+    RSFG_CHECK(epi_fn(ncomp, G, e->nm) != nullptr, RSFG_EINVAL,
+               "rsfg_ensemble_create: no epilogue for this layout");
+    rc = build_plan(e, cfg);
   }
   if (rc != RSFG_OK) {
     rsfg_ensemble_destroy(e);
@@ -722,8 +840,8 @@ int rsfg_ensemble_tune(rsfg_ensemble* e, const rsfg_solver_opts* o, int* choice,
   const dim3 gp((max_patches(e) + 255) / 256, e->M);
   for (int cfg = 0; cfg < kEnsNCfg; ++cfg) {
     if (ms) ms[cfg] = -1.0;
-    if (!stage_fn(e->ncomp, e->G, e->nm, cfg)) continue;
-    rc = build_tiles(e, cfg);
+    if (!prod_fn(e->nm, cfg)) continue;
+    rc = build_plan(e, cfg);
     if (rc) return rc;
     rc = ens_start(e, &oo, t_end);
     if (rc) return rc;
@@ -733,11 +851,10 @@ int rsfg_ensemble_tune(rsfg_ensemble* e, const rsfg_solver_opts* o, int* choice,
     // a middle stage, timed over repeated launches (it rewrites the same
     // stage buffers; the solve starts again from the accepted state)
     a.s = 1;
-    EnsStageFn sf = stage_fn(e->ncomp, e->G, e->nm, cfg);
-    sf(a, e->ntiles, e->stream);
+    ens_stage(e, a);
     RSFG_HIP(hipEventRecord(e->ev0, e->stream));
     const int nrep = 20;
-    for (int r = 0; r < nrep; ++r) sf(a, e->ntiles, e->stream);
+    for (int r = 0; r < nrep; ++r) ens_stage(e, a);
     RSFG_HIP(hipEventRecord(e->ev1, e->stream));
     RSFG_HIP(hipEventSynchronize(e->ev1));
     float t = 0.f;
@@ -749,8 +866,8 @@ int rsfg_ensemble_tune(rsfg_ensemble* e, const rsfg_solver_opts* o, int* choice,
       best = cfg;
     }
   }
-  RSFG_CHECK(best >= 0, RSFG_EINVAL, "rsfg_ensemble_tune: no tile shape fits this layout");
-  rc = build_tiles(e, best);
+  RSFG_CHECK(best >= 0, RSFG_EINVAL, "rsfg_ensemble_tune: no unit shape fits this layout");
+  rc = build_plan(e, best);
   if (rc) return rc;
   if (choice) *choice = best;
   return RSFG_OK;
@@ -763,9 +880,9 @@ int rsfg_ensemble_config(rsfg_ensemble* e, int* cfg, int* ncfg, int* shape) {
   if (shape) {
     for (int c = 0; c < kEnsNCfg; ++c) {
       shape[4 * c] = 16 * kEnsCfgs[c].rf;
-      shape[4 * c + 1] = kEnsCfgs[c].wr;
-      shape[4 * c + 2] = kEnsCfgs[c].wk;
-      shape[4 * c + 3] = kEnsCfgs[c].u;
+      shape[4 * c + 1] = kEnsCfgs[c].w;
+      shape[4 * c + 2] = kEnsCfgs[c].u;
+      shape[4 * c + 3] = kEnsCfgs[c].cb;
     }
   }
   return RSFG_OK;
@@ -780,6 +897,9 @@ int rsfg_ensemble_destroy(rsfg_ensemble* e) {
   hipFree(e->d_geo);
   hipFree(e->d_args);
   hipFree(e->d_tiles);
+  hipFree(e->d_units);
+  hipFree(e->d_P);
+  hipFree(e->d_Kt);
   hipFree(e->d_halted);
   if (e->ev0) hipEventDestroy(e->ev0);
   if (e->ev1) hipEventDestroy(e->ev1);
