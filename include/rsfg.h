@@ -293,6 +293,11 @@ int rsfg_ensemble_tune(rsfg_ensemble *e, const rsfg_solver_opts *o, int *choice,
    (operator rows per wave, waves per product unit, 16-column chunks in
    flight per wave, columns per product unit) */
 int rsfg_ensemble_config(rsfg_ensemble *e, int *cfg, int *ncfg, int *shape);
+/* mean durations by HIP events of nrep launches each of the current
+   shape's product kernel and stage epilogue (a middle stage at the
+   members' states); measurement only */
+int rsfg_ensemble_bench(rsfg_ensemble *e, const rsfg_solver_opts *o, int nrep,
+                        double *ms_product, double *ms_epilogue);
 int rsfg_ensemble_destroy(rsfg_ensemble *e);
 
 /* times nrep back-to-back launches of the fused stage kernel (K v with the

@@ -873,6 +873,41 @@ int rsfg_ensemble_tune(rsfg_ensemble* e, const rsfg_solver_opts* o, int* choice,
   return RSFG_OK;
 }
 
+int rsfg_ensemble_bench(rsfg_ensemble* e, const rsfg_solver_opts* o, int nrep, double* ms_product,
+                        double* ms_epilogue) {
+  RSFG_CHECK(e && o && nrep > 0, RSFG_EINVAL, "rsfg_ensemble_bench: bad arguments");
+  RSFG_HIP(hipSetDevice(e->device));
+  rsfg_solver_opts oo = *o;
+  oo.prec = RSFG_PREC_FP32;
+  int rc = fill_member_args(e, &oo);
+  if (rc) return rc;
+  rc = ens_start(e, &oo, 1e300);
+  if (rc) return rc;
+  const dim3 gp((max_patches(e) + 255) / 256, e->M);
+  EnsArgs a = ens_args(e, 0);
+  if (e->ncomp == 2) hipLaunchKernelGGL(k_prep_b<2>, gp, dim3(256), 0, e->stream, a);
+  else hipLaunchKernelGGL(k_prep_b<1>, gp, dim3(256), 0, e->stream, a);
+  a.s = 1;
+  ens_stage(e, a);
+  EnsProdFn pf = prod_fn(e->nm, e->cfg);
+  EnsEpiFn ef = epi_fn(e->ncomp, e->G, e->nm);
+  float t = 0.f;
+  RSFG_HIP(hipEventRecord(e->ev0, e->stream));
+  for (int r = 0; r < nrep; ++r) pf(a, e->nunits, e->stream);
+  RSFG_HIP(hipEventRecord(e->ev1, e->stream));
+  RSFG_HIP(hipEventSynchronize(e->ev1));
+  RSFG_HIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
+  if (ms_product) *ms_product = t / nrep;
+  RSFG_HIP(hipEventRecord(e->ev0, e->stream));
+  for (int r = 0; r < nrep; ++r) ef(a, e->ntiles, e->stream);
+  RSFG_HIP(hipEventRecord(e->ev1, e->stream));
+  RSFG_HIP(hipEventSynchronize(e->ev1));
+  RSFG_HIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
+  if (ms_epilogue) *ms_epilogue = t / nrep;
+  RSFG_HIP(hipGetLastError());
+  return RSFG_OK;
+}
+
 int rsfg_ensemble_config(rsfg_ensemble* e, int* cfg, int* ncfg, int* shape) {
   RSFG_CHECK(e, RSFG_EINVAL, "rsfg_ensemble_config: null argument");
   if (cfg) *cfg = e->cfg;

@@ -362,6 +362,14 @@ class Ensemble:
                 "rsfg_ensemble_tune")
         return ch.value, list(ms), self.shapes()
 
+    def bench(self, nrep=20, **kw):
+        """(product ms, epilogue ms) of the current shape by HIP events"""
+        o = FaultSystem.solver_options(**kw)
+        mp, me = C.c_double(), C.c_double()
+        L.check(self._lib.rsfg_ensemble_bench(self.e, C.byref(o), nrep, C.byref(mp), C.byref(me)),
+                "rsfg_ensemble_bench")
+        return mp.value, me.value
+
     def shapes(self):
         ncfg, cur = C.c_int(), C.c_int()
         self._lib.rsfg_ensemble_config(self.e, C.byref(cur), C.byref(ncfg), None)

@@ -141,6 +141,8 @@ def test_ensemble_matches_single_fp32_runs_and_tuner():
     timed = [t for t in ms if t > 0]
     assert len(timed) >= 4 and ms[choice] == min(timed)
     assert ens.config() == choice
+    pm, pe = ens.bench(5, rtol=1e-6)
+    assert 0 < pm < ms[choice] and 0 < pe < ms[choice]
     for m, (gi, f, X0) in zip(ens.members, runs):
         m.set_state(X0)
     t_end = 20.0 * SEC_PER_YEAR
