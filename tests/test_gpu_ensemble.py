@@ -163,6 +163,42 @@ def test_ensemble_matches_single_fp32_runs_and_tuner():
         q[0].close()
 
 
+def test_ensemble_of_64_members():
+    """64 members on one geometry (four 16-member MFMA blocks), after
+    tuning: every eighth member reproduces the single-fault FP32 solver's
+    onsets within 1e-4 yr and its step count within 3 %."""
+    from interact_amd import faults, lib
+    from interact_amd.model import Ensemble
+    geoms = _geoms(2.0, lib.SLIP_VECTOR, outlines=[(40, 20)])
+    fs, g, L, W = geoms[0]
+    nm = 64
+    ens = Ensemble([fs], nm)
+    runs = []
+    for k in range(nm):
+        f = faults.ensemble_fields(g, L, W, k)
+        runs.append((f, _configure(ens.members[k], f, 2)))
+    choice, ms, shapes = ens.tune(rtol=1e-6)
+    assert ms[choice] > 0
+    for m, (f, X0) in zip(ens.members, runs):
+        m.set_state(X0)
+    t_end = 20.0 * SEC_PER_YEAR
+    st = ens.solve(t_end, rtol=1e-6)
+    assert all(q["status"] == 0 for q in st)
+    for i in range(0, nm, 8):
+        f, X0 = runs[i]
+        s = fs.member()
+        _configure(s, f, 2)
+        s1 = s.solve(t_end, rtol=1e-6, prec=lib.PREC_FP32)
+        on1, on2 = _onsets(s.events()), _onsets(ens.members[i].events())
+        assert len(on1) == len(on2)
+        if len(on1):
+            assert np.abs(on1 - on2).max() < 1e-4
+        assert abs(s1["steps"] - st[i]["steps"]) <= 0.03 * s1["steps"] + 3
+        s.close()
+    ens.close()
+    fs.close()
+
+
 def test_ensemble_argument_checks():
     from interact_amd import faults, lib
     from interact_amd.model import Ensemble
