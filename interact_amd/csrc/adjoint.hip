@@ -607,17 +607,19 @@ static void rp(const AdjSetup& s, const RpLaunch& l) {
 }
 
 // Replay steps [s0, s1) from the state in Ys[0] whose slip rates are in
-// vb[*vc].  record: stage states and stressing rates of the steps into the
-// segment record; trace: ln|v| rows; cp: checkpoints at multiples of C.
+// vb[*vc].  Steps in [r0, r1) record their stage states and stressing rates
+// into the segment record (step r0 first); trace: ln|v| rows; cp:
+// checkpoints at multiples of C.
 static void replay_steps(AdjSetup& s, AdjWork& W, const double* steps, long long s0,
-                         long long s1, bool first_eval, bool record, bool trace, bool cp,
-                         int* vc) {
+                         long long s1, bool first_eval, long long r0, long long r1, bool trace,
+                         bool cp, int* vc) {
   rsfg_ctx* h = s.h;
   const int S = s.S, n = s.n;
   const size_t st = (size_t)s.D * n;
   const size_t vbytes = W.vlen * sizeof(double);
   auto rec_of = [&](long long step, int stage, double** ry, double** rt) {
-    const size_t k = (size_t)(step - s0) * (S - 1) + stage;
+    if (step < r0 || step >= r1) return;
+    const size_t k = (size_t)(step - r0) * (S - 1) + stage;
     *ry = W.segY + k * st;
     *rt = W.segT + k * (size_t)s.NC * n;
   };
@@ -627,7 +629,7 @@ static void replay_steps(AdjSetup& s, AdjWork& W, const double* steps, long long
     memset(&l, 0, sizeof l);
     l.Y = W.Ys[0];
     l.Kout = W.Kd[0];
-    if (record) rec_of(s0, 0, &l.recY, &l.recT);
+    rec_of(s0, 0, &l.recY, &l.recT);
     l.trow = trace ? s0 : -1;
     l.Xb = W.Ys[0];
     l.Kj[0] = W.Kd[0];
@@ -649,7 +651,7 @@ static void replay_steps(AdjSetup& s, AdjWork& W, const double* steps, long long
       l.vin = W.vb[*vc];
       l.vout = W.vb[*vc ^ 1];
       if (i < S - 1) {
-        if (record) rec_of(m, i, &l.recY, &l.recT);
+        rec_of(m, i, &l.recY, &l.recT);
         l.trow = -1;
         l.Xb = W.Ys[0];
         for (int j = 0; j <= i; ++j) {
@@ -660,7 +662,7 @@ static void replay_steps(AdjSetup& s, AdjWork& W, const double* steps, long long
         l.Yn = W.Ys[i + 1];
       } else {
         // Y_{S-1} = X_{m+1}: its derivative is K_0 of step m + 1
-        if (record && m + 1 < s1) rec_of(m + 1, 0, &l.recY, &l.recT);
+        if (m + 1 < s1) rec_of(m + 1, 0, &l.recY, &l.recT);
         l.trow = trace ? m + 1 : -1;
         if (m + 1 < s.N) {
           l.Xb = W.Ys[S - 1];
@@ -720,8 +722,11 @@ static int adj_setup(rsfg_ctx* h, const double* steps, long long nsteps,
   return RSFG_OK;
 }
 
-// first pass: X_0 from the handle, trace, checkpoints, J
-static int forward_pass(AdjSetup& s, AdjWork& W, const double* steps, int* vc) {
+// first pass: X_0 from the handle, trace, checkpoints, J; with a gradient
+// to follow (record_last) the stages of the last segment are recorded on
+// the way, so that the backward sweep starts without recomputing them
+static int forward_pass(AdjSetup& s, AdjWork& W, const double* steps, int* vc,
+                        bool record_last) {
   rsfg_ctx* h = s.h;
   const size_t st = (size_t)s.D * s.n;
   s.ra.recv_of = W.recv_of;
@@ -741,7 +746,9 @@ static int forward_pass(AdjSetup& s, AdjWork& W, const double* steps, int* vc) {
   RSFG_HIP(hipMemcpyAsync(W.cpX, W.Ys[0], st * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
   RSFG_HIP(hipMemcpyAsync(W.cpV, W.vb[0], W.vlen * sizeof(double), hipMemcpyDeviceToDevice,
                           h->stream));
-  replay_steps(s, W, steps, 0, s.N, true, false, true, true, vc);
+  const long long last0 = (long long)(s.ncp - 1) * s.C;
+  replay_steps(s, W, steps, 0, s.N, true, record_last ? last0 : 0, record_last ? s.N : 0, true,
+               true, vc);
   if (s.pb->nrecv > 0)
     hipLaunchKernelGGL(k_objective, dim3(1), dim3(kAdjThreads), 0, h->stream, W.trace, W.target,
                        W.weight, (long)(s.N + 1), s.pb->nrecv, W.J);
@@ -791,13 +798,16 @@ static int backward_pass(AdjSetup& s, AdjWork& W, const double* steps) {
   const bool f64 = s.pb->prec == RSFG_PREC_FP64;
   for (long k = s.ncp - 1; k >= 0; --k) {
     const long long s0 = (long long)k * s.C, s1 = std::min<long long>(s.N, s0 + s.C);
-    // recompute the segment from its checkpoint, recording every stage
-    RSFG_HIP(hipMemcpyAsync(W.Ys[0], W.cpX + k * st, st * sizeof(double), hipMemcpyDeviceToDevice,
-                            h->stream));
-    RSFG_HIP(hipMemcpyAsync(W.vb[0], W.cpV + k * W.vlen, W.vlen * sizeof(double),
-                            hipMemcpyDeviceToDevice, h->stream));
-    int vc = 0;
-    replay_steps(s, W, steps, s0, s1, true, true, false, false, &vc);
+    // recompute the segment from its checkpoint, recording every stage (the
+    // last segment was recorded by the forward pass)
+    if (k < s.ncp - 1) {
+      RSFG_HIP(hipMemcpyAsync(W.Ys[0], W.cpX + k * st, st * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+      RSFG_HIP(hipMemcpyAsync(W.vb[0], W.cpV + k * W.vlen, W.vlen * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+      int vc = 0;
+      replay_steps(s, W, steps, s0, s1, true, s0, s1, false, false, &vc);
+    }
     for (long long m = s1 - 1; m >= s0; --m) {
       const double hm = steps[m];
       const size_t base = (size_t)(m - s0) * (S - 1);
@@ -843,7 +853,7 @@ int rsfg_replay(rsfg_handle* h, const double* steps, long long nsteps,
   rc = adj_alloc(s, W, false);
   if (rc) return rc;
   int vc = 0;
-  rc = forward_pass(s, W, steps, &vc);
+  rc = forward_pass(s, W, steps, &vc, false);
   if (rc) return rc;
   if (J) RSFG_HIP(hipMemcpyAsync(J, W.J, sizeof(double), hipMemcpyDeviceToHost, h->stream));
   if (trace && pb->nrecv > 0)
@@ -876,7 +886,7 @@ int rsfg_adjoint_gradient(rsfg_handle* h, const double* steps, long long nsteps,
   rc = adj_alloc(s, W, true);
   if (rc) return rc;
   int vc = 0;
-  rc = forward_pass(s, W, steps, &vc);
+  rc = forward_pass(s, W, steps, &vc, true);
   if (rc) return rc;
   rc = (s.NC == 2) ? backward_pass<2>(s, W, steps) : backward_pass<1>(s, W, steps);
   if (rc) return rc;
