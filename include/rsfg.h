@@ -289,10 +289,16 @@ int rsfg_ensemble_solve(rsfg_ensemble *e, const rsfg_solver_opts *o, double t_en
    ensemble (members must be configured), keeps the fastest; ms (may be
    NULL) receives one time per candidate, -1 where a shape does not fit */
 int rsfg_ensemble_tune(rsfg_ensemble *e, const rsfg_solver_opts *o, int *choice, double *ms);
-/* current unit shape, number of candidates, and per candidate four ints
-   (operator rows per wave, waves per product unit, 16-column chunks in
-   flight per wave, columns per product unit) */
+/* current unit shape, number of candidates, and per candidate five ints
+   (operator rows per wave, waves per product unit, 16-column chunks per
+   load group, columns per product unit, load groups each wave keeps in its
+   register ring -- 0 where the unit's waves share the member slip-rate
+   slices through LDS instead) */
 int rsfg_ensemble_config(rsfg_ensemble *e, int *cfg, int *ncfg, int *shape);
+/* use candidate shape cfg (as listed by rsfg_ensemble_config) without
+   tuning, e.g. a shape tuned earlier on the same layout; RSFG_EINVAL when
+   it is out of range or does not fit the members per geometry */
+int rsfg_ensemble_select(rsfg_ensemble *e, int cfg);
 /* mean durations by HIP events of nrep launches each of the current
    shape's product kernel and stage epilogue (a middle stage at the
    members' states); measurement only */

@@ -32,11 +32,14 @@ typedef float v4f_t __attribute__((ext_vector_type(4)));
 namespace rsfg {
 
 struct EnsGeo {        // one geometry of the ensemble
-  const float* K;      // FP32 operator, rows x ld (zero-padded columns)
+  const float* K;      // FP32 operator, rows x ldk (zero-padded columns)
   float* Kt;           // the same in MFMA tiles: rows_pad / 16 x ld / 16 tiles of
                        // 16 x 16, tile (b, c) at Kt + (b ld / 16 + c) 256, lane
                        // l = 16 j + i holding row 16 b + i, columns 16 c + 4 j .. + 3
-  long ld;             // leading dimension of K and of V
+                       // (zero past the operator's rows and columns)
+  long ldk;            // leading dimension of K
+  long ld;             // ldk rounded up to whole 256-column segments: columns of Kt
+                       // and leading dimension of V
   int rows, n;         // G n operator rows, n patches
   int m0, nm;          // first member and member count
   int tile0, ntiles;   // epilogue tiles (16 rows) of this geometry in the tile list
@@ -47,6 +50,7 @@ struct EnsGeo {        // one geometry of the ensemble
 
 constexpr int kEpiRows = 16;      // operator rows per epilogue tile
 constexpr int kRowPad = 256;      // rows_pad granularity (largest product row group)
+constexpr int kColPad = 256;      // ld granularity (the product's FP64 flush segment)
 
 struct EnsArgs {
   const StageArgs* args;  // per member
@@ -115,8 +119,9 @@ __global__ void k_halted_b(EnsArgs e, int M, long long* out) {
   }
 }
 
-// the operator in MFMA tiles (rows past the last are zero)
-__global__ void k_tile_k(const float* __restrict__ K, long ld, int rows, int nrb,
+// the operator in MFMA tiles (rows past the last and columns past ldk are
+// zero; ldk and ld are multiples of 16)
+__global__ void k_tile_k(const float* __restrict__ K, long ldk, int rows, int nrb, long ld,
                          float* __restrict__ Kt) {
   const long nk = ld / 16;
   const long t = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);  // tile
@@ -125,7 +130,8 @@ __global__ void k_tile_k(const float* __restrict__ K, long ld, int rows, int nrb
   const long b = t / nk, c = t % nk;
   const long row = 16 * b + i;
   v4f_t v = v4f_t{0.f, 0.f, 0.f, 0.f};
-  if (row < rows) v = *reinterpret_cast<const v4f_t*>(K + row * ld + 16 * c + 4 * j);
+  if (row < rows && 16 * c < ldk)
+    v = *reinterpret_cast<const v4f_t*>(K + row * ldk + 16 * c + 4 * j);
   *reinterpret_cast<v4f_t*>(Kt + t * 256 + 4 * l) = v;
 }
 
@@ -136,24 +142,34 @@ __global__ void k_tile_k(const float* __restrict__ K, long ld, int rows, int nrb
 // slices of the member-major product input; the four MFMAs of a chunk take
 // component q of both, so A[i][j] and B[j][i] index the same column
 // k0 + 4j + q.  FP32 partial sums are added into FP64 every 16 chunks (256
-// columns).  Output lane layout (checked on the device): acc[rf][mf][r] =
-// Y[row 16 rf + 4 (lane / 16) + r][member 16 mf + lane % 16].
-// PIPE: the loads of the next U chunks are issued before the MFMAs of the
-// current ones (two register sets), so that a wave keeps its own loads in
-// flight while its matrix instructions run; the products and the order of
-// summation are the same.
-template <int RF, int MF, int U, int PIPE>
+// columns; unit ranges start and end on such segments).  Output lane layout
+// (checked on the device): acc[rf][mf][r] = Y[row 16 rf + 4 (lane / 16) +
+// r][member 16 mf + lane % 16].
+// The loads rotate through a ring of D register sets of U chunks each: the
+// loads of group g + D - 1 are issued before the MFMAs of group g, so a wave
+// keeps D - 1 groups (D - 1 U RF kilobytes of operator) in flight while its
+// matrix instructions run.  The ring's slot of a group is fixed by its place
+// in the 256-column segment, so every slot index is a compile-time constant.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* global_ptr(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
+template <int RF, int MF, int U, int D>
 __device__ __forceinline__ void wave_product(const float* __restrict__ Kt, long ld, int rw,
                                              const float* __restrict__ V, int kc0, int kc1,
                                              double (&acc)[RF][MF][4]) {
+  constexpr int NG = 16 / U;   // load groups per 256-column segment
+  static_assert(16 % U == 0 && D >= 2 && NG % D == 0, "ring slots repeat every segment");
   const int lane = threadIdx.x & 63, i = lane & 15, j = lane >> 4;
   const long nk = ld / 16;
-  const float* ka[RF];
-  const float* vb[MF];
+  const __attribute__((address_space(1))) float* ka[RF];
+  const __attribute__((address_space(1))) float* vb[MF];
 #pragma unroll
-  for (int rf = 0; rf < RF; ++rf) ka[rf] = Kt + ((long)(rw / 16 + rf) * nk) * 256 + 4 * lane;
+  for (int rf = 0; rf < RF; ++rf)
+    ka[rf] = global_ptr(Kt + ((long)(rw / 16 + rf) * nk) * 256 + 4 * lane);
 #pragma unroll
-  for (int mf = 0; mf < MF; ++mf) vb[mf] = V + (size_t)(16 * mf + i) * ld + 4 * j;
+  for (int mf = 0; mf < MF; ++mf) vb[mf] = global_ptr(V + (size_t)(16 * mf + i) * ld + 4 * j);
 #pragma unroll
   for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
@@ -161,44 +177,150 @@ __device__ __forceinline__ void wave_product(const float* __restrict__ Kt, long 
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[rf][mf][r] = 0.0;
   if (kc0 >= kc1) return;
-  v4f_t A[U][RF], B[U][MF];
+  typedef const __attribute__((address_space(1))) v4f_t gv4;
+  v4f_t A[D][U][RF], B[D][U][MF];
   auto load = [&](int kc, v4f_t (&a)[U][RF], v4f_t (&b)[U][MF]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
 #pragma unroll
       for (int rf = 0; rf < RF; ++rf)
         a[u][rf] = __builtin_nontemporal_load(
-            reinterpret_cast<const v4f_t*>(ka[rf] + (size_t)(kc + u) * 256));
+            reinterpret_cast<gv4*>(ka[rf] + (size_t)(kc + u) * 256));
 #pragma unroll
       for (int mf = 0; mf < MF; ++mf)
-        b[u][mf] = *reinterpret_cast<const v4f_t*>(vb[mf] + (size_t)(kc + u) * 16);
+        b[u][mf] = *reinterpret_cast<gv4*>(vb[mf] + (size_t)(kc + u) * 16);
     }
+  };
+  // a unit holds whole segments, so the first D - 1 groups exist; past the
+  // range's end the ring reloads its last group instead (a branch around the
+  // loads would make the compiler wait for every load in flight there)
+#pragma unroll
+  for (int d = 0; d < D - 1; ++d) load(kc0 + d * U, A[d], B[d]);
+  for (int seg = kc0; seg < kc1; seg += 16) {
+    v4f_t c[RF][MF];
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf) c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      const int kn = min(seg + (g + D - 1) * U, kc1 - U);
+      load(kn, A[(g + D - 1) % D], B[(g + D - 1) % D]);
+      // keep the loads here: the scheduler would otherwise sink them to
+      // their use and empty the ring
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+            for (int mf = 0; mf < MF; ++mf)
+              c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[g % D][u][rf][q],
+                                                                B[g % D][u][mf][q], c[rf][mf], 0,
+                                                                0, 0);
+    }
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[rf][mf][r] += (double)c[rf][mf][r];
+  }
+}
+
+// The product of a whole unit with the member slices staged in LDS: the W
+// waves of the workgroup load each chunk group's slices once (NM members x
+// 16 U columns, double-buffered) and every wave reads them from there, so
+// the per-CU memory traffic is mostly the operator stream.  The operator
+// tiles stream as in wave_product (loads one group ahead); products and
+// summation order are those of wave_product.
+constexpr int kLdsRow = 20;   // floats per staged member row of a chunk (16 + 4 pad)
+
+template <int RF, int MF, int W, int U>
+__device__ __forceinline__ void unit_product_lds(const float* __restrict__ Kt, long ld, int rw,
+                                                 const float* __restrict__ V, int kc0, int kc1,
+                                                 float* sV, double (&acc)[RF][MF][4]) {
+  constexpr int NM = 16 * MF;
+  constexpr int NQ = NM * U * 4;                    // 16-byte pieces of one staged group
+  constexpr int NV = (NQ + 64 * W - 1) / (64 * W);  // pieces per thread
+  const int tid = threadIdx.x, lane = tid & 63, i = lane & 15, j = lane >> 4;
+  const long nk = ld / 16;
+  const float* ka[RF];
+#pragma unroll
+  for (int rf = 0; rf < RF; ++rf) ka[rf] = Kt + ((long)(rw / 16 + rf) * nk) * 256 + 4 * lane;
+#pragma unroll
+  for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+    for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[rf][mf][r] = 0.0;
+  if (kc0 >= kc1) return;
+  // piece x = tid + 64 W t: member x / (4 U), chunk (x / 4) % U, quarter x % 4;
+  // staged at sV + ((buffer U + chunk) NM + member) kLdsRow + 4 quarter
+  v4f_t rv[NV];
+  auto vload = [&](int kc) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) {
+      const int x = tid + 64 * W * t;
+      if (NQ % (64 * W) == 0 || x < NQ)
+        rv[t] = *reinterpret_cast<const v4f_t*>(V + (size_t)(x / (4 * U)) * ld +
+                                                (size_t)kc * 16 + 4 * (x % (4 * U)));
+    }
+  };
+  auto vstore = [&](int b) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) {
+      const int x = tid + 64 * W * t;
+      if (NQ % (64 * W) == 0 || x < NQ) {
+        const int m = x / (4 * U), u = (x / 4) % U, q = x % 4;
+        *reinterpret_cast<v4f_t*>(sV + ((b * U + u) * NM + m) * kLdsRow + 4 * q) = rv[t];
+      }
+    }
+  };
+  v4f_t A[U][RF];
+  auto kload = [&](int kc, v4f_t (&a)[U][RF]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int rf = 0; rf < RF; ++rf)
+        a[u][rf] = __builtin_nontemporal_load(
+            reinterpret_cast<const v4f_t*>(ka[rf] + (size_t)(kc + u) * 256));
   };
   v4f_t c[RF][MF];
 #pragma unroll
   for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
     for (int mf = 0; mf < MF; ++mf) c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
-  if (PIPE) load(kc0, A, B);
+  kload(kc0, A);
+  vload(kc0);
+  vstore(0);
+  __syncthreads();
+  int b = 0;
   for (int kc = kc0; kc < kc1; kc += U) {
     const bool more = kc + U < kc1;
-    v4f_t An[U][RF], Bn[U][MF];
-    if (PIPE) {
-      if (more) load(kc + U, An, Bn);
-    } else {
-      load(kc, A, B);
+    v4f_t An[U][RF];
+    if (more) {
+      kload(kc + U, An);
+      vload(kc + U);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u)
+    for (int u = 0; u < U; ++u) {
+      v4f_t B[MF];
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf)
+        B[mf] = *reinterpret_cast<const v4f_t*>(sV + ((b * U + u) * NM + 16 * mf + i) * kLdsRow +
+                                                4 * j);
 #pragma unroll
       for (int q = 0; q < 4; ++q)
 #pragma unroll
         for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
           for (int mf = 0; mf < MF; ++mf)
-            c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u][rf][q], B[u][mf][q],
-                                                              c[rf][mf], 0, 0, 0);
-    // FP64 flush every 16 chunks from the start of the range, and at its end
+            c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u][rf][q], B[mf][q], c[rf][mf],
+                                                              0, 0, 0);
+    }
     if (((kc + U - kc0) & 15) == 0 || !more) {
 #pragma unroll
       for (int rf = 0; rf < RF; ++rf)
@@ -209,15 +331,16 @@ __device__ __forceinline__ void wave_product(const float* __restrict__ Kt, long 
           c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
         }
     }
-    if (PIPE && more) {
+    // the other buffer was last read before the previous barrier
+    if (more) vstore(b ^ 1);
+    __syncthreads();
+    if (more) {
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
+      for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int rf = 0; rf < RF; ++rf) A[u][rf] = An[u][rf];
-#pragma unroll
-        for (int mf = 0; mf < MF; ++mf) B[u][mf] = Bn[u][mf];
-      }
     }
+    b ^= 1;
   }
 }
 
@@ -324,8 +447,11 @@ __device__ __forceinline__ bool geometry_running(const EnsArgs& e, const EnsGeo&
 // One product unit per workgroup: rows [r0, r0 + 16 RF W) of one geometry
 // (16 RF per wave) over the 16-column chunks [kc0, kc1), for all 16 MF
 // members, into that column block's partials.
-template <int RF, int MF, int W, int U, int PIPE>
+// D >= 2: each wave streams through a ring of D register sets; D = 0: the
+// unit's waves share the member slices through LDS.
+template <int RF, int MF, int W, int U, int D>
 __global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
+  __shared__ float sV[(D == 0) ? 2 * U * 16 * MF * kLdsRow : 1];
   constexpr int NM = 16 * MF;
   const int4 u = e.units[blockIdx.x];
   const EnsGeo& gg = e.geo[u.x];
@@ -333,7 +459,10 @@ __global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int rw = u.y + wave * 16 * RF;
   double acc[RF][MF][4];
-  wave_product<RF, MF, U, PIPE>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], u.z, u.w, acc);
+  if constexpr (D == 0)
+    unit_product_lds<RF, MF, W, U>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], u.z, u.w, sV, acc);
+  else
+    wave_product<RF, MF, U, D>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], u.z, u.w, acc);
   double* P = gg.P + (size_t)(u.z / e.cb_chunks) * gg.rows_pad * NM;
 #pragma unroll
   for (int rf = 0; rf < RF; ++rf)
@@ -410,13 +539,15 @@ __global__ __launch_bounds__(64 * (kEpiRows / G) * MF / 4) void k_epi_b(EnsArgs 
 
 // ------------------------------------------------------------ host side
 struct EnsCfg {
-  int rf, w, u, cb, pipe;  // 16 RF rows per wave, W waves per unit, U chunks per load
-                           // group, CB columns per unit, loads issued one group ahead
+  int rf, w, u, cb, d;  // 16 RF rows per wave, W waves per unit, U chunks per load group,
+                        // CB columns per unit, D register sets in each wave's load ring
+                        // (0: the unit's waves share the member slices through LDS)
 };
 static const EnsCfg kEnsCfgs[] = {
-    {4, 2, 1, 1024, 1}, {4, 4, 1, 1024, 1}, {4, 2, 1, 2048, 1}, {4, 1, 1, 1024, 1},
-    {2, 4, 1, 1024, 1}, {2, 2, 1, 1024, 1}, {4, 2, 2, 1024, 1}, {8, 1, 1, 1024, 1},
-    {8, 2, 1, 1024, 1}, {4, 2, 1, 512, 1}};
+    {4, 1, 1, 1024, 2}, {4, 2, 1, 1024, 2}, {4, 1, 1, 512, 2},  {2, 1, 1, 1024, 2},
+    {2, 2, 1, 1024, 2}, {4, 1, 2, 1024, 2}, {8, 1, 1, 1024, 2}, {4, 1, 1, 2048, 2},
+    {2, 4, 1, 1024, 2}, {4, 1, 1, 1024, 4}, {2, 2, 1, 1024, 8}, {2, 8, 1, 1024, 0},
+    {4, 4, 2, 1024, 0}};
 constexpr int kEnsNCfg = sizeof(kEnsCfgs) / sizeof(kEnsCfgs[0]);
 
 static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].w; }
@@ -424,9 +555,9 @@ static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].w; }
 typedef void (*EnsProdFn)(const EnsArgs&, int nblocks, hipStream_t);
 typedef void (*EnsEpiFn)(const EnsArgs&, int nblocks, hipStream_t);
 
-template <int RF, int MF, int W, int U, int PIPE>
+template <int RF, int MF, int W, int U, int D>
 static void launch_prod_b(const EnsArgs& e, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_prod_b<RF, MF, W, U, PIPE>), dim3(nblocks), dim3(64 * W), 0, st, e);
+  hipLaunchKernelGGL((k_prod_b<RF, MF, W, U, D>), dim3(nblocks), dim3(64 * W), 0, st, e);
 }
 
 template <int NC, int G, int MF>
@@ -436,20 +567,23 @@ static void launch_epi_b(const EnsArgs& e, int nblocks, hipStream_t st) {
 }
 
 // shapes with four or more 16-row blocks per wave only with up to two
-// member blocks, eight only with one or two (register budget)
+// member blocks, eight only with one (register budget)
 template <int MF>
 static EnsProdFn prod_fn_mf(int cfg) {
   switch (cfg) {
-    case 0: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 1>; else return nullptr;
-    case 1: if constexpr (MF <= 2) return launch_prod_b<4, MF, 4, 1, 1>; else return nullptr;
-    case 2: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 1>; else return nullptr;
-    case 3: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 1>; else return nullptr;
-    case 4: return launch_prod_b<2, MF, 4, 1, 1>;
-    case 5: return launch_prod_b<2, MF, 2, 1, 1>;
-    case 6: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 2, 1>; else return nullptr;
-    case 7: if constexpr (MF <= 2) return launch_prod_b<8, MF, 1, 1, 1>; else return nullptr;
-    case 8: if constexpr (MF <= 2) return launch_prod_b<8, MF, 2, 1, 1>; else return nullptr;
-    default: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 1>; else return nullptr;
+    case 0: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 2>; else return nullptr;
+    case 1: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 2>; else return nullptr;
+    case 2: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 2>; else return nullptr;
+    case 3: return launch_prod_b<2, MF, 1, 1, 2>;
+    case 4: return launch_prod_b<2, MF, 2, 1, 2>;
+    case 5: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 2, 2>; else return nullptr;
+    case 6: if constexpr (MF <= 1) return launch_prod_b<8, MF, 1, 1, 2>; else return nullptr;
+    case 7: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 2>; else return nullptr;
+    case 8: return launch_prod_b<2, MF, 4, 1, 2>;
+    case 9: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 4>; else return nullptr;
+    case 10: if constexpr (MF <= 2) return launch_prod_b<2, MF, 2, 1, 8>; else return nullptr;
+    case 11: return launch_prod_b<2, MF, 8, 1, 0>;
+    default: if constexpr (MF <= 2) return launch_prod_b<4, MF, 4, 2, 0>; else return nullptr;
   }
 }
 
@@ -657,7 +791,8 @@ int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_g
     EnsGeo q;
     memset(&q, 0, sizeof q);
     q.K = geoms[g]->K32;
-    q.ld = geoms[g]->ld;
+    q.ldk = geoms[g]->ld;
+    q.ld = (q.ldk + kColPad - 1) / kColPad * kColPad;
     q.rows = geoms[g]->rows_all;
     q.n = geoms[g]->n;
     q.m0 = g * members_per_geom;
@@ -698,7 +833,7 @@ int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_g
       const int nrb = q.rows_pad / 16;
       const long ntile = (long)nrb * (q.ld / 16);
       hipLaunchKernelGGL(k_tile_k, dim3((unsigned)((ntile + 3) / 4)), dim3(256), 0, e->stream,
-                         q.K, q.ld, q.rows, nrb, q.Kt);
+                         q.K, q.ldk, q.rows, nrb, q.ld, q.Kt);
       kt += (size_t)q.rows_pad * q.ld;
     }
     he = hipGetLastError();
@@ -914,13 +1049,23 @@ int rsfg_ensemble_config(rsfg_ensemble* e, int* cfg, int* ncfg, int* shape) {
   if (ncfg) *ncfg = kEnsNCfg;
   if (shape) {
     for (int c = 0; c < kEnsNCfg; ++c) {
-      shape[4 * c] = 16 * kEnsCfgs[c].rf;
-      shape[4 * c + 1] = kEnsCfgs[c].w;
-      shape[4 * c + 2] = kEnsCfgs[c].u;
-      shape[4 * c + 3] = kEnsCfgs[c].cb;
+      shape[5 * c] = 16 * kEnsCfgs[c].rf;
+      shape[5 * c + 1] = kEnsCfgs[c].w;
+      shape[5 * c + 2] = kEnsCfgs[c].u;
+      shape[5 * c + 3] = kEnsCfgs[c].cb;
+      shape[5 * c + 4] = kEnsCfgs[c].d;
     }
   }
   return RSFG_OK;
+}
+
+int rsfg_ensemble_select(rsfg_ensemble* e, int cfg) {
+  RSFG_CHECK(e, RSFG_EINVAL, "rsfg_ensemble_select: null argument");
+  RSFG_CHECK(cfg >= 0 && cfg < kEnsNCfg, RSFG_EINVAL, "rsfg_ensemble_select: shape out of range");
+  RSFG_CHECK(prod_fn(e->nm, cfg), RSFG_EINVAL,
+             "rsfg_ensemble_select: shape does not fit the members per geometry");
+  RSFG_HIP(hipSetDevice(e->device));
+  return build_plan(e, cfg);
 }
 
 int rsfg_ensemble_destroy(rsfg_ensemble* e) {

@@ -373,9 +373,13 @@ class Ensemble:
     def shapes(self):
         ncfg, cur = C.c_int(), C.c_int()
         self._lib.rsfg_ensemble_config(self.e, C.byref(cur), C.byref(ncfg), None)
-        sh = (C.c_int * (4 * ncfg.value))()
+        sh = (C.c_int * (5 * ncfg.value))()
         self._lib.rsfg_ensemble_config(self.e, None, None, sh)
-        return [tuple(sh[4 * i:4 * i + 4]) for i in range(ncfg.value)]
+        return [tuple(sh[5 * i:5 * i + 5]) for i in range(ncfg.value)]
+
+    def select(self, cfg):
+        """use candidate shape cfg (an index into shapes()) without tuning"""
+        L.check(self._lib.rsfg_ensemble_select(self.e, cfg), "rsfg_ensemble_select")
 
     def config(self):
         cur = C.c_int()

@@ -137,7 +137,7 @@ def test_ensemble_matches_single_fp32_runs_and_tuner():
             X0 = _configure(ens.members[gi * nm + k], f, 2)
             runs.append((gi, f, X0))
     choice, ms, shapes = ens.tune(rtol=1e-6)
-    assert len(shapes) == len(ms) and all(len(s) == 4 for s in shapes)
+    assert len(shapes) == len(ms) and all(len(s) == 5 for s in shapes)
     timed = [t for t in ms if t > 0]
     assert len(timed) >= 4 and ms[choice] == min(timed)
     assert ens.config() == choice
@@ -158,6 +158,63 @@ def test_ensemble_matches_single_fp32_runs_and_tuner():
             assert np.abs(on1 - on2).max() < 1e-4
         assert abs(s1["steps"] - st[i]["steps"]) <= 0.03 * s1["steps"] + 3
         s.close()
+    ens.close()
+    for q in geoms:
+        q[0].close()
+
+
+def test_every_unit_shape_gives_the_same_run():
+    """Every candidate product shape, selected without tuning, on two
+    geometries x 32 members over 15 yr: shapes with 1024-column units sum
+    each operator row in the same order (FP32 over 16-column chunks from the
+    unit's start, flushed to FP64 every 256 columns, blocks added in order),
+    whichever rows a wave holds, how many chunks it loads at once, how deep
+    its load ring is and whether the unit's waves share the slip-rate slices
+    through LDS, so their
+    runs are bitwise the same; the other column-block widths change only the
+    FP64 partial sums (onsets within 1e-4 yr)."""
+    from interact_amd import faults, lib
+    from interact_amd.model import Ensemble
+    geoms = _geoms(2.0, lib.SLIP_VECTOR)
+    nm = 32
+    ens = Ensemble([q[0] for q in geoms], nm)
+    X0s = []
+    for gi, (fs, g, L, W) in enumerate(geoms):
+        for k in range(nm):
+            X0s.append(_configure(ens.members[gi * nm + k], faults.ensemble_fields(g, L, W, k, gi), 2))
+    shapes = ens.shapes()
+    assert any(s[4] == 0 for s in shapes) and any(s[4] >= 4 for s in shapes)
+    t_end = 15.0 * SEC_PER_YEAR
+    ref = None
+    ntried = 0
+    for cfg, sh in enumerate(shapes):
+        try:
+            ens.select(cfg)
+        except lib.RsfgError as err:
+            assert "does not fit" in str(err)   # register budget of 32 members
+            continue
+        ntried += 1
+        assert ens.config() == cfg
+        for m, X0 in zip(ens.members, X0s):
+            m.set_state(X0)
+        st = ens.solve(t_end, rtol=1e-6)
+        out = [(m.get_state()[0], m.events(), q["steps"]) for m, q in zip(ens.members, st)]
+        if ref is None:
+            assert sh[3] == 1024
+            ref = out
+            assert sum(len(_onsets(ev)) for _, ev, _ in out) >= 8
+            continue
+        for i, ((X, ev, n), (Xr, evr, nr)) in enumerate(zip(out, ref)):
+            if sh[3] == 1024:
+                assert n == nr and np.array_equal(X, Xr) and np.array_equal(ev, evr), (cfg, sh, i)
+            else:
+                on, onr = _onsets(ev), _onsets(evr)
+                assert len(on) == len(onr), (cfg, sh, i)
+                if len(on):
+                    assert np.abs(on - onr).max() < 1e-4, (cfg, sh, i)
+    assert ntried >= 8
+    with pytest.raises(lib.RsfgError, match="out of range"):
+        ens.select(len(shapes))
     ens.close()
     for q in geoms:
         q[0].close()
