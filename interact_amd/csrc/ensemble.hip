@@ -230,26 +230,32 @@ __device__ __forceinline__ void wave_product(const float* __restrict__ Kt, long 
   }
 }
 
-// The product of a whole unit with the member slices staged in LDS: the W
-// waves of the workgroup load each chunk group's slices once (NM members x
-// 16 U columns, double-buffered) and every wave reads them from there, so
-// the per-CU memory traffic is mostly the operator stream.  The operator
-// tiles stream as in wave_product (loads one group ahead); products and
-// summation order are those of wave_product.
-constexpr int kLdsRow = 20;   // floats per staged member row of a chunk (16 + 4 pad)
+// The product of a whole unit with the member slip rates shared through LDS:
+// the W waves of the workgroup stage each 256-column segment of all NM
+// members once (double-buffered, one barrier per segment) and read their
+// MFMA operands from there; the operator streams through each wave's ring
+// of D register sets as in wave_product, whose products and order of
+// summation this repeats.  The next segment's slip rates are loaded into
+// registers at the start of a segment and written to LDS at its end.
+constexpr int kSlabRow = 256 + 4;   // floats per member row of a staged segment (padded)
 
-template <int RF, int MF, int W, int U>
-__device__ __forceinline__ void unit_product_lds(const float* __restrict__ Kt, long ld, int rw,
-                                                 const float* __restrict__ V, int kc0, int kc1,
-                                                 float* sV, double (&acc)[RF][MF][4]) {
-  constexpr int NM = 16 * MF;
-  constexpr int NQ = NM * U * 4;                    // 16-byte pieces of one staged group
-  constexpr int NV = (NQ + 64 * W - 1) / (64 * W);  // pieces per thread
+template <int RF, int MF, int W, int U, int D>
+__device__ __forceinline__ void unit_product_slab(const float* __restrict__ Kt, long ld, int rw,
+                                                  const float* __restrict__ V, int kc0, int kc1,
+                                                  float* sV, double (&acc)[RF][MF][4]) {
+  constexpr int NM = 16 * MF, NG = 16 / U;
+  constexpr int NQ = NM * 64;         // 16-byte pieces of one segment's slip rates
+  constexpr int NV = NQ / (64 * W);   // pieces per thread
+  static_assert(NQ % (64 * W) == 0, "whole pieces per thread");
+  static_assert(16 % U == 0 && D >= 2 && NG % D == 0, "ring slots repeat every segment");
   const int tid = threadIdx.x, lane = tid & 63, i = lane & 15, j = lane >> 4;
   const long nk = ld / 16;
-  const float* ka[RF];
+  typedef const __attribute__((address_space(1))) v4f_t gv4;
+  const __attribute__((address_space(1))) float* ka[RF];
 #pragma unroll
-  for (int rf = 0; rf < RF; ++rf) ka[rf] = Kt + ((long)(rw / 16 + rf) * nk) * 256 + 4 * lane;
+  for (int rf = 0; rf < RF; ++rf)
+    ka[rf] = global_ptr(Kt + ((long)(rw / 16 + rf) * nk) * 256 + 4 * lane);
+  const __attribute__((address_space(1))) float* vg = global_ptr(V);
 #pragma unroll
   for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
@@ -257,90 +263,77 @@ __device__ __forceinline__ void unit_product_lds(const float* __restrict__ Kt, l
 #pragma unroll
       for (int r = 0; r < 4; ++r) acc[rf][mf][r] = 0.0;
   if (kc0 >= kc1) return;
-  // piece x = tid + 64 W t: member x / (4 U), chunk (x / 4) % U, quarter x % 4;
-  // staged at sV + ((buffer U + chunk) NM + member) kLdsRow + 4 quarter
+  // piece x = tid + 64 W t: member x / 64, columns 4 (x % 64) .. + 3 of the segment
   v4f_t rv[NV];
-  auto vload = [&](int kc) {
+  auto vload = [&](int seg) {
 #pragma unroll
     for (int t = 0; t < NV; ++t) {
       const int x = tid + 64 * W * t;
-      if (NQ % (64 * W) == 0 || x < NQ)
-        rv[t] = *reinterpret_cast<const v4f_t*>(V + (size_t)(x / (4 * U)) * ld +
-                                                (size_t)kc * 16 + 4 * (x % (4 * U)));
+      rv[t] = *reinterpret_cast<gv4*>(vg + (size_t)(x / 64) * ld + (size_t)seg * 16 + 4 * (x % 64));
     }
   };
-  auto vstore = [&](int b) {
+  auto vstore = [&](int buf) {
 #pragma unroll
     for (int t = 0; t < NV; ++t) {
       const int x = tid + 64 * W * t;
-      if (NQ % (64 * W) == 0 || x < NQ) {
-        const int m = x / (4 * U), u = (x / 4) % U, q = x % 4;
-        *reinterpret_cast<v4f_t*>(sV + ((b * U + u) * NM + m) * kLdsRow + 4 * q) = rv[t];
-      }
+      *reinterpret_cast<v4f_t*>(sV + (buf * NM + x / 64) * kSlabRow + 4 * (x % 64)) = rv[t];
     }
   };
-  v4f_t A[U][RF];
+  v4f_t A[D][U][RF];
   auto kload = [&](int kc, v4f_t (&a)[U][RF]) {
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int rf = 0; rf < RF; ++rf)
         a[u][rf] = __builtin_nontemporal_load(
-            reinterpret_cast<const v4f_t*>(ka[rf] + (size_t)(kc + u) * 256));
+            reinterpret_cast<gv4*>(ka[rf] + (size_t)(kc + u) * 256));
   };
-  v4f_t c[RF][MF];
-#pragma unroll
-  for (int rf = 0; rf < RF; ++rf)
-#pragma unroll
-    for (int mf = 0; mf < MF; ++mf) c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
-  kload(kc0, A);
   vload(kc0);
   vstore(0);
   __syncthreads();
-  int b = 0;
-  for (int kc = kc0; kc < kc1; kc += U) {
-    const bool more = kc + U < kc1;
-    v4f_t An[U][RF];
-    if (more) {
-      kload(kc + U, An);
-      vload(kc + U);
+#pragma unroll
+  for (int d = 0; d < D - 1; ++d) kload(kc0 + d * U, A[d]);
+  int buf = 0;
+  for (int seg = kc0; seg < kc1; seg += 16) {
+    // the last segment reloads itself (loads stay unconditional); its copy
+    // goes to the buffer no wave reads any more
+    vload(seg + 16 < kc1 ? seg + 16 : seg);
+    v4f_t c[RF][MF];
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf) c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      kload(min(seg + (g + D - 1) * U, kc1 - U), A[(g + D - 1) % D]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        v4f_t B[MF];
+#pragma unroll
+        for (int mf = 0; mf < MF; ++mf)
+          B[mf] = *reinterpret_cast<const v4f_t*>(sV + (buf * NM + 16 * mf + i) * kSlabRow +
+                                                  16 * (g * U + u) + 4 * j);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+          for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+            for (int mf = 0; mf < MF; ++mf)
+              c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[g % D][u][rf][q], B[mf][q],
+                                                                c[rf][mf], 0, 0, 0);
+      }
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      v4f_t B[MF];
+    for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
       for (int mf = 0; mf < MF; ++mf)
-        B[mf] = *reinterpret_cast<const v4f_t*>(sV + ((b * U + u) * NM + 16 * mf + i) * kLdsRow +
-                                                4 * j);
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int rf = 0; rf < RF; ++rf)
-#pragma unroll
-          for (int mf = 0; mf < MF; ++mf)
-            c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[u][rf][q], B[mf][q], c[rf][mf],
-                                                              0, 0, 0);
-    }
-    if (((kc + U - kc0) & 15) == 0 || !more) {
-#pragma unroll
-      for (int rf = 0; rf < RF; ++rf)
-#pragma unroll
-        for (int mf = 0; mf < MF; ++mf) {
-#pragma unroll
-          for (int r = 0; r < 4; ++r) acc[rf][mf][r] += (double)c[rf][mf][r];
-          c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
-        }
-    }
+        for (int r = 0; r < 4; ++r) acc[rf][mf][r] += (double)c[rf][mf][r];
     // the other buffer was last read before the previous barrier
-    if (more) vstore(b ^ 1);
+    vstore(buf ^ 1);
     __syncthreads();
-    if (more) {
-#pragma unroll
-      for (int u = 0; u < U; ++u)
-#pragma unroll
-        for (int rf = 0; rf < RF; ++rf) A[u][rf] = An[u][rf];
-    }
-    b ^= 1;
+    buf ^= 1;
   }
 }
 
@@ -447,11 +440,11 @@ __device__ __forceinline__ bool geometry_running(const EnsArgs& e, const EnsGeo&
 // One product unit per workgroup: rows [r0, r0 + 16 RF W) of one geometry
 // (16 RF per wave) over the 16-column chunks [kc0, kc1), for all 16 MF
 // members, into that column block's partials.
-// D >= 2: each wave streams through a ring of D register sets; D = 0: the
-// unit's waves share the member slices through LDS.
-template <int RF, int MF, int W, int U, int D>
+// Each wave streams the operator through a ring of D register sets; S: the
+// unit's waves share the member slip rates through LDS.
+template <int RF, int MF, int W, int U, int D, int S>
 __global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
-  __shared__ float sV[(D == 0) ? 2 * U * 16 * MF * kLdsRow : 1];
+  __shared__ float sV[S ? 2 * 16 * MF * kSlabRow : 1];
   constexpr int NM = 16 * MF;
   const int4 u = e.units[blockIdx.x];
   const EnsGeo& gg = e.geo[u.x];
@@ -459,8 +452,8 @@ __global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int rw = u.y + wave * 16 * RF;
   double acc[RF][MF][4];
-  if constexpr (D == 0)
-    unit_product_lds<RF, MF, W, U>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], u.z, u.w, sV, acc);
+  if constexpr (S)
+    unit_product_slab<RF, MF, W, U, D>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], u.z, u.w, sV, acc);
   else
     wave_product<RF, MF, U, D>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], u.z, u.w, acc);
   double* P = gg.P + (size_t)(u.z / e.cb_chunks) * gg.rows_pad * NM;
@@ -539,15 +532,15 @@ __global__ __launch_bounds__(64 * (kEpiRows / G) * MF / 4) void k_epi_b(EnsArgs 
 
 // ------------------------------------------------------------ host side
 struct EnsCfg {
-  int rf, w, u, cb, d;  // 16 RF rows per wave, W waves per unit, U chunks per load group,
-                        // CB columns per unit, D register sets in each wave's load ring
-                        // (0: the unit's waves share the member slices through LDS)
+  int rf, w, u, cb, d, s;  // 16 RF rows per wave, W waves per unit, U chunks per load
+                           // group, CB columns per unit, D register sets in each wave's
+                           // load ring, S: slip rates shared through LDS
 };
 static const EnsCfg kEnsCfgs[] = {
-    {4, 1, 1, 1024, 2}, {4, 2, 1, 1024, 2}, {4, 1, 1, 512, 2},  {2, 1, 1, 1024, 2},
-    {2, 2, 1, 1024, 2}, {4, 1, 2, 1024, 2}, {8, 1, 1, 1024, 2}, {4, 1, 1, 2048, 2},
-    {2, 4, 1, 1024, 2}, {4, 1, 1, 1024, 4}, {2, 2, 1, 1024, 8}, {2, 8, 1, 1024, 0},
-    {4, 4, 2, 1024, 0}};
+    {4, 1, 1, 1024, 2, 0}, {4, 2, 1, 1024, 2, 0}, {4, 1, 1, 512, 2, 0}, {2, 1, 1, 1024, 2, 0},
+    {2, 2, 1, 1024, 2, 0}, {4, 1, 2, 1024, 2, 0}, {8, 1, 1, 1024, 2, 0}, {4, 1, 1, 2048, 2, 0},
+    {4, 1, 1, 1024, 4, 0}, {4, 4, 1, 1024, 2, 1}, {4, 2, 1, 1024, 2, 1}, {2, 4, 1, 1024, 2, 1},
+    {4, 4, 1, 1024, 4, 1}, {2, 8, 1, 1024, 2, 1}};
 constexpr int kEnsNCfg = sizeof(kEnsCfgs) / sizeof(kEnsCfgs[0]);
 
 static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].w; }
@@ -555,9 +548,9 @@ static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].w; }
 typedef void (*EnsProdFn)(const EnsArgs&, int nblocks, hipStream_t);
 typedef void (*EnsEpiFn)(const EnsArgs&, int nblocks, hipStream_t);
 
-template <int RF, int MF, int W, int U, int D>
+template <int RF, int MF, int W, int U, int D, int S = 0>
 static void launch_prod_b(const EnsArgs& e, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_prod_b<RF, MF, W, U, D>), dim3(nblocks), dim3(64 * W), 0, st, e);
+  hipLaunchKernelGGL((k_prod_b<RF, MF, W, U, D, S>), dim3(nblocks), dim3(64 * W), 0, st, e);
 }
 
 template <int NC, int G, int MF>
@@ -579,11 +572,12 @@ static EnsProdFn prod_fn_mf(int cfg) {
     case 5: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 2, 2>; else return nullptr;
     case 6: if constexpr (MF <= 1) return launch_prod_b<8, MF, 1, 1, 2>; else return nullptr;
     case 7: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 2>; else return nullptr;
-    case 8: return launch_prod_b<2, MF, 4, 1, 2>;
-    case 9: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 4>; else return nullptr;
-    case 10: if constexpr (MF <= 2) return launch_prod_b<2, MF, 2, 1, 8>; else return nullptr;
-    case 11: return launch_prod_b<2, MF, 8, 1, 0>;
-    default: if constexpr (MF <= 2) return launch_prod_b<4, MF, 4, 2, 0>; else return nullptr;
+    case 8: if constexpr (MF <= 2) return launch_prod_b<4, MF, 1, 1, 4>; else return nullptr;
+    case 9: if constexpr (MF <= 2) return launch_prod_b<4, MF, 4, 1, 2, 1>; else return nullptr;
+    case 10: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 2, 1>; else return nullptr;
+    case 11: if constexpr (MF <= 2) return launch_prod_b<2, MF, 4, 1, 2, 1>; else return nullptr;
+    case 12: if constexpr (MF <= 2) return launch_prod_b<4, MF, 4, 1, 4, 1>; else return nullptr;
+    default: return launch_prod_b<2, MF, 8, 1, 2, 1>;
   }
 }
 
@@ -1049,11 +1043,12 @@ int rsfg_ensemble_config(rsfg_ensemble* e, int* cfg, int* ncfg, int* shape) {
   if (ncfg) *ncfg = kEnsNCfg;
   if (shape) {
     for (int c = 0; c < kEnsNCfg; ++c) {
-      shape[5 * c] = 16 * kEnsCfgs[c].rf;
-      shape[5 * c + 1] = kEnsCfgs[c].w;
-      shape[5 * c + 2] = kEnsCfgs[c].u;
-      shape[5 * c + 3] = kEnsCfgs[c].cb;
-      shape[5 * c + 4] = kEnsCfgs[c].d;
+      shape[6 * c] = 16 * kEnsCfgs[c].rf;
+      shape[6 * c + 1] = kEnsCfgs[c].w;
+      shape[6 * c + 2] = kEnsCfgs[c].u;
+      shape[6 * c + 3] = kEnsCfgs[c].cb;
+      shape[6 * c + 4] = kEnsCfgs[c].d;
+      shape[6 * c + 5] = kEnsCfgs[c].s;
     }
   }
   return RSFG_OK;

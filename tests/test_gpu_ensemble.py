@@ -137,7 +137,7 @@ def test_ensemble_matches_single_fp32_runs_and_tuner():
             X0 = _configure(ens.members[gi * nm + k], f, 2)
             runs.append((gi, f, X0))
     choice, ms, shapes = ens.tune(rtol=1e-6)
-    assert len(shapes) == len(ms) and all(len(s) == 5 for s in shapes)
+    assert len(shapes) == len(ms) and all(len(s) == 6 for s in shapes)
     timed = [t for t in ms if t > 0]
     assert len(timed) >= 4 and ms[choice] == min(timed)
     assert ens.config() == choice
@@ -183,7 +183,7 @@ def test_every_unit_shape_gives_the_same_run():
         for k in range(nm):
             X0s.append(_configure(ens.members[gi * nm + k], faults.ensemble_fields(g, L, W, k, gi), 2))
     shapes = ens.shapes()
-    assert any(s[4] == 0 for s in shapes) and any(s[4] >= 4 for s in shapes)
+    assert any(s[5] == 1 for s in shapes) and any(s[4] >= 4 for s in shapes)
     t_end = 15.0 * SEC_PER_YEAR
     ref = None
     ntried = 0
