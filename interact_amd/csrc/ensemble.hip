@@ -42,13 +42,14 @@ struct EnsGeo {        // one geometry of the ensemble
                        // and leading dimension of V
   int rows, n;         // G n operator rows, n patches
   int m0, nm;          // first member and member count
-  int tile0, ntiles;   // epilogue tiles (16 rows) of this geometry in the tile list
+  int tile0, ntiles;   // epilogue tiles (16 patches) of this geometry in the tile list
   float* V[2];         // product inputs by stage parity, nm x ld (member-major)
   double* P;           // product partials [column block][rows_pad][nm]
   int rows_pad;        // rows rounded up to a whole number of product row groups
 };
 
-constexpr int kEpiRows = 16;      // operator rows per epilogue tile
+constexpr int kEpiPatches = 16;   // patches per epilogue tile (16 G operator rows)
+constexpr int kEpiWaves = 4;      // epilogue waves per SIMD the register budget allows (no spills)
 constexpr int kRowPad = 256;      // rows_pad granularity (largest product row group)
 constexpr int kColPad = 256;      // ld granularity (the product's FP64 flush segment)
 
@@ -468,23 +469,23 @@ __global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
       }
 }
 
-// Stage epilogue: one workgroup per 16 operator rows (16 / G patches) of one
-// geometry, one thread per (patch, member).  The stressing rates of a row
+// Stage epilogue: one workgroup per 16 patches (16 G operator rows) of one
+// geometry and 16 of its members, one thread per (patch, member): each
+// member's fields are read as 128-byte runs of 16 consecutive patches.  The stressing rates of a row
 // are its partials summed over the column blocks in order; then the
 // single-fault stage update, and each member's reductions over the
 // consecutive lanes holding its patches.
 template <int NC, int G, int MF>
-__global__ __launch_bounds__(64 * (kEpiRows / G) * MF / 4) void k_epi_b(EnsArgs e) {
-  constexpr int NPT = kEpiRows / G;   // patches per tile
+__global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e) {
+  constexpr int NPT = kEpiPatches;
   constexpr int NM = 16 * MF;
-  static_assert(NPT == 8 || NPT == 16, "tile patches");
-  __shared__ MemberSnap snap[NM];
+  __shared__ MemberSnap snap[16];
   const int2 tile = e.tiles[blockIdx.x];
   const EnsGeo& gg = e.geo[tile.x];
   const int r0 = tile.y, s = e.s;
-  const int tid = threadIdx.x;
-  for (int m = tid; m < NM; m += blockDim.x) {
-    const DevCtl* c = e.args[gg.m0 + m].ctl;
+  const int tid = threadIdx.x, mb = 16 * blockIdx.y;   // this block's 16 members
+  for (int m = tid; m < 16; m += blockDim.x) {
+    const DevCtl* c = e.args[gg.m0 + mb + m].ctl;
     MemberSnap q;
     q.halted = c->halted;
     q.xc = c->xcur;
@@ -497,12 +498,12 @@ __global__ __launch_bounds__(64 * (kEpiRows / G) * MF / 4) void k_epi_b(EnsArgs 
   const bool reduce = (s == 0) || (s == S - 1);
   const int ncb = (int)((gg.ld / 16 + e.cb_chunks - 1) / e.cb_chunks);
   const size_t cbs = (size_t)gg.rows_pad * NM;
-  const int m = tid / NPT, pl = tid % NPT, p = r0 / G + pl;
+  const int ml = tid / NPT, m = mb + ml, pl = tid % NPT, p = r0 / G + pl;
   double r9[kNRed];
   r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
   r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
   r9[RED_DX2] = 0.0;
-  if (p < gg.n && !snap[m].halted) {
+  if (p < gg.n && !snap[ml].halted) {
     double td[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) {
@@ -511,7 +512,7 @@ __global__ __launch_bounds__(64 * (kEpiRows / G) * MF / 4) void k_epi_b(EnsArgs 
       for (int cb = 0; cb < ncb; ++cb) v += q[cb * cbs];
       td[k] = v;
     }
-    member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[m], s, p, td,
+    member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[ml], s, p, td,
                               gg.V[(s + 1) & 1] + (size_t)m * gg.ld, r9);
   }
   if (reduce) {
@@ -555,8 +556,7 @@ static void launch_prod_b(const EnsArgs& e, int nblocks, hipStream_t st) {
 
 template <int NC, int G, int MF>
 static void launch_epi_b(const EnsArgs& e, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_epi_b<NC, G, MF>), dim3(nblocks), dim3((kEpiRows / G) * 16 * MF), 0,
-                     st, e);
+  hipLaunchKernelGGL((k_epi_b<NC, G, MF>), dim3(nblocks, MF), dim3(kEpiPatches * 16), 0, st, e);
 }
 
 // shapes with four or more 16-row blocks per wave only with up to two
@@ -627,8 +627,8 @@ struct rsfg_ensemble {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
-// Work lists for configuration cfg.  Epilogue: per geometry ceil(rows /
-// 16) tiles of 16 rows.  Product: per geometry and row group of cfg_rows
+// Work lists for configuration cfg.  Epilogue: per geometry ceil(n / 16)
+// tiles of 16 patches.  Product: per geometry and row group of cfg_rows
 // rows, one unit per block of CB columns; geometries with the longest rows
 // first.
 static int build_plan(rsfg_ensemble* e, int cfg) {
@@ -642,8 +642,8 @@ static int build_plan(rsfg_ensemble* e, int cfg) {
   for (int g : order) {
     EnsGeo& q = e->geo[g];
     q.tile0 = (int)tiles.size();
-    q.ntiles = (q.rows + kEpiRows - 1) / kEpiRows;
-    for (int t = 0; t < q.ntiles; ++t) tiles.push_back(make_int2(g, t * kEpiRows));
+    q.ntiles = (q.n + kEpiPatches - 1) / kEpiPatches;
+    for (int t = 0; t < q.ntiles; ++t) tiles.push_back(make_int2(g, t * kEpiPatches * e->G));
     const int nk = (int)(q.ld / 16);
     for (int r = 0; r < q.rows; r += TR)
       for (int k = 0; k < nk; k += CBK) units.push_back(make_int4(g, r, k, std::min(nk, k + CBK)));
