@@ -492,8 +492,9 @@ def ensemble_bench(args):
 
 def ensemble_kernels_only(args):
     """c4 kernels for counter and trace runs: the ensemble's setup (no
-    solve) and the tuner, which launches the batched stage kernel 21 times
-    per tile shape."""
+    solve), the tuner, which launches a middle stage (product + epilogue)
+    21 times per unit shape, and 20 more launches of each for the tuned
+    shape timed by HIP events."""
     from interact_amd import faults, lib
     from interact_amd.model import Ensemble, FaultSystem, initial_state
     geoms, gdat = [], []
@@ -514,6 +515,9 @@ def ensemble_kernels_only(args):
     choice, ms, shapes = ens.tune(rtol=args.rtol)
     log("kernels-only c4: " + ", ".join(f"{shapes[i]} {ms[i]*1e3:.1f} us" for i in range(len(ms))
                                          if ms[i] > 0))
+    pm, pe = ens.bench(20, rtol=args.rtol)
+    log(f"kernels-only c4: tuned shape {shapes[choice]}: product {pm*1e3:.1f} us, "
+        f"epilogue {pe*1e3:.1f} us by HIP events (20 launches each)")
     ens.close()
     for g in geoms:
         g.close()
