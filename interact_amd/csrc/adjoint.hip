@@ -37,7 +37,7 @@
 
 namespace rsfg {
 
-constexpr int kRpWaves = 4;    // patches (waves) per workgroup of the replay stage kernel
+constexpr int kRpWaves = 8;    // patches (waves) per workgroup of the replay stage kernel
 constexpr int kAdjThreads = 256;
 
 struct RpArgs {
