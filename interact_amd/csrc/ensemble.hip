@@ -15,9 +15,11 @@
 // epilogue adds a row's partials over its column blocks in a fixed order.
 // One attempt of rk5dp for the whole ensemble is k_prep_b, six times
 // (k_prod_b, k_epi_b), and k_ctrl_b; the host captures batches of attempts
-// in a hipGraph.  Unit shapes (rows per wave, waves per unit, chunks in
-// flight, columns per block) are chosen at run time by timing the candidates
-// on the ensemble's own stage (rsfg_ensemble_tune).
+// in a hipGraph.  Unit shapes (rows per wave, waves per unit, chunks per
+// load group, columns per block, depth of each wave's load ring, slip rates
+// shared through LDS or not) are chosen at run time by timing the
+// candidates on the ensemble's own stage (rsfg_ensemble_tune), or pinned
+// with rsfg_ensemble_select.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -471,10 +473,10 @@ __global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
 
 // Stage epilogue: one workgroup per 16 patches (16 G operator rows) of one
 // geometry and 16 of its members, one thread per (patch, member): each
-// member's fields are read as 128-byte runs of 16 consecutive patches.  The stressing rates of a row
-// are its partials summed over the column blocks in order; then the
-// single-fault stage update, and each member's reductions over the
-// consecutive lanes holding its patches.
+// member's fields are read as 128-byte runs of 16 consecutive patches.  The
+// stressing rates of a row are its partials summed over the column blocks
+// in order; then the single-fault stage update, and each member's
+// reductions over the consecutive lanes holding its patches.
 template <int NC, int G, int MF>
 __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e) {
   constexpr int NPT = kEpiPatches;
