@@ -289,11 +289,12 @@ int rsfg_ensemble_solve(rsfg_ensemble *e, const rsfg_solver_opts *o, double t_en
    ensemble (members must be configured), keeps the fastest; ms (may be
    NULL) receives one time per candidate, -1 where a shape does not fit */
 int rsfg_ensemble_tune(rsfg_ensemble *e, const rsfg_solver_opts *o, int *choice, double *ms);
-/* current unit shape, number of candidates, and per candidate six ints
+/* current unit shape, number of candidates, and per candidate seven ints
    (operator rows per wave, waves per product unit, 16-column chunks per
    load group, columns per product unit, load groups each wave keeps in its
    register ring, 1 where the unit's waves share the member slip rates
-   through LDS) */
+   through LDS, and for stream-K shapes the workgroups per CU, each taking
+   an equal share of the product, with 0 columns per unit; 0 otherwise) */
 int rsfg_ensemble_config(rsfg_ensemble *e, int *cfg, int *ncfg, int *shape);
 /* use candidate shape cfg (as listed by rsfg_ensemble_config) without
    tuning, e.g. a shape tuned earlier on the same layout; RSFG_EINVAL when

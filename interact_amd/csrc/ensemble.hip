@@ -48,6 +48,7 @@ struct EnsGeo {        // one geometry of the ensemble
   float* V[2];         // product inputs by stage parity, nm x ld (member-major)
   double* P;           // product partials [column block][rows_pad][nm]
   int rows_pad;        // rows rounded up to a whole number of product row groups
+  int rg0;             // stream-K plans: this geometry's first row group
 };
 
 constexpr int kEpiPatches = 16;   // patches per epilogue tile (16 G operator rows)
@@ -63,6 +64,15 @@ struct EnsArgs {
   int ngeo;
   int cb_chunks;          // 16-column chunks per column block
   int s;                  // stage
+  // stream-K plans (sk_tr > 0): pieces (geometry, first row, first chunk,
+  // end chunk) of sk_tr-row groups, workgroup w taking pieces
+  // [wpc[w], wpc[w + 1]); per row group (first piece, pieces), in order;
+  // piece k's partials at Psk + k sk_tr nm
+  const int4* pieces;
+  const int* wpc;
+  const int2* rg;
+  double* Psk;
+  int sk_tr;
 };
 
 // members of a workgroup's geometry: controller fields read once
@@ -471,6 +481,39 @@ __global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
       }
 }
 
+// Stream-K product: a grid of a fixed number of workgroups per CU, each
+// taking an equal, contiguous share of all geometries' (row group, 256-column
+// segment) work, so that every SIMD gets the same number of MFMAs whatever
+// the unit count.  A share is cut into pieces at row-group boundaries; each
+// piece runs as a unit (slip rates shared through LDS) and writes its own
+// FP64 partials, which the epilogue adds over the row group's pieces in
+// order.
+template <int RF, int MF, int W, int U, int D>
+__global__ __launch_bounds__(64 * W) void k_prod_sk(EnsArgs e) {
+  __shared__ float sV[2 * 16 * MF * kSlabRow];
+  constexpr int NM = 16 * MF, TR = 16 * RF * W;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k1 = e.wpc[blockIdx.x + 1];
+  for (int k = e.wpc[blockIdx.x]; k < k1; ++k) {
+    const int4 pc = e.pieces[k];
+    const EnsGeo& gg = e.geo[pc.x];
+    if (!geometry_running(e, gg)) continue;
+    const int rw = pc.y + wave * 16 * RF;
+    double acc[RF][MF][4];
+    unit_product_slab<RF, MF, W, U, D>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], pc.z, pc.w, sV, acc);
+    double* P = e.Psk + (size_t)k * TR * NM;
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wave * 16 * RF + 16 * rf + 4 * (lane >> 4) + r;
+          P[(size_t)row * NM + 16 * mf + (lane & 15)] = acc[rf][mf][r];
+        }
+  }
+}
+
 // Stage epilogue: one workgroup per 16 patches (16 G operator rows) of one
 // geometry and 16 of its members, one thread per (patch, member): each
 // member's fields are read as 128-byte runs of 16 consecutive patches.  The
@@ -509,9 +552,17 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
     double td[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) {
-      const double* q = gg.P + (size_t)(p * G + k) * NM + m;
       double v = 0.0;
-      for (int cb = 0; cb < ncb; ++cb) v += q[cb * cbs];
+      if (e.sk_tr > 0) {
+        const int row = p * G + k;
+        const int2 rg = e.rg[gg.rg0 + row / e.sk_tr];
+        const size_t ps = (size_t)e.sk_tr * NM;
+        const double* q = e.Psk + (size_t)rg.x * ps + (size_t)(row % e.sk_tr) * NM + m;
+        for (int c = 0; c < rg.y; ++c) v += q[c * ps];
+      } else {
+        const double* q = gg.P + (size_t)(p * G + k) * NM + m;
+        for (int cb = 0; cb < ncb; ++cb) v += q[cb * cbs];
+      }
       td[k] = v;
     }
     member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[ml], s, p, td,
@@ -535,15 +586,18 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
 
 // ------------------------------------------------------------ host side
 struct EnsCfg {
-  int rf, w, u, cb, d, s;  // 16 RF rows per wave, W waves per unit, U chunks per load
-                           // group, CB columns per unit, D register sets in each wave's
-                           // load ring, S: slip rates shared through LDS
+  int rf, w, u, cb, d, s, sk;  // 16 RF rows per wave, W waves per unit, U chunks per load
+                               // group, CB columns per unit, D register sets in each
+                               // wave's load ring, S: slip rates shared through LDS,
+                               // SK > 0: stream-K with SK workgroups per CU (CB unused)
 };
 static const EnsCfg kEnsCfgs[] = {
-    {4, 1, 1, 1024, 2, 0}, {4, 2, 1, 1024, 2, 0}, {4, 1, 1, 512, 2, 0}, {2, 1, 1, 1024, 2, 0},
-    {2, 2, 1, 1024, 2, 0}, {4, 1, 2, 1024, 2, 0}, {8, 1, 1, 1024, 2, 0}, {4, 1, 1, 2048, 2, 0},
-    {4, 1, 1, 1024, 4, 0}, {4, 4, 1, 1024, 2, 1}, {4, 2, 1, 1024, 2, 1}, {2, 4, 1, 1024, 2, 1},
-    {4, 4, 1, 1024, 4, 1}, {2, 8, 1, 1024, 2, 1}};
+    {4, 1, 1, 1024, 2, 0, 0}, {4, 2, 1, 1024, 2, 0, 0}, {4, 1, 1, 512, 2, 0, 0},
+    {2, 1, 1, 1024, 2, 0, 0}, {2, 2, 1, 1024, 2, 0, 0}, {4, 1, 2, 1024, 2, 0, 0},
+    {8, 1, 1, 1024, 2, 0, 0}, {4, 1, 1, 2048, 2, 0, 0}, {4, 1, 1, 1024, 4, 0, 0},
+    {4, 4, 1, 1024, 2, 1, 0}, {4, 2, 1, 1024, 2, 1, 0}, {2, 4, 1, 1024, 2, 1, 0},
+    {4, 4, 1, 1024, 4, 1, 0}, {2, 8, 1, 1024, 2, 1, 0}, {4, 4, 1, 0, 4, 1, 1},
+    {4, 4, 1, 0, 2, 1, 2},    {4, 4, 1, 0, 4, 1, 2},    {2, 8, 1, 0, 4, 1, 1}};
 constexpr int kEnsNCfg = sizeof(kEnsCfgs) / sizeof(kEnsCfgs[0]);
 
 static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].w; }
@@ -554,6 +608,11 @@ typedef void (*EnsEpiFn)(const EnsArgs&, int nblocks, hipStream_t);
 template <int RF, int MF, int W, int U, int D, int S = 0>
 static void launch_prod_b(const EnsArgs& e, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL((k_prod_b<RF, MF, W, U, D, S>), dim3(nblocks), dim3(64 * W), 0, st, e);
+}
+
+template <int RF, int MF, int W, int U, int D>
+static void launch_prod_sk(const EnsArgs& e, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_prod_sk<RF, MF, W, U, D>), dim3(nblocks), dim3(64 * W), 0, st, e);
 }
 
 template <int NC, int G, int MF>
@@ -579,7 +638,11 @@ static EnsProdFn prod_fn_mf(int cfg) {
     case 10: if constexpr (MF <= 2) return launch_prod_b<4, MF, 2, 1, 2, 1>; else return nullptr;
     case 11: if constexpr (MF <= 2) return launch_prod_b<2, MF, 4, 1, 2, 1>; else return nullptr;
     case 12: if constexpr (MF <= 2) return launch_prod_b<4, MF, 4, 1, 4, 1>; else return nullptr;
-    default: return launch_prod_b<2, MF, 8, 1, 2, 1>;
+    case 13: return launch_prod_b<2, MF, 8, 1, 2, 1>;
+    case 14: if constexpr (MF <= 2) return launch_prod_sk<4, MF, 4, 1, 4>; else return nullptr;
+    case 15: if constexpr (MF <= 2) return launch_prod_sk<4, MF, 4, 1, 2>; else return nullptr;
+    case 16: if constexpr (MF <= 2) return launch_prod_sk<4, MF, 4, 1, 4>; else return nullptr;
+    default: return launch_prod_sk<2, MF, 8, 1, 4>;
   }
 }
 
@@ -620,6 +683,13 @@ struct rsfg_ensemble {
   int ntiles = 0;
   int4* d_units = nullptr;
   int nunits = 0;
+  int nprod = 0;             // product workgroups of the current plan
+  int4* d_pieces = nullptr;  // stream-K plan
+  int* d_wpc = nullptr;
+  int2* d_rg = nullptr;
+  double* d_Psk = nullptr;
+  size_t psk_cap = 0;        // doubles allocated at d_Psk
+  int ncu = 0;
   int cfg = 0;
   float* d_v = nullptr;
   double* d_P = nullptr;
@@ -629,12 +699,78 @@ struct rsfg_ensemble {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
+// Stream-K plan for row groups of TR rows and nwg workgroups: the (row
+// group, segment) items of all geometries in order, item range
+// [w T / nwg, (w + 1) T / nwg) to workgroup w, cut at row-group boundaries.
+static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR, int nwg) {
+  struct RowGroup {
+    int g, r, nseg;
+  };
+  std::vector<RowGroup> rgs;
+  for (int g : order) {
+    EnsGeo& q = e->geo[g];
+    q.rg0 = (int)rgs.size();
+    for (int r = 0; r < q.rows; r += TR) rgs.push_back({g, r, (int)(q.ld / 256)});
+  }
+  long long T = 0;
+  for (const auto& x : rgs) T += x.nseg;
+  std::vector<int4> pieces;
+  std::vector<int> wpc(nwg + 1);
+  std::vector<int2> rgtab(rgs.size(), make_int2(0, 0));
+  long long item = 0;
+  size_t ri = 0;
+  int seg = 0;
+  for (int w = 0; w < nwg; ++w) {
+    wpc[w] = (int)pieces.size();
+    const long long end = (long long)(w + 1) * T / nwg;
+    while (item < end) {
+      const RowGroup& x = rgs[ri];
+      const int take = (int)std::min<long long>(end - item, x.nseg - seg);
+      if (rgtab[ri].y == 0) rgtab[ri].x = (int)pieces.size();
+      rgtab[ri].y++;
+      pieces.push_back(make_int4(x.g, x.r, 16 * seg, 16 * (seg + take)));
+      item += take;
+      seg += take;
+      if (seg == x.nseg) {
+        ++ri;
+        seg = 0;
+      }
+    }
+  }
+  wpc[nwg] = (int)pieces.size();
+  RSFG_CHECK(item == T && ri == rgs.size(), RSFG_EINVAL, "rsfg_ensemble: stream-K plan");
+  const size_t need = pieces.size() * (size_t)TR * e->nm;
+  hipFree(e->d_pieces);
+  hipFree(e->d_wpc);
+  hipFree(e->d_rg);
+  e->d_pieces = nullptr;
+  e->d_wpc = nullptr;
+  e->d_rg = nullptr;
+  if (need > e->psk_cap) {
+    hipFree(e->d_Psk);
+    e->d_Psk = nullptr;
+    e->psk_cap = 0;
+    RSFG_HIP(hipMalloc(&e->d_Psk, sizeof(double) * need));
+    e->psk_cap = need;
+  }
+  RSFG_HIP(hipMalloc(&e->d_pieces, sizeof(int4) * pieces.size()));
+  RSFG_HIP(hipMemcpy(e->d_pieces, pieces.data(), sizeof(int4) * pieces.size(),
+                     hipMemcpyHostToDevice));
+  RSFG_HIP(hipMalloc(&e->d_wpc, sizeof(int) * wpc.size()));
+  RSFG_HIP(hipMemcpy(e->d_wpc, wpc.data(), sizeof(int) * wpc.size(), hipMemcpyHostToDevice));
+  RSFG_HIP(hipMalloc(&e->d_rg, sizeof(int2) * rgtab.size()));
+  RSFG_HIP(hipMemcpy(e->d_rg, rgtab.data(), sizeof(int2) * rgtab.size(), hipMemcpyHostToDevice));
+  e->nprod = nwg;
+  return RSFG_OK;
+}
+
 // Work lists for configuration cfg.  Epilogue: per geometry ceil(n / 16)
 // tiles of 16 patches.  Product: per geometry and row group of cfg_rows
 // rows, one unit per block of CB columns; geometries with the longest rows
-// first.
+// first (or, for stream-K shapes, the plan of build_sk_plan).
 static int build_plan(rsfg_ensemble* e, int cfg) {
-  const int TR = cfg_rows(cfg), CBK = kEnsCfgs[cfg].cb / 16;
+  const int TR = cfg_rows(cfg), SK = kEnsCfgs[cfg].sk;
+  const int CBK = SK ? 1 : kEnsCfgs[cfg].cb / 16;
   std::vector<int> order(e->ngeo);
   for (int g = 0; g < e->ngeo; ++g) order[g] = g;
   std::sort(order.begin(), order.end(),
@@ -647,8 +783,14 @@ static int build_plan(rsfg_ensemble* e, int cfg) {
     q.ntiles = (q.n + kEpiPatches - 1) / kEpiPatches;
     for (int t = 0; t < q.ntiles; ++t) tiles.push_back(make_int2(g, t * kEpiPatches * e->G));
     const int nk = (int)(q.ld / 16);
+    if (SK) continue;
     for (int r = 0; r < q.rows; r += TR)
       for (int k = 0; k < nk; k += CBK) units.push_back(make_int4(g, r, k, std::min(nk, k + CBK)));
+  }
+  if (SK) {
+    const int rc = build_sk_plan(e, order, TR, SK * e->ncu);
+    if (rc) return rc;
+    units.push_back(make_int4(0, 0, 0, 0));   // keeps the unit list allocated
   }
   hipFree(e->d_tiles);
   hipFree(e->d_units);
@@ -661,6 +803,7 @@ static int build_plan(rsfg_ensemble* e, int cfg) {
   RSFG_HIP(hipMemcpy(e->d_geo, e->geo.data(), sizeof(EnsGeo) * e->ngeo, hipMemcpyHostToDevice));
   e->ntiles = (int)tiles.size();
   e->nunits = (int)units.size();
+  if (!SK) e->nprod = e->nunits;
   e->cfg = cfg;
   return RSFG_OK;
 }
@@ -691,14 +834,20 @@ static EnsArgs ens_args(rsfg_ensemble* e, int s) {
   a.tiles = e->d_tiles;
   a.units = e->d_units;
   a.ngeo = e->ngeo;
-  a.cb_chunks = kEnsCfgs[e->cfg].cb / 16;
+  a.cb_chunks = std::max(1, kEnsCfgs[e->cfg].cb / 16);
   a.s = s;
+  const bool sk = kEnsCfgs[e->cfg].sk > 0;
+  a.pieces = sk ? e->d_pieces : nullptr;
+  a.wpc = sk ? e->d_wpc : nullptr;
+  a.rg = sk ? e->d_rg : nullptr;
+  a.Psk = sk ? e->d_Psk : nullptr;
+  a.sk_tr = sk ? cfg_rows(e->cfg) : 0;
   return a;
 }
 
 // the stage product and epilogue of stage a.s for every member
 static void ens_stage(rsfg_ensemble* e, const EnsArgs& a) {
-  prod_fn(e->nm, e->cfg)(a, e->nunits, e->stream);
+  prod_fn(e->nm, e->cfg)(a, e->nprod, e->stream);
   epi_fn(e->ncomp, e->G, e->nm)(a, e->ntiles, e->stream);
 }
 
@@ -779,9 +928,14 @@ int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_g
   e->M = ngeom * members_per_geom;
   e->ncomp = ncomp;
   e->G = G;
+  if (hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, e->device) !=
+          hipSuccess ||
+      e->ncu <= 0)
+    e->ncu = 256;
   size_t vtot = 0, ptot = 0, ktot = 0;
   int cb_min = kEnsCfgs[0].cb;
-  for (int c = 0; c < kEnsNCfg; ++c) cb_min = std::min(cb_min, kEnsCfgs[c].cb);
+  for (int c = 0; c < kEnsNCfg; ++c)
+    if (kEnsCfgs[c].cb > 0) cb_min = std::min(cb_min, kEnsCfgs[c].cb);
   for (int g = 0; g < ngeom; ++g) {
     e->geoms.push_back(geoms[g]);
     EnsGeo q;
@@ -1024,7 +1178,7 @@ int rsfg_ensemble_bench(rsfg_ensemble* e, const rsfg_solver_opts* o, int nrep, d
   EnsEpiFn ef = epi_fn(e->ncomp, e->G, e->nm);
   float t = 0.f;
   RSFG_HIP(hipEventRecord(e->ev0, e->stream));
-  for (int r = 0; r < nrep; ++r) pf(a, e->nunits, e->stream);
+  for (int r = 0; r < nrep; ++r) pf(a, e->nprod, e->stream);
   RSFG_HIP(hipEventRecord(e->ev1, e->stream));
   RSFG_HIP(hipEventSynchronize(e->ev1));
   RSFG_HIP(hipEventElapsedTime(&t, e->ev0, e->ev1));
@@ -1045,12 +1199,13 @@ int rsfg_ensemble_config(rsfg_ensemble* e, int* cfg, int* ncfg, int* shape) {
   if (ncfg) *ncfg = kEnsNCfg;
   if (shape) {
     for (int c = 0; c < kEnsNCfg; ++c) {
-      shape[6 * c] = 16 * kEnsCfgs[c].rf;
-      shape[6 * c + 1] = kEnsCfgs[c].w;
-      shape[6 * c + 2] = kEnsCfgs[c].u;
-      shape[6 * c + 3] = kEnsCfgs[c].cb;
-      shape[6 * c + 4] = kEnsCfgs[c].d;
-      shape[6 * c + 5] = kEnsCfgs[c].s;
+      shape[7 * c] = 16 * kEnsCfgs[c].rf;
+      shape[7 * c + 1] = kEnsCfgs[c].w;
+      shape[7 * c + 2] = kEnsCfgs[c].u;
+      shape[7 * c + 3] = kEnsCfgs[c].cb;
+      shape[7 * c + 4] = kEnsCfgs[c].d;
+      shape[7 * c + 5] = kEnsCfgs[c].s;
+      shape[7 * c + 6] = kEnsCfgs[c].sk;
     }
   }
   return RSFG_OK;
@@ -1075,6 +1230,10 @@ int rsfg_ensemble_destroy(rsfg_ensemble* e) {
   hipFree(e->d_args);
   hipFree(e->d_tiles);
   hipFree(e->d_units);
+  hipFree(e->d_pieces);
+  hipFree(e->d_wpc);
+  hipFree(e->d_rg);
+  hipFree(e->d_Psk);
   hipFree(e->d_P);
   hipFree(e->d_Kt);
   hipFree(e->d_halted);

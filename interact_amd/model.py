@@ -373,9 +373,9 @@ class Ensemble:
     def shapes(self):
         ncfg, cur = C.c_int(), C.c_int()
         self._lib.rsfg_ensemble_config(self.e, C.byref(cur), C.byref(ncfg), None)
-        sh = (C.c_int * (6 * ncfg.value))()
+        sh = (C.c_int * (7 * ncfg.value))()
         self._lib.rsfg_ensemble_config(self.e, None, None, sh)
-        return [tuple(sh[6 * i:6 * i + 6]) for i in range(ncfg.value)]
+        return [tuple(sh[7 * i:7 * i + 7]) for i in range(ncfg.value)]
 
     def select(self, cfg):
         """use candidate shape cfg (an index into shapes()) without tuning"""

@@ -137,7 +137,7 @@ def test_ensemble_matches_single_fp32_runs_and_tuner():
             X0 = _configure(ens.members[gi * nm + k], f, 2)
             runs.append((gi, f, X0))
     choice, ms, shapes = ens.tune(rtol=1e-6)
-    assert len(shapes) == len(ms) and all(len(s) == 6 for s in shapes)
+    assert len(shapes) == len(ms) and all(len(s) == 7 for s in shapes)
     timed = [t for t in ms if t > 0]
     assert len(timed) >= 4 and ms[choice] == min(timed)
     assert ens.config() == choice
@@ -172,8 +172,9 @@ def test_every_unit_shape_gives_the_same_run(nm, outlines):
     FP64 every 256 columns, blocks added in order), whichever rows a wave
     holds, how many chunks it loads at once, how deep its load ring is and
     whether the unit's waves share the slip rates through LDS, so their runs
-    are bitwise the same; the other column-block widths change only the FP64
-    partial sums (onsets within 1e-4 yr)."""
+    are bitwise the same; the other column-block widths and the stream-K
+    shapes (equal shares of the product per workgroup, cut at row-group
+    boundaries) change only the FP64 partial sums (onsets within 1e-4 yr)."""
     from interact_amd import faults, lib
     from interact_amd.model import Ensemble
     geoms = _geoms(2.0, lib.SLIP_VECTOR, outlines=outlines)
@@ -184,6 +185,7 @@ def test_every_unit_shape_gives_the_same_run(nm, outlines):
             X0s.append(_configure(ens.members[gi * nm + k], faults.ensemble_fields(g, L, W, k, gi), 2))
     shapes = ens.shapes()
     assert any(s[5] == 1 for s in shapes) and any(s[4] >= 4 for s in shapes)
+    assert any(s[6] > 0 and s[3] == 0 for s in shapes)
     t_end = 15.0 * SEC_PER_YEAR
     ref = None
     ntried = 0
