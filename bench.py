@@ -624,6 +624,19 @@ def adjoint_bench(args):
     log(f"adjoint: J {r['J']:.6e} (replay {J_rep:.6e}); gradient in {wall:.2f} s, "
         f"replay alone {replay_s:.2f} s; |dJ/d(a-b)| {np.linalg.norm(g_amb):.3e} "
         f"|dJ/dDc| {np.linalg.norm(g_dc):.3e}")
+    # the same gradient with the FP32 operator (FP64 state and adjoint)
+    fs.set_state(Xs, ts)
+    fs.adjoint_gradient(steps, recv, target, weight, prec=lib.PREC_FP32, checkpoint_every=256)
+    t0 = time.perf_counter()
+    r32 = fs.adjoint_gradient(steps, recv, target, weight, prec=lib.PREC_FP32,
+                              checkpoint_every=256)
+    wall32 = time.perf_counter() - t0
+    fp32 = dict(gradient_s=wall32, J_rel=abs(r32["J"] - r["J"]) / abs(r["J"]),
+                **{k + "_rel": float(np.linalg.norm(r32[k] - r[k]) / np.linalg.norm(r[k]))
+                   for k in ("grad_a", "grad_b", "grad_dc")})
+    log(f"adjoint: FP32 operator gradient in {wall32:.2f} s; relative differences to FP64 "
+        f"J {fp32['J_rel']:.1e}, grad_a {fp32['grad_a_rel']:.1e}, "
+        f"grad_b {fp32['grad_b_rel']:.1e}, grad_dc {fp32['grad_dc_rel']:.1e}")
     # central differences on 32 random directions of (a - b at fixed a, D_c)
     amb0 = f["a"] - f["b"]
     eps = 1e-5
@@ -694,7 +707,7 @@ def adjoint_bench(args):
                         fd_worst_rel=float(max(rels)) if rels else None,
                         fd_median_rel=float(np.median(rels)) if rels else None,
                         oracle_steps=Np, oracle_rel=ref_err, first_event_yr=t_first,
-                        window_start_yr=ts / SEC_PER_YEAR),
+                        window_start_yr=ts / SEC_PER_YEAR, fp32_operator=fp32),
         "roofline": dict(kernel="k_adj_kt<double> (K^T w of the backward sweep, row-split "
                                 "partials)", bound="hbm", achieved=kt_bytes / kt_ms / 1e6,
                          peak=HBM_PEAK_GBS, unit="GB/s", frac=kt_bytes / kt_ms / 1e6 / HBM_PEAK_GBS,
