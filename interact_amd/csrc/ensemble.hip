@@ -48,7 +48,6 @@ struct EnsGeo {        // one geometry of the ensemble
   float* V[2];         // product inputs by stage parity, nm x ld (member-major)
   double* P;           // product partials [column block][rows_pad][nm]
   int rows_pad;        // rows rounded up to a whole number of product row groups
-  int rg0;             // stream-K plans: this geometry's first row group
 };
 
 constexpr int kEpiPatches = 16;   // patches per epilogue tile (16 G operator rows)
@@ -59,18 +58,17 @@ constexpr int kColPad = 256;      // ld granularity (the product's FP64 flush se
 struct EnsArgs {
   const StageArgs* args;  // per member
   const EnsGeo* geo;      // per geometry
-  const int2* tiles;      // epilogue: (geometry, first row) per workgroup
+  const int4* tiles;      // epilogue: (geometry, first row, first piece, pieces) per
+                          // workgroup; the pieces (stream-K plans) of the tile's row group
   const int4* units;      // product: (geometry, first row, first chunk, end chunk)
   int ngeo;
   int cb_chunks;          // 16-column chunks per column block
   int s;                  // stage
   // stream-K plans (sk_tr > 0): pieces (geometry, first row, first chunk,
   // end chunk) of sk_tr-row groups, workgroup w taking pieces
-  // [wpc[w], wpc[w + 1]); per row group (first piece, pieces), in order;
-  // piece k's partials at Psk + k sk_tr nm
+  // [wpc[w], wpc[w + 1]); piece k's partials at Psk + k sk_tr nm
   const int4* pieces;
   const int* wpc;
-  const int2* rg;
   double* Psk;
   int sk_tr;
 };
@@ -525,7 +523,7 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
   constexpr int NPT = kEpiPatches;
   constexpr int NM = 16 * MF;
   __shared__ MemberSnap snap[16];
-  const int2 tile = e.tiles[blockIdx.x];
+  const int4 tile = e.tiles[blockIdx.x];
   const EnsGeo& gg = e.geo[tile.x];
   const int r0 = tile.y, s = e.s;
   const int tid = threadIdx.x, mb = 16 * blockIdx.y;   // this block's 16 members
@@ -554,11 +552,10 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
     for (int k = 0; k < G; ++k) {
       double v = 0.0;
       if (e.sk_tr > 0) {
-        const int row = p * G + k;
-        const int2 rg = e.rg[gg.rg0 + row / e.sk_tr];
+        // the tile lies in one row group, whose pieces are tile.z .. + tile.w
         const size_t ps = (size_t)e.sk_tr * NM;
-        const double* q = e.Psk + (size_t)rg.x * ps + (size_t)(row % e.sk_tr) * NM + m;
-        for (int c = 0; c < rg.y; ++c) v += q[c * ps];
+        const double* q = e.Psk + (size_t)tile.z * ps + (size_t)((p * G + k) % e.sk_tr) * NM + m;
+        for (int c = 0; c < tile.w; ++c) v += q[c * ps];
       } else {
         const double* q = gg.P + (size_t)(p * G + k) * NM + m;
         for (int cb = 0; cb < ncb; ++cb) v += q[cb * cbs];
@@ -679,14 +676,13 @@ struct rsfg_ensemble {
   std::vector<EnsGeo> geo;
   EnsGeo* d_geo = nullptr;
   StageArgs* d_args = nullptr;
-  int2* d_tiles = nullptr;
+  int4* d_tiles = nullptr;
   int ntiles = 0;
   int4* d_units = nullptr;
   int nunits = 0;
   int nprod = 0;             // product workgroups of the current plan
   int4* d_pieces = nullptr;  // stream-K plan
   int* d_wpc = nullptr;
-  int2* d_rg = nullptr;
   double* d_Psk = nullptr;
   size_t psk_cap = 0;        // doubles allocated at d_Psk
   int ncu = 0;
@@ -702,14 +698,16 @@ struct rsfg_ensemble {
 // Stream-K plan for row groups of TR rows and nwg workgroups: the (row
 // group, segment) items of all geometries in order, item range
 // [w T / nwg, (w + 1) T / nwg) to workgroup w, cut at row-group boundaries.
-static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR, int nwg) {
+static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR, int nwg,
+                         std::vector<int4>& tiles) {
   struct RowGroup {
     int g, r, nseg;
   };
   std::vector<RowGroup> rgs;
+  std::vector<int> rg0(e->ngeo);
   for (int g : order) {
-    EnsGeo& q = e->geo[g];
-    q.rg0 = (int)rgs.size();
+    const EnsGeo& q = e->geo[g];
+    rg0[g] = (int)rgs.size();
     for (int r = 0; r < q.rows; r += TR) rgs.push_back({g, r, (int)(q.ld / 256)});
   }
   long long T = 0;
@@ -739,13 +737,17 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
   }
   wpc[nwg] = (int)pieces.size();
   RSFG_CHECK(item == T && ri == rgs.size(), RSFG_EINVAL, "rsfg_ensemble: stream-K plan");
+  // each epilogue tile (16 G rows, TR a multiple of them) lies in one row group
+  for (int4& t : tiles) {
+    const int2 rg = rgtab[rg0[t.x] + t.y / TR];
+    t.z = rg.x;
+    t.w = rg.y;
+  }
   const size_t need = pieces.size() * (size_t)TR * e->nm;
   hipFree(e->d_pieces);
   hipFree(e->d_wpc);
-  hipFree(e->d_rg);
   e->d_pieces = nullptr;
   e->d_wpc = nullptr;
-  e->d_rg = nullptr;
   if (need > e->psk_cap) {
     hipFree(e->d_Psk);
     e->d_Psk = nullptr;
@@ -758,8 +760,6 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
                      hipMemcpyHostToDevice));
   RSFG_HIP(hipMalloc(&e->d_wpc, sizeof(int) * wpc.size()));
   RSFG_HIP(hipMemcpy(e->d_wpc, wpc.data(), sizeof(int) * wpc.size(), hipMemcpyHostToDevice));
-  RSFG_HIP(hipMalloc(&e->d_rg, sizeof(int2) * rgtab.size()));
-  RSFG_HIP(hipMemcpy(e->d_rg, rgtab.data(), sizeof(int2) * rgtab.size(), hipMemcpyHostToDevice));
   e->nprod = nwg;
   return RSFG_OK;
 }
@@ -775,20 +775,20 @@ static int build_plan(rsfg_ensemble* e, int cfg) {
   for (int g = 0; g < e->ngeo; ++g) order[g] = g;
   std::sort(order.begin(), order.end(),
             [&](int x, int y) { return e->geo[x].ld > e->geo[y].ld; });
-  std::vector<int2> tiles;
+  std::vector<int4> tiles;
   std::vector<int4> units;
   for (int g : order) {
     EnsGeo& q = e->geo[g];
     q.tile0 = (int)tiles.size();
     q.ntiles = (q.n + kEpiPatches - 1) / kEpiPatches;
-    for (int t = 0; t < q.ntiles; ++t) tiles.push_back(make_int2(g, t * kEpiPatches * e->G));
+    for (int t = 0; t < q.ntiles; ++t) tiles.push_back(make_int4(g, t * kEpiPatches * e->G, 0, 0));
     const int nk = (int)(q.ld / 16);
     if (SK) continue;
     for (int r = 0; r < q.rows; r += TR)
       for (int k = 0; k < nk; k += CBK) units.push_back(make_int4(g, r, k, std::min(nk, k + CBK)));
   }
   if (SK) {
-    const int rc = build_sk_plan(e, order, TR, SK * e->ncu);
+    const int rc = build_sk_plan(e, order, TR, SK * e->ncu, tiles);
     if (rc) return rc;
     units.push_back(make_int4(0, 0, 0, 0));   // keeps the unit list allocated
   }
@@ -796,8 +796,8 @@ static int build_plan(rsfg_ensemble* e, int cfg) {
   hipFree(e->d_units);
   e->d_tiles = nullptr;
   e->d_units = nullptr;
-  RSFG_HIP(hipMalloc(&e->d_tiles, sizeof(int2) * tiles.size()));
-  RSFG_HIP(hipMemcpy(e->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice));
+  RSFG_HIP(hipMalloc(&e->d_tiles, sizeof(int4) * tiles.size()));
+  RSFG_HIP(hipMemcpy(e->d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
   RSFG_HIP(hipMalloc(&e->d_units, sizeof(int4) * units.size()));
   RSFG_HIP(hipMemcpy(e->d_units, units.data(), sizeof(int4) * units.size(), hipMemcpyHostToDevice));
   RSFG_HIP(hipMemcpy(e->d_geo, e->geo.data(), sizeof(EnsGeo) * e->ngeo, hipMemcpyHostToDevice));
@@ -839,7 +839,6 @@ static EnsArgs ens_args(rsfg_ensemble* e, int s) {
   const bool sk = kEnsCfgs[e->cfg].sk > 0;
   a.pieces = sk ? e->d_pieces : nullptr;
   a.wpc = sk ? e->d_wpc : nullptr;
-  a.rg = sk ? e->d_rg : nullptr;
   a.Psk = sk ? e->d_Psk : nullptr;
   a.sk_tr = sk ? cfg_rows(e->cfg) : 0;
   return a;
@@ -1232,7 +1231,6 @@ int rsfg_ensemble_destroy(rsfg_ensemble* e) {
   hipFree(e->d_units);
   hipFree(e->d_pieces);
   hipFree(e->d_wpc);
-  hipFree(e->d_rg);
   hipFree(e->d_Psk);
   hipFree(e->d_P);
   hipFree(e->d_Kt);
