@@ -687,6 +687,7 @@ struct rsfg_ensemble {
   size_t psk_cap = 0;        // doubles allocated at d_Psk
   int ncu = 0;
   int cfg = 0;
+  bool plan_ok = false;      // the work lists of cfg are complete on the device
   float* d_v = nullptr;
   double* d_P = nullptr;
   float* d_Kt = nullptr;
@@ -769,6 +770,7 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
 // rows, one unit per block of CB columns; geometries with the longest rows
 // first (or, for stream-K shapes, the plan of build_sk_plan).
 static int build_plan(rsfg_ensemble* e, int cfg) {
+  e->plan_ok = false;   // until every list of the new plan is on the device
   const int TR = cfg_rows(cfg), SK = kEnsCfgs[cfg].sk;
   const int CBK = SK ? 1 : kEnsCfgs[cfg].cb / 16;
   std::vector<int> order(e->ngeo);
@@ -805,6 +807,7 @@ static int build_plan(rsfg_ensemble* e, int cfg) {
   e->nunits = (int)units.size();
   if (!SK) e->nprod = e->nunits;
   e->cfg = cfg;
+  e->plan_ok = true;
   return RSFG_OK;
 }
 
@@ -1024,6 +1027,8 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
              "rsfg_ensemble_solve: method must be rk5dp or rk3bs");
   RSFG_CHECK(o->prec == RSFG_PREC_FP32, RSFG_EINVAL,
              "rsfg_ensemble_solve: the batched product uses the FP32 operator (prec FP32)");
+  RSFG_CHECK(e->plan_ok, RSFG_ESTATE,
+             "rsfg_ensemble_solve: no complete work plan (select or tune a shape again)");
   RSFG_HIP(hipSetDevice(e->device));
   const auto wall0 = std::chrono::steady_clock::now();
   int rc = fill_member_args(e, o);
@@ -1160,6 +1165,8 @@ int rsfg_ensemble_tune(rsfg_ensemble* e, const rsfg_solver_opts* o, int* choice,
 int rsfg_ensemble_bench(rsfg_ensemble* e, const rsfg_solver_opts* o, int nrep, double* ms_product,
                         double* ms_epilogue) {
   RSFG_CHECK(e && o && nrep > 0, RSFG_EINVAL, "rsfg_ensemble_bench: bad arguments");
+  RSFG_CHECK(e->plan_ok, RSFG_ESTATE,
+             "rsfg_ensemble_bench: no complete work plan (select or tune a shape again)");
   RSFG_HIP(hipSetDevice(e->device));
   rsfg_solver_opts oo = *o;
   oo.prec = RSFG_PREC_FP32;
