@@ -594,7 +594,8 @@ static const EnsCfg kEnsCfgs[] = {
     {8, 1, 1, 1024, 2, 0, 0}, {4, 1, 1, 2048, 2, 0, 0}, {4, 1, 1, 1024, 4, 0, 0},
     {4, 4, 1, 1024, 2, 1, 0}, {4, 2, 1, 1024, 2, 1, 0}, {2, 4, 1, 1024, 2, 1, 0},
     {4, 4, 1, 1024, 4, 1, 0}, {2, 8, 1, 1024, 2, 1, 0}, {4, 4, 1, 0, 4, 1, 1},
-    {4, 4, 1, 0, 2, 1, 2},    {4, 4, 1, 0, 4, 1, 2},    {2, 8, 1, 0, 4, 1, 1}};
+    {4, 4, 1, 0, 2, 1, 2},    {4, 4, 1, 0, 4, 1, 2},    {2, 8, 1, 0, 4, 1, 1},
+    {2, 4, 1, 0, 8, 1, 1},    {2, 4, 1, 0, 4, 1, 2}};
 constexpr int kEnsNCfg = sizeof(kEnsCfgs) / sizeof(kEnsCfgs[0]);
 
 static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].w; }
@@ -639,7 +640,9 @@ static EnsProdFn prod_fn_mf(int cfg) {
     case 14: if constexpr (MF <= 2) return launch_prod_sk<4, MF, 4, 1, 4>; else return nullptr;
     case 15: if constexpr (MF <= 2) return launch_prod_sk<4, MF, 4, 1, 2>; else return nullptr;
     case 16: if constexpr (MF <= 2) return launch_prod_sk<4, MF, 4, 1, 4>; else return nullptr;
-    default: return launch_prod_sk<2, MF, 8, 1, 4>;
+    case 17: return launch_prod_sk<2, MF, 8, 1, 4>;
+    case 18: if constexpr (MF <= 2) return launch_prod_sk<2, MF, 4, 1, 8>; else return nullptr;
+    default: if constexpr (MF <= 2) return launch_prod_sk<2, MF, 4, 1, 4>; else return nullptr;
   }
 }
 
