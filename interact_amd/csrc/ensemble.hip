@@ -708,7 +708,7 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
     int g, r, nseg;
   };
   std::vector<RowGroup> rgs;
-  std::vector<int> rg0(e->ngeo);
+  std::vector<int> rg0(e->ngeo, -1);
   for (int g : order) {
     const EnsGeo& q = e->geo[g];
     rg0[g] = (int)rgs.size();
@@ -743,6 +743,7 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
   RSFG_CHECK(item == T && ri == rgs.size(), RSFG_EINVAL, "rsfg_ensemble: stream-K plan");
   // each epilogue tile (16 G rows, TR a multiple of them) lies in one row group
   for (int4& t : tiles) {
+    if (rg0[t.x] < 0) continue;   // not in this plan (all its members have halted)
     const int2 rg = rgtab[rg0[t.x] + t.y / TR];
     t.z = rg.x;
     t.w = rg.y;
@@ -771,8 +772,10 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
 // Work lists for configuration cfg.  Epilogue: per geometry ceil(n / 16)
 // tiles of 16 patches.  Product: per geometry and row group of cfg_rows
 // rows, one unit per block of CB columns; geometries with the longest rows
-// first (or, for stream-K shapes, the plan of build_sk_plan).
-static int build_plan(rsfg_ensemble* e, int cfg) {
+// first (or, for stream-K shapes, the plan of build_sk_plan over the
+// geometries flagged in run, all when run is null: a static split of the
+// work stays balanced only while every geometry in it is running).
+static int build_plan(rsfg_ensemble* e, int cfg, const std::vector<char>* run = nullptr) {
   e->plan_ok = false;   // until every list of the new plan is on the device
   const int TR = cfg_rows(cfg), SK = kEnsCfgs[cfg].sk;
   const int CBK = SK ? 1 : kEnsCfgs[cfg].cb / 16;
@@ -793,7 +796,10 @@ static int build_plan(rsfg_ensemble* e, int cfg) {
       for (int k = 0; k < nk; k += CBK) units.push_back(make_int4(g, r, k, std::min(nk, k + CBK)));
   }
   if (SK) {
-    const int rc = build_sk_plan(e, order, TR, SK * e->ncu, tiles);
+    std::vector<int> sk_order;
+    for (int g : order)
+      if (!run || (*run)[g]) sk_order.push_back(g);
+    const int rc = build_sk_plan(e, sk_order, TR, SK * e->ncu, tiles);
     if (rc) return rc;
     units.push_back(make_int4(0, 0, 0, 0));   // keeps the unit list allocated
   }
@@ -1059,6 +1065,9 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
   RSFG_HIP(hipStreamEndCapture(e->stream, &graph));
   RSFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   std::vector<long long> hs(2 * e->M, 0), prev(e->M, -1);
+  std::vector<char> running(e->ngeo, 1);
+  const bool sk = kEnsCfgs[e->cfg].sk > 0;
+  bool replanned = false;
   long launches = 0;
   for (;;) {
     hipError_t he = hipGraphLaunch(exec, e->stream);
@@ -1086,9 +1095,42 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
       rc = RSFG_ESTATE;
       break;
     }
+    if (sk) {
+      // a stream-K split covers the geometries still running
+      std::vector<char> now(e->ngeo, 0);
+      for (int m = 0; m < e->M; ++m)
+        if (!hs[2 * m]) now[m / e->nm] = 1;
+      if (now != running) {
+        running = now;
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        rc = build_plan(e, e->cfg, &running);
+        if (rc) break;
+        replanned = true;
+        he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+        if (he == hipSuccess) {
+          for (int k = 0; k < per; ++k) ens_attempt(e, tb.s);
+          hipLaunchKernelGGL(k_halted_b, dim3((e->M + 255) / 256), dim3(256), 0, e->stream,
+                             ens_args(e, 0), e->M, e->d_halted);
+          he = hipStreamEndCapture(e->stream, &graph);
+        }
+        if (he == hipSuccess) he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (he != hipSuccess) {
+          rc = hip_fail(he, "rsfg_ensemble_solve: capture", __FILE__, __LINE__);
+          break;
+        }
+      }
+    }
   }
-  hipGraphExecDestroy(exec);
-  hipGraphDestroy(graph);
+  if (exec) hipGraphExecDestroy(exec);
+  if (graph) hipGraphDestroy(graph);
+  // later solves, tuning and timing start from the plan over every geometry
+  if (replanned) {
+    const int rc2 = build_plan(e, e->cfg);
+    if (rc == RSFG_OK) rc = rc2;
+  }
   if (rc) return rc;
   const double wall =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
