@@ -258,6 +258,43 @@ def test_ensemble_of_64_members():
     fs.close()
 
 
+def test_ensemble_rk3bs_matches_single_runs():
+    """The rk3bs pair in the batched stages (four-stage tableau, its own
+    controller exponent): 16 members on one geometry over 20 yr reproduce
+    the single-fault FP32 rk3bs runs' onsets within 1e-4 yr and their step
+    counts within 3 %."""
+    from interact_amd import faults, lib
+    from interact_amd.model import Ensemble
+    geoms = _geoms(2.0, lib.SLIP_VECTOR, outlines=[(40, 20)])
+    fs, g, L, W = geoms[0]
+    nm = 16
+    ens = Ensemble([fs], nm)
+    runs = []
+    for k in range(nm):
+        f = faults.ensemble_fields(g, L, W, k)
+        runs.append((f, _configure(ens.members[k], f, 2)))
+    t_end = 20.0 * SEC_PER_YEAR
+    st = ens.solve(t_end, rtol=1e-6, method=lib.RK3BS)
+    assert all(q["status"] == 0 and q["t"] >= t_end for q in st)
+    n_on = 0
+    for i in range(0, nm, 3):
+        f, X0 = runs[i]
+        s = fs.member()
+        _configure(s, f, 2)
+        s1 = s.solve(t_end, rtol=1e-6, prec=lib.PREC_FP32, method=lib.RK3BS)
+        on1, on2 = _onsets(s.events()), _onsets(ens.members[i].events())
+        assert len(on1) == len(on2), (i, on1, on2)
+        if len(on1):
+            assert np.abs(on1 - on2).max() < 1e-4, (i, on1, on2)
+        n_on += len(on1)
+        assert abs(s1["steps"] - st[i]["steps"]) <= 0.03 * s1["steps"] + 3, \
+            (i, s1["steps"], st[i]["steps"])
+        s.close()
+    assert n_on >= 3
+    ens.close()
+    fs.close()
+
+
 def test_ensemble_argument_checks():
     from interact_amd import faults, lib
     from interact_amd.model import Ensemble
