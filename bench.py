@@ -54,6 +54,13 @@ HBM_PEAK_GBS = 8000.0
 STAGE_TRAFFIC = {"c2": {"fp64": 517.6e6, "fp32": 262.1e6, "bf16x2": 262.2e6, "bf16": 132.6e6,
                         "bf16nf": 145.9e6},
                  "c3": {}}
+# HBM bytes per launch of the c4 batched product by unit shape (the shapes
+# the tuner picks on these boxes), from the same two counter runs of
+# `--config c4 --kernels-only` with the copy-kernel factors above
+# (profiles/k_prod_b_ensemble_c4.md)
+PRODUCT_TRAFFIC_C4 = {(32, 4, 1, 0, 8, 1, 1): 449.9e6, (32, 4, 1, 0, 4, 1, 2): 459.4e6,
+                      (64, 4, 1, 1024, 2, 1, 0): 462.8e6, (32, 8, 1, 1024, 2, 1, 0): 462.5e6,
+                      (32, 8, 1, 0, 4, 1, 1): 464.8e6, (64, 4, 1, 1024, 4, 1, 0): 465.1e6}
 # the mixed path reported as value: bf16 hi for every entry plus the bf16 lo
 # of each patch's near-field column chunks (compensated where it matters)
 HEADLINE = "bf16nf"
@@ -428,7 +435,8 @@ def ensemble_bench(args):
                        f"operators), shape {shapes[choice]}",
                 bound="hbm", achieved=k_bytes / launch_s / 1e9, peak=HBM_PEAK_GBS,
                 unit="GB/s", frac=k_bytes / launch_s / 1e9 / HBM_PEAK_GBS,
-                traffic=None, bytes_per_launch=k_bytes, launch_us=launch_s * 1e6,
+                traffic=PRODUCT_TRAFFIC_C4.get(tuple(shapes[choice])) if nm == 32 else None,
+                bytes_per_launch=k_bytes, launch_us=launch_s * 1e6,
                 flops_per_launch=flops, mfma_tflops=flops / launch_s / 1e12,
                 mfma_frac=flops / launch_s / 1e12 / FP32_MFMA_PEAK_TFLOPS,
                 stage_us=ms[choice] * 1e3, epilogue_us=epi_ms * 1e3)
