@@ -61,6 +61,10 @@ STAGE_TRAFFIC = {"c2": {"fp64": 517.6e6, "fp32": 262.1e6, "bf16x2": 262.2e6, "bf
 PRODUCT_TRAFFIC_C4 = {(32, 4, 1, 0, 8, 1, 1): 449.9e6, (32, 4, 1, 0, 4, 1, 2): 459.4e6,
                       (64, 4, 1, 1024, 2, 1, 0): 462.8e6, (32, 8, 1, 1024, 2, 1, 0): 462.5e6,
                       (32, 8, 1, 0, 4, 1, 1): 464.8e6, (64, 4, 1, 1024, 4, 1, 0): 465.1e6}
+# HBM bytes per launch of the backward sweep's K^T w at c5 (FP64 operator),
+# from the same two counter runs of `--config c5 --kernels-only`
+# (profiles/adjoint_planar1km_8000dof.md)
+ADJ_KT_TRAFFIC_C5 = 521.2e6
 # the mixed path reported as value: bf16 hi for every entry plus the bf16 lo
 # of each patch's near-field column chunks (compensated where it matters)
 HEADLINE = "bf16nf"
@@ -723,7 +727,8 @@ def adjoint_bench(args):
         "roofline": dict(kernel="k_adj_kt<double> (K^T w of the backward sweep, row-split "
                                 "partials)", bound="hbm", achieved=kt_bytes / kt_ms / 1e6,
                          peak=HBM_PEAK_GBS, unit="GB/s", frac=kt_bytes / kt_ms / 1e6 / HBM_PEAK_GBS,
-                         traffic=None, bytes_per_launch=kt_bytes, launch_us=kt_ms * 1e3),
+                         traffic=ADJ_KT_TRAFFIC_C5, bytes_per_launch=kt_bytes,
+                         launch_us=kt_ms * 1e3),
         "cpu_baseline": dict(value=cpu_rate, unit="sim-years/hour", cores=threads, kind="port",
                              sample=(f"the oracle's FP64 reference adjoint (replay + backward, "
                                      f"OpenMP products on {threads} threads) on the window's "
