@@ -1,12 +1,16 @@
 """Benchmark of the dense rsf_solve hot path on one MI355X.
 
-Workload (default, the 1 km planar configuration): vertical strike-slip
-fault 100 x 40 km at 1 km patches = 4000 patches x 2 slip components =
-8000 DOF, dense K of 8000 x 8000; aging-law rate-and-state friction with
-radiation damping; seeded initial slip rate vpl (1 + 1e-3 N(0,1)); rk5dp
-with relative tolerance 1e-8 over 400 yr.  --config c3 runs the same fault
-at 0.5 km patches (16000 patches, 32000 DOF) over 200 yr.  One bench
-"step" is one pass of the whole cycle integration over that window.
+Workload (default, the large dense configuration): vertical strike-slip
+fault 100 x 40 km at 0.5 km patches = 16000 patches x 2 slip components =
+32000 DOF, dense K of 32000 x 32000 (8.2 GB in FP64); aging-law
+rate-and-state friction with radiation damping; seeded initial slip rate
+vpl (1 + 1e-3 N(0,1)); rk5dp with relative tolerance 1e-8 over 200 yr.
+--config c2 runs the same fault at 1 km patches (4000 patches, 8000 DOF)
+over 400 yr.  The headline cycle is timed as --steps continued segments of
+equal length (rsfg_solve_continue, whose accepted steps are exactly those
+of one solve over the whole window): one bench "step" is one segment, and
+the timed steps together are always the whole cycle; --warmup segments of
+the same length run first from the initial state, untimed.
 
 Reported: simulated years per wall-clock hour of the mixed-precision path
 as `value` (K as bf16 hi for every entry plus the bf16 lo of each patch's
@@ -32,7 +36,7 @@ ensemble (K [v_1 .. v_32] per geometry on the matrix cores, FP32 operators in MF
 against the same 128 members run one after another through the single-fault
 solver.
 
-    python bench.py [--gpus 1] [--steps 1] [--warmup 0] [--config c2|c3|c4|c5]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--config c3|c2|c4|c5]
 """
 import argparse
 import json
@@ -54,6 +58,8 @@ HBM_PEAK_GBS = 8000.0
 STAGE_TRAFFIC = {"c2": {"fp64": 517.6e6, "fp32": 262.1e6, "bf16x2": 262.2e6, "bf16": 132.6e6,
                         "bf16nf": 145.9e6},
                  "c3": {}}
+# HBM bytes per launch of the FP32 product k_matvec<float,2> (same counters)
+MATVEC_TRAFFIC = {"c2": {}, "c3": {}}
 # HBM bytes per launch of the c4 batched product by unit shape (the shapes
 # the tuner picks on these boxes), from the same two counter runs of
 # `--config c4 --kernels-only` with the copy-kernel factors above
@@ -101,45 +107,92 @@ def peak_after(ev, t_yr):
     return float(rows[0, 5]) if len(rows) else float("nan")
 
 
+def host_cpu():
+    """CPU model, sockets, physical cores and the cores this process may use
+    (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    info = dict(model="unknown", sockets=None, physical_cores=None,
+                threads_per_core=None, affinity_cpus=len(os.sched_getaffinity(0)),
+                nproc=os.cpu_count())
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name", info["model"])
+        sockets = int(kv.get("Socket(s)", "0") or 0)
+        cps = int(kv.get("Core(s) per socket", "0") or 0)
+        tpc = int(kv.get("Thread(s) per core", "0") or 0)
+        if sockets and cps:
+            info.update(sockets=sockets, physical_cores=sockets * cps,
+                        threads_per_core=tpc or None)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    if info["model"] == "unknown":
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    return info
+
+
 def cpu_baseline(g, f, K, rhs_evals_ref, years, budget_s, log_fn):
     """Time the FP64 CPU oracle on the same workload (same operator values,
-    same state): dense K v + RHS evaluations for about budget_s seconds,
-    scaled by the RHS count of the GPU FP64 run to simulated years/hour;
-    assembly throughput timed on a 400-patch sample of the mesh."""
+    same state): dense K v + RHS evaluations for about budget_s seconds at
+    one thread and at all physical cores this process may use, each scaled
+    by the RHS count of the FP64 GPU run to simulated years/hour; assembly
+    throughput timed on a 400-patch sample of the mesh at both counts."""
     import oracle
     from interact_amd.model import initial_state
-    threads = len(os.sched_getaffinity(0))
+    cpu = host_cpu()
+    allowed = cpu["affinity_cpus"]
+    # all physical cores, as far as the affinity mask lets this process use them
+    full = max(1, min(cpu["physical_cores"] or allowed, allowed))
     n = len(g["l"])
     D = K.shape[0]
     rng = np.random.default_rng(0)
     sub = 400
     idx = np.sort(rng.choice(n, sub, replace=False))
     gs = {k: g[k][idx] for k in ("xyz", "strike", "dip", "l", "w")}
-    t0 = time.perf_counter()
-    oracle.traction_tensor(gs, 0, threads)
-    t_pairs = (time.perf_counter() - t0) / (sub * sub)
     xpl = np.zeros(D)
     xpl[0::2] = -f["vpl"]
-    P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ xpl, ncomp=2, eta=f["eta"],
-                          v0=f["v0"], f0=f["f0"], nthreads=threads)
     X = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
-    P.rhs(X)
-    nev, t0 = 0, time.perf_counter()
-    while time.perf_counter() - t0 < budget_s:
+    res = {}
+    for threads in (1, full):
+        t0 = time.perf_counter()
+        oracle.traction_tensor(gs, 0, threads)
+        t_pairs = (time.perf_counter() - t0) / (sub * sub)
+        P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ xpl, ncomp=2, eta=f["eta"],
+                              v0=f["v0"], f0=f["f0"], nthreads=threads)
         P.rhs(X)
-        nev += 1
-    t_rhs = (time.perf_counter() - t0) / nev
-    proj_s = rhs_evals_ref * t_rhs
-    log_fn(f"cpu oracle: {nev} RHS in {budget_s:.0f}s -> {t_rhs*1e3:.2f} ms/RHS, "
-           f"{t_pairs*1e9:.0f} ns/pair on {threads} threads")
-    return dict(value=years / (proj_s / 3600.0), unit="sim-years/hour", cores=threads,
-                kind="port",
-                sample=(f"{nev} FP64 right-hand sides (dense {D}x{D} K v + RHS, OpenMP "
-                        f"over rows, {threads} threads) on this workload, {t_rhs*1e3:.2f} ms "
-                        f"each, scaled by the {rhs_evals_ref} RHS evaluations the FP64 GPU "
-                        f"run needed for {years:g} yr; assembly {t_pairs*1e9:.0f} ns/pair "
-                        f"timed on a {sub}x{sub} sample"),
-                rhs_ms=t_rhs * 1e3, assembly_pairs_per_s=1.0 / t_pairs)
+        nev, t0 = 0, time.perf_counter()
+        while nev < 2 or time.perf_counter() - t0 < budget_s / 2:
+            P.rhs(X)
+            nev += 1
+        t_rhs = (time.perf_counter() - t0) / nev
+        res[threads] = (t_rhs, t_pairs, nev)
+        log_fn(f"cpu oracle, {threads} threads: {nev} RHS -> {t_rhs*1e3:.2f} ms/RHS, "
+               f"{t_pairs*1e9:.0f} ns/pair")
+        del P
+    t_rhs, t_pairs, nev = res[full]
+    t1, p1, n1 = res[1]
+    rate = lambda t: years / (rhs_evals_ref * t / 3600.0)
+    return dict(value=rate(t_rhs), unit="sim-years/hour", cores=full, kind="port",
+                value_1thread=rate(t1), cores_1thread=1,
+                sample=(f"FP64 right-hand sides (dense {D}x{D} K v + RHS, OpenMP over rows) "
+                        f"on this workload's state: {nev} at {full} threads ({t_rhs*1e3:.2f} ms "
+                        f"each) and {n1} at 1 thread ({t1*1e3:.1f} ms each), scaled by the "
+                        f"{rhs_evals_ref} RHS evaluations the FP64 GPU run needed for "
+                        f"{years:g} yr; assembly on a {sub}x{sub} sample: {t_pairs*1e9:.0f} "
+                        f"ns/pair at {full} threads, {p1*1e9:.0f} ns/pair at 1"),
+                rhs_ms=t_rhs * 1e3, rhs_ms_1thread=t1 * 1e3,
+                assembly_pairs_per_s=1.0 / t_pairs, assembly_pairs_per_s_1thread=1.0 / p1,
+                host=cpu)
 
 
 def kernels_only(fs, store):
@@ -174,12 +227,20 @@ def kernels_only(fs, store):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c2",
-                    help="c2: 1 km patches, 8000 DOF, 400 yr (default); c3: 0.5 km patches, "
-                         "32000 DOF, 200 yr; c4: ensemble of 4 geometries x 32 friction "
+    ap.add_argument("--steps", type=int, default=None,
+                    help="c2/c3: timed segments of the headline cycle (together always the "
+                         "whole cycle; default 20); c4/c5: timed repetitions (default 1)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="c2/c3: untimed segments of the same length from the initial state "
+                         "before the timed cycle (default 5); c4/c5: untimed repetitions "
+                         "(default 0)")
+    ap.add_argument("--config", choices=["c2", "c3", "c4", "c5"], default="c3",
+                    help="c3: 0.5 km patches, 32000 DOF, 200 yr (default); c2: 1 km patches, "
+                         "8000 DOF, 400 yr; c4: ensemble of 4 geometries x 32 friction "
                          "draws, 100 yr each; c5: adjoint gradient on the 1 km fault")
+    ap.add_argument("--no-oracle-c1", action="store_true",
+                    help="skip the c1 cycle (2000 DOF, 300 yr) on the device against the "
+                         "FP64 oracle's cycle")
     ap.add_argument("--years", type=float, default=None)
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--fd-directions", type=int, default=32,
@@ -191,7 +252,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default=None,
                     help="cycle runs, comma separated (fp64 first is the reference); "
-                         "default fp64,bf16nf,fp32,bf16x2,bf16 (c3: fp64,bf16nf,fp32)")
+                         "default fp64,bf16nf,fp32,bf16x2 (c2 adds bf16)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="one assembly, a few launches of the stage and matvec kernels "
                          "per precision and a 256 MiB device copy of known size, for "
@@ -199,6 +260,11 @@ def main():
                          "tuner's launches of every tile shape; c5: one gradient over 64 "
                          "fixed steps and 20 K^T w launches)")
     args = ap.parse_args()
+    cyc = args.config in ("c2", "c3")
+    if args.steps is None:
+        args.steps = 20 if cyc else 1
+    if args.warmup is None:
+        args.warmup = 5 if cyc else 0
     if args.gpus != 1:
         print(json.dumps({"error": "runs on one GPU only: the row-block sharding over "
                           "several GPUs is not built", "n_gpus": args.gpus}))
@@ -220,7 +286,9 @@ def main():
         args.years = {"c2": 400.0, "c3": 200.0}[args.config]
     if args.precisions is None:
         args.precisions = ("fp64,bf16nf,fp32,bf16x2,bf16" if args.config == "c2"
-                           else "fp64,bf16nf,fp32")
+                           else "fp64,bf16nf,fp32,bf16x2")
+    if args.steps < 1 or args.warmup < 0:
+        raise SystemExit("--steps must be >= 1 and --warmup >= 0")
 
     from interact_amd import faults, lib
     from interact_amd.model import FaultSystem, initial_state
@@ -262,28 +330,42 @@ def main():
             f"{kv[name]['matvec_gbs']:.0f} GB/s  stage {st_ms*1e3:.1f} us {kv[name]['stage_gbs']:.0f} GB/s")
 
     t_end = args.years * SEC_PER_YEAR
+    seg = t_end / args.steps
+    opts = dict(rtol=args.rtol, steps_per_launch=64)
     runs = {}
     for name in args.precisions.split(","):
         p = lib.PREC_BY_NAME[name]
-        headline = (name == HEADLINE)
-        nrep = args.steps if headline else 1
-        if headline:
-            for _ in range(args.warmup):
-                fs.set_state(X0)
-                fs.solve(t_end, rtol=args.rtol, prec=p, steps_per_launch=64)
-        walls = []
-        for _ in range(nrep):
+        if name == HEADLINE:
+            # warmup: W segments from the initial state (graph capture, caches)
+            fs.set_state(X0)
+            for k in range(args.warmup):
+                (fs.solve if k == 0 else fs.solve_continue)(min(k + 1, args.steps) * seg,
+                                                              prec=p, **opts)
+            # timed: the whole cycle as K continued segments (one step each);
+            # their accepted steps are exactly those of one solve to t_end
+            fs.set_state(X0)
+            walls = []
+            for k in range(args.steps):
+                t0 = time.perf_counter()
+                st = (fs.solve if k == 0 else fs.solve_continue)((k + 1) * seg, prec=p, **opts)
+                walls.append(time.perf_counter() - t0)
+            wall = float(np.sum(walls))
+            ms_step = wall / args.steps * 1e3
+        else:
             fs.set_state(X0)
             t0 = time.perf_counter()
-            st = fs.solve(t_end, rtol=args.rtol, prec=p, steps_per_launch=64)
-            walls.append(time.perf_counter() - t0)
+            st = fs.solve(t_end, prec=p, **opts)
+            wall = time.perf_counter() - t0
+            ms_step = None
         ev = fs.events()
         t1 = first_onset(ev, 1.0)
-        runs[name] = dict(wall_s=float(np.mean(walls)), steps=st["steps"], rejects=st["rejects"],
+        runs[name] = dict(wall_s=wall, steps=st["steps"], rejects=st["rejects"],
                           rhs_evals=st["rhs_evals"], events=int(len(ev)),
                           first_recurrence_yr=t1, peak_slip_rate=peak_after(ev, t1),
-                          sim_years_per_hour=args.years / (np.mean(walls) / 3600.0))
-        log(f"cycle {name}: {args.years:g} yr in {np.mean(walls):.2f} s "
+                          sim_years_per_hour=args.years / (wall / 3600.0))
+        if ms_step is not None:
+            runs[name].update(segments=args.steps, ms_per_segment=ms_step)
+        log(f"cycle {name}: {args.years:g} yr in {wall:.2f} s "
             f"({runs[name]['sim_years_per_hour']:.3e} sim-yr/h), steps {st['steps']} "
             f"rejects {st['rejects']}, first event {t1:.4f} yr, peak v {runs[name]['peak_slip_rate']:.4f}")
     ref = runs.get("fp64")
@@ -295,7 +377,9 @@ def main():
             e["peak_slip_rate_rel"] = (r["peak_slip_rate"] - ref["peak_slip_rate"]) / ref["peak_slip_rate"]
         errors[name] = e
 
-    hname = HEADLINE if HEADLINE in runs else next(iter(runs))
+    if HEADLINE not in runs:
+        raise SystemExit(f"--precisions must include the headline {HEADLINE}")
+    hname = HEADLINE
     head = runs[hname]
     stg = kv[hname]
     traffic = STAGE_TRAFFIC[args.config].get(hname)
@@ -303,6 +387,14 @@ def main():
                 achieved=stg["stage_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=stg["stage_gbs"] / HBM_PEAK_GBS, traffic=traffic,
                 bytes_per_launch=stg["bytes"], launch_us=stg["stage_ms"] * 1e3)
+    # the FP32 product K v on its own (the storage the north-star K v figure
+    # names), HIP events on the handle's stream
+    k32 = kv["fp32"]
+    roof_kv32 = dict(kernel="k_matvec<float,2> (K v, FP32 operator)", bound="hbm",
+                     achieved=k32["matvec_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=k32["matvec_gbs"] / HBM_PEAK_GBS,
+                     traffic=MATVEC_TRAFFIC[args.config].get("fp32"),
+                     bytes_per_launch=k32["bytes"], launch_us=k32["matvec_ms"] * 1e3)
     out = {
         "metric": METRIC,
         "value": head["sim_years_per_hour"],
@@ -310,7 +402,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": head["wall_s"] * 1e3,
+        "ms_per_step": head["ms_per_segment"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -318,7 +410,9 @@ def main():
         "data": "synthetic (seeded fault and friction fields)",
         "config": {"workload": f"{args.config}: planar strike-slip fault 100x40 km, {args.ds:g} km "
                                f"patches, {n} patches x 2 slip components = {D} DOF, dense K "
-                               f"{D}x{D}, {args.years:g} yr, rk5dp rtol {args.rtol:g}",
+                               f"{D}x{D}, {args.years:g} yr, rk5dp rtol {args.rtol:g}; one step "
+                               f"= one of {args.steps} continued segments of "
+                               f"{args.years / args.steps:g} yr (together the whole cycle)",
                    "headline_precision": hname, "parallelism": "1 GPU"},
         "speedup_mixed_vs_fp64": (head["sim_years_per_hour"] / ref["sim_years_per_hour"]) if ref else None,
         "speedup_vs_fp64": ({k: r["sim_years_per_hour"] / ref["sim_years_per_hour"]
@@ -330,6 +424,7 @@ def main():
                          f_pair=F_PAIR, tflops=pairs / asm_ms * 1e3 * F_PAIR / 1e12,
                          frac_fp64_peak=pairs / asm_ms * 1e3 * F_PAIR / 1e12 / FP64_PEAK_TFLOPS),
         "roofline": roof,
+        "roofline_kv_fp32": roof_kv32,
     }
     if not args.no_cpu and ref is not None:
         K64 = fs.dense()
@@ -339,7 +434,55 @@ def main():
     fs.close()
     if args.config == "c3":
         out["assembly_rough_triangles"] = rough_triangle_assembly(args.ds)
+    if not args.no_oracle_c1:
+        out["errors_vs_oracle_c1"] = oracle_c1(args.rtol)
     print(json.dumps(out))
+
+
+def oracle_c1(rtol):
+    """c1 (2 km patches, 2000 DOF, 300 yr) on the device in every storage
+    format against the FP64 oracle's cycle on the host cores: onsets,
+    recurrence, peak slip rates and the log10 max|v| trace, scored as the
+    reference's rsf_accuracy.py does."""
+    import oracle
+    from interact_amd import accuracy, faults, lib
+    from interact_amd.model import FaultSystem, initial_state
+    g = faults.planar_grid(2.0)
+    f = faults.cycle_fields(g)
+    n = len(g["l"])
+    store = lib.PREC_FP32 | lib.PREC_BF16X2 | lib.PREC_BF16NF
+    fs = FaultSystem(g)
+    fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"], store=store)
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=f["v0"], f0=f["f0"], eta=f["eta"])
+    fs.set_loading(f["vpl"])
+    X0 = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
+    K = fs.dense()
+    xpl = np.zeros(2 * n)
+    xpl[0::2] = -f["vpl"]
+    threads = len(os.sched_getaffinity(0))
+    P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ xpl, ncomp=2, eta=f["eta"],
+                          v0=f["v0"], f0=f["f0"], nthreads=threads)
+    t_end = 300 * SEC_PER_YEAR
+    t0 = time.perf_counter()
+    ro = P.solve(X0, t_end, rtol=rtol)
+    t_or = time.perf_counter() - t0
+    out = dict(workload="c1: 100x40 km at 2 km patches, 1000 patches x 2 = 2000 DOF, 300 yr, "
+                        f"rk5dp rtol {rtol:g}; the device's FP64 operator in the oracle",
+               oracle=dict(wall_s=t_or, threads=threads, steps=ro["steps"],
+                           onsets_yr=[float(x) for x in accuracy.onsets(ro["events"])]))
+    for name in ("fp64", "fp32", "bf16x2", "bf16nf"):
+        fs.set_state(X0)
+        t0 = time.perf_counter()
+        st = fs.solve(t_end, rtol=rtol, prec=lib.PREC_BY_NAME[name], steps_per_launch=64)
+        wall = time.perf_counter() - t0
+        sc = accuracy.score(ro["monitor"], ro["events"], fs.monitor(), fs.events())
+        sc.update(steps=st["steps"], wall_s=wall)
+        out[name] = sc
+        log(f"c1 vs oracle, {name}: onsets {sc['donset']} yr, peaks {sc['dpeak']}, "
+            f"trace max {sc['maxdev']:.3g} rms {sc['rmsdev']:.3g} (log10 v), {wall:.2f} s")
+    log(f"c1 oracle cycle: {ro['steps']} steps in {t_or:.1f} s on {threads} threads")
+    fs.close()
+    return out
 
 
 ENS_OUTLINES = [(40, 20), (60, 24), (80, 32), (100, 40)]   # km

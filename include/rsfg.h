@@ -96,8 +96,14 @@ int rsfg_create(int device, const rsfg_patch *patches, int n,
                 rsfg_handle **h);
 int rsfg_destroy(rsfg_handle *h);
 
-/* Reads a reference patch file (rectangles; format above). On entry *n is
-   the capacity of patches; on return the number read. */
+/* Reads a reference patch file (read_geometry, src/read_geometry.c:115-117;
+   rectangles, and triangles with L, W < 0 followed by their nine vertex
+   coordinates, :148-215).  On entry *n is the capacity of patches; on
+   return the number of patches the file holds.  patches may be NULL (with
+   any capacity) to count them.  A file holding more patches than the
+   capacity returns RSFG_EINVAL after storing the first *n-in patches, with
+   *n set to the full count, so the caller can retry with enough room (the
+   reference reads every patch). */
 int rsfg_read_patch_file(const char *fname, rsfg_patch *patches, int *n);
 
 /* A handle that shares geom's patches and assembled operators (read only)
@@ -141,7 +147,10 @@ int rsfg_operator_bytes(rsfg_handle *h, int prec, double *bytes);
 
 /* GenKEntries_petsc semantics (src/interact_matrix_assembly.c:102): the
    m x k block of the assembled FP64 shear operator with rows[] and cols[]
-   (0-based), written column-major into out (out[r + m*c]). */
+   (0-based), written column-major into out (out[r + m*c]).  With the
+   normal rows assembled, rows ncomp*n .. ncomp*n + n - 1 address the "In"
+   operator (row ncomp*n + p: normal traction at patch p, scaled by
+   -scale, src/rsf_solve.c:245-247). */
 int rsfg_entries(rsfg_handle *h, int m, const int *rows, int k,
                  const int *cols, double *out);
 
@@ -222,6 +231,8 @@ typedef struct {
   long max_steps;
   int max_domain_reject;            /* storm guard (<= 0 off) */
   int steps_per_launch;             /* accepted-step attempts per graph */
+  int max_events;       /* rows of the event log (<= 0: 4096) */
+  int max_monitor;      /* rows of the monitor log (<= 0: 262144) */
 } rsfg_solver_opts;
 
 /* fills o with the reference defaults (src/rsf_init.c:331-350) */
@@ -259,11 +270,29 @@ typedef struct {
   int status;           /* 0 ok, RSFG_ESTORM, RSFG_EDOMAIN */
   int nevent, nmonitor; /* rows available */
   double wall_s;        /* host wall time of the solve call */
+  int events_dropped;   /* event / monitor rows past the logs' capacity */
+  int monitor_dropped;  /*   (max_events, max_monitor): counted, not stored */
 } rsfg_stats;
 
 /* integrates from the current state to t_end (STEPOVER semantics) */
 int rsfg_solve(rsfg_handle *h, const rsfg_solver_opts *o, double t_end,
                rsfg_stats *st);
+/* continues the last rsfg_solve (or rsfg_solve_continue) of this handle to
+   a later t_end.  It resumes from the true state at the last accepted step
+   (STEPOVER, src/rsf_solve.c:506), with the next step size kept as a
+   checkpoint restart keeps it (:634-640), never from an interpolated
+   state (the drift the reference warns of at :622-632): the step size, the
+   FSAL derivative, the event state (slipping flag, onset snapshots) and the
+   monitor reference are kept, event, monitor and step-log rows are
+   appended, and the accepted steps are exactly those of one rsfg_solve to
+   the final t_end.  o->method and o->prec must be those of the solve being
+   continued (o->dt_init is unused); any call that changes or reuses the
+   solver buffers in between (rsfg_set_state, rsfg_set_friction,
+   rsfg_set_loading, rsfg_rhs, rsfg_stage_bench, the adjoint calls, a
+   checkpoint read) returns RSFG_ESTATE here.  Statistics count from the
+   first solve. */
+int rsfg_solve_continue(rsfg_handle *h, const rsfg_solver_opts *o, double t_end,
+                        rsfg_stats *st);
 
 /* ---- ensembles ---------------------------------------------------------
    Batched runs over shared operators (no counterpart in the reference,

@@ -28,13 +28,33 @@ def hipcc():
     return "hipcc"
 
 
-def build(verbose=False):
+def build(verbose=False, jobs=None):
+    """Compiles each source to an object in parallel (no kernel is launched
+    outside the translation unit that defines it), then links the library."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + srcs + ["-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
+    with tempfile.TemporaryDirectory(prefix="rsfg_build_") as d:
+        objs = [os.path.join(d, os.path.basename(s) + ".o") for s in srcs]
+        cflags = [f for f in HIPCC_FLAGS if f != "-shared"]
+        cmds = [[hipcc()] + cflags + ["-c", s, "-o", o] for s, o in zip(srcs, objs)]
+        if verbose:
+            for c in cmds:
+                print(" ".join(c), flush=True)
+        jobs = jobs or min(len(cmds), os.cpu_count() or 1)
+        with ThreadPoolExecutor(jobs) as ex:
+            res = list(ex.map(lambda c: subprocess.run(c, capture_output=True, text=True), cmds))
+        for c, r in zip(cmds, res):
+            if r.stderr and (verbose or r.returncode):
+                sys.stderr.write("".join(l + "\n" for l in r.stderr.splitlines()
+                                         if "packed-fp32-ops" not in l))
+            if r.returncode:
+                raise subprocess.CalledProcessError(r.returncode, c)
+        link = [hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}"] + objs + ["-o", tmp]
+        if verbose:
+            print(" ".join(link), flush=True)
+        subprocess.run(link, check=True)
     os.replace(tmp, LIB)
     return LIB
 

@@ -842,6 +842,7 @@ extern "C" {
 
 int rsfg_replay(rsfg_handle* h, const double* steps, long long nsteps,
                 const rsfg_adjoint_problem* pb, double* J, double* trace, double* X_end) {
+  if (h) h->cont_ok = false;
   int rc = adj_check(h, steps, nsteps, pb, "rsfg_replay");
   if (rc) return rc;
   RSFG_HIP(hipSetDevice(h->device));
@@ -875,6 +876,7 @@ int rsfg_replay(rsfg_handle* h, const double* steps, long long nsteps,
 int rsfg_adjoint_gradient(rsfg_handle* h, const double* steps, long long nsteps,
                           const rsfg_adjoint_problem* pb, double* J, double* grad_a,
                           double* grad_b, double* grad_dc) {
+  if (h) h->cont_ok = false;
   int rc = adj_check(h, steps, nsteps, pb, "rsfg_adjoint_gradient");
   if (rc) return rc;
   RSFG_CHECK(pb->nrecv > 0 && pb->target && pb->weight, RSFG_EINVAL,
@@ -904,6 +906,7 @@ int rsfg_adjoint_gradient(rsfg_handle* h, const double* steps, long long nsteps,
 }
 
 int rsfg_adjoint_bench(rsfg_handle* h, int prec, int nrep, double* ms) {
+  if (h) h->cont_ok = false;
   RSFG_CHECK(h && ms && nrep > 0, RSFG_EINVAL, "rsfg_adjoint_bench: bad arguments");
   RSFG_CHECK(h->assembled && h->G == h->ncomp, RSFG_ESTATE,
              "rsfg_adjoint_bench: needs an assembled operator without normal rows");

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <type_traits>
 #include <cstdio>
@@ -30,7 +31,8 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line) {
 int launch_assemble(rsfg_ctx* h, double* gpu_ms);
 int launch_pairs(rsfg_ctx* h, long np, const int* d_recv, const int* d_src, int full,
                  double* d_out);
-int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st);
+int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st,
+                 bool cont);
 int stage_bench(rsfg_ctx* h, int prec, int nrep, double* ms);
 
 constexpr double kDeg2Rad = 0.0174532925199432957692369076848861271344287188854172545609719144;
@@ -120,6 +122,8 @@ static void free_operator(rsfg_ctx* h) {
 }
 
 static void free_state(rsfg_ctx* h) {
+  drop_graph(h);
+  h->cont_ok = false;
   hipFree(h->d_state);
   hipFree(h->partials);
   hipFree(h->d_ev);
@@ -130,6 +134,34 @@ static void free_state(rsfg_ctx* h) {
   h->partials = nullptr;
   h->d_ev = h->d_mon = h->d_cat = nullptr;
   h->d_rupt = nullptr;
+  h->ev_cap = h->mon_cap = 0;
+}
+
+int ensure_logs(rsfg_ctx* h, int ev_rows, int mon_rows) {
+  // grown logs keep the rows already written (a continued solve appends)
+  if (ev_rows > h->ev_cap) {
+    double* p = nullptr;
+    RSFG_HIP(hipMalloc(&p, (size_t)ev_rows * 8 * sizeof(double)));
+    if (h->d_ev && h->ev_cap > 0)
+      RSFG_HIP(hipMemcpyAsync(p, h->d_ev, (size_t)h->ev_cap * 8 * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+    RSFG_HIP(hipStreamSynchronize(h->stream));
+    hipFree(h->d_ev);
+    h->d_ev = p;
+    h->ev_cap = ev_rows;
+  }
+  if (mon_rows > h->mon_cap) {
+    double* p = nullptr;
+    RSFG_HIP(hipMalloc(&p, (size_t)mon_rows * 10 * sizeof(double)));
+    if (h->d_mon && h->mon_cap > 0)
+      RSFG_HIP(hipMemcpyAsync(p, h->d_mon, (size_t)h->mon_cap * 10 * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+    RSFG_HIP(hipStreamSynchronize(h->stream));
+    hipFree(h->d_mon);
+    h->d_mon = p;
+    h->mon_cap = mon_rows;
+  }
+  return RSFG_OK;
 }
 
 int alloc_state(rsfg_ctx* h) {
@@ -152,8 +184,10 @@ int alloc_state(rsfg_ctx* h) {
   for (int i = 0; i < 2; ++i) { h->vxb[i] = p; p += h->vin_elems; }
   h->partial_count = (size_t)(n + 1) * kNRed;
   RSFG_HIP(hipMalloc(&h->partials, h->partial_count * sizeof(double)));
-  RSFG_HIP(hipMalloc(&h->d_ev, (size_t)kMaxEv * 8 * sizeof(double)));
-  RSFG_HIP(hipMalloc(&h->d_mon, (size_t)kMaxMon * 10 * sizeof(double)));
+  {
+    const int rc = ensure_logs(h, kMaxEv, kMaxMon);
+    if (rc != RSFG_OK) return rc;
+  }
   RSFG_HIP(hipMalloc(&h->d_cat, 2 * n * sizeof(double)));
   RSFG_HIP(hipMalloc(&h->d_rupt, n * sizeof(int)));
   h->xcur = 0;
@@ -483,15 +517,16 @@ int rsfg_device_count(int* count) {
 }
 
 int rsfg_read_patch_file(const char* fname, rsfg_patch* patches, int* n) {
-  RSFG_CHECK(fname && patches && n, RSFG_EINVAL, "rsfg_read_patch_file: null argument");
+  RSFG_CHECK(fname && n && *n >= 0, RSFG_EINVAL, "rsfg_read_patch_file: bad arguments");
   FILE* in = fopen(fname, "r");
   RSFG_CHECK(in != nullptr, RSFG_EINVAL, std::string("cannot open ") + fname);
+  const int cap = patches ? *n : 0;
   int k = 0;
   rsfg_patch p;
   memset(&p, 0, sizeof p);
   int bad = 0;
-  while (k < *n && fscanf(in, "%lf %lf %lf %f %f %lf %lf %i", &p.x, &p.y, &p.z, &p.strike,
-                          &p.dip, &p.l, &p.w, &p.group) == 8) {
+  while (fscanf(in, "%lf %lf %lf %f %f %lf %lf %i", &p.x, &p.y, &p.z, &p.strike, &p.dip, &p.l,
+                &p.w, &p.group) == 8) {
     p.rake[0] = 1.0;
     p.rake[1] = 0.0;
     for (int j = 0; j < 9; ++j) p.vert[j] = 0.0;
@@ -506,13 +541,26 @@ int rsfg_read_patch_file(const char* fname, rsfg_patch* patches, int* n) {
       bad = 2;   // point sources, 2-D segments and irregular quads are not supported
       break;
     }
-    patches[k++] = p;
+    if (k < cap) patches[k] = p;
+    if (k == INT_MAX) {
+      bad = 3;
+      break;
+    }
+    ++k;
   }
+  // as in the reference, reading ends at the first line that does not parse
+  // as a patch
   fclose(in);
   *n = k;
-  RSFG_CHECK(bad != 1, RSFG_EINVAL, "rsfg_read_patch_file: truncated triangle line");
+  RSFG_CHECK(bad != 1, RSFG_EINVAL,
+             "rsfg_read_patch_file: truncated triangle line (patch " + std::to_string(k) + ")");
   RSFG_CHECK(bad != 2, RSFG_EINVAL,
-             "rsfg_read_patch_file: only rectangles and triangles are supported");
+             "rsfg_read_patch_file: only rectangles and triangles are supported (patch " +
+                 std::to_string(k) + ")");
+  RSFG_CHECK(bad != 3, RSFG_EINVAL, "rsfg_read_patch_file: too many patches");
+  RSFG_CHECK(!patches || k <= cap, RSFG_EINVAL,
+             std::string("rsfg_read_patch_file: ") + fname + " holds " + std::to_string(k) +
+                 " patches, more than the capacity " + std::to_string(cap));
   return RSFG_OK;
 }
 
@@ -673,6 +721,7 @@ int rsfg_destroy(rsfg_handle* h) {
   hipFree(h->d_ctl);
   hipFree(h->d_tmp);
   hipFree(h->d_hlog);
+  drop_graph(h);
   if (h->h_ctl) hipHostFree(h->h_ctl);
   if (h->ev0) hipEventDestroy(h->ev0);
   if (h->ev1) hipEventDestroy(h->ev1);
@@ -762,10 +811,12 @@ int rsfg_entries(rsfg_handle* h, int m, const int* rows, int k, const int* cols,
   RSFG_CHECK(m >= 0 && k >= 0 && rows && cols && out, RSFG_EINVAL, "rsfg_entries: bad args");
   if (m == 0 || k == 0) return RSFG_OK;
   const int nshear = h->ncomp * h->n;
+  const int nrows = nshear + (h->with_normal ? h->n : 0);
   std::vector<int> rr(m), cc(k);
   for (int i = 0; i < m; ++i) {
-    RSFG_CHECK(rows[i] >= 0 && rows[i] < nshear, RSFG_EINVAL, "rsfg_entries: row out of range");
-    rr[i] = shear_row(h, rows[i]);
+    RSFG_CHECK(rows[i] >= 0 && rows[i] < nrows, RSFG_EINVAL, "rsfg_entries: row out of range");
+    // rows past the shear rows address the normal (In) rows, patch by patch
+    rr[i] = rows[i] < nshear ? shear_row(h, rows[i]) : (rows[i] - nshear) * h->G + h->ncomp;
   }
   for (int j = 0; j < k; ++j) {
     RSFG_CHECK(cols[j] >= 0 && cols[j] < h->cols, RSFG_EINVAL, "rsfg_entries: col out of range");
@@ -963,6 +1014,7 @@ int rsfg_set_loading(rsfg_handle* h, double vpl, const double* vpl_patch, const 
                      const double* sigdot_add) {
   RSFG_CHECK(h && h->have_friction, RSFG_ESTATE, "rsfg_set_loading: set friction first");
   RSFG_HIP(hipSetDevice(h->device));
+  h->cont_ok = false;
   const int n = h->n, nc = h->ncomp;
   // backslip: slip rate -vpl along the slip direction (strike component in
   // the two-component mode), src/rsf_solve.c:268-331
@@ -1012,6 +1064,7 @@ int rsfg_set_state(rsfg_handle* h, double t, const double* X) {
   std::vector<double> soa((size_t)D * n);
   for (int p = 0; p < n; ++p)
     for (int f = 0; f < D; ++f) soa[(size_t)f * n + p] = X[(size_t)p * D + f];
+  h->cont_ok = false;
   h->xcur = 0;
   RSFG_HIP(hipMemcpy(h->Xb[0], soa.data(), sizeof(double) * D * n, hipMemcpyHostToDevice));
   h->t = t;
@@ -1183,6 +1236,7 @@ int rsfg_rhs(rsfg_handle* h, int prec, const double* X, double* F, int* domain_o
              "rsfg_rhs: set friction and loading first");
   RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_rhs: precision not assembled");
   RSFG_HIP(hipSetDevice(h->device));
+  h->cont_ok = false;
   const int n = h->n, D = h->dim, nc = h->ncomp;
   std::vector<double> soa((size_t)D * n);
   for (int p = 0; p < n; ++p)
@@ -1279,7 +1333,14 @@ int rsfg_solve(rsfg_handle* h, const rsfg_solver_opts* o, double t_end, rsfg_sta
                             hipMemcpyDeviceToDevice, h->stream));
     h->xcur = 0;
   }
-  return solve_device(h, o, t_end, st);
+  return solve_device(h, o, t_end, st, false);
+}
+
+int rsfg_solve_continue(rsfg_handle* h, const rsfg_solver_opts* o, double t_end,
+                        rsfg_stats* st) {
+  RSFG_CHECK(h && o, RSFG_EINVAL, "rsfg_solve_continue: null argument");
+  RSFG_HIP(hipSetDevice(h->device));
+  return solve_device(h, o, t_end, st, true);
 }
 
 int rsfg_stage_bench(rsfg_handle* h, int prec, int nrep, double* ms) {

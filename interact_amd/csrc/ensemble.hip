@@ -830,6 +830,9 @@ static int fill_member_args(rsfg_ensemble* e, const rsfg_solver_opts* o) {
       RSFG_CHECK(h->have_friction && h->have_loading && h->have_state, RSFG_ESTATE,
                  "rsfg_ensemble: member " + std::to_string(m) +
                      " needs friction, loading and a state");
+      const int rc = ensure_logs(h, o->max_events > 0 ? o->max_events : kMaxEv,
+                                 o->max_monitor > 0 ? o->max_monitor : kMaxMon);
+      if (rc != RSFG_OK) return rc;
       fill_stage_args(h, o, e->geo[g].K, args[m]);
       args[m].vin = e->geo[g].V[0] + (size_t)k * e->geo[g].ld;   // initial evaluation
       args[m].vout = e->geo[g].V[1] + (size_t)k * e->geo[g].ld;  // first stage of an attempt
@@ -873,6 +876,7 @@ static int max_patches(rsfg_ensemble* e) {
 static int ens_start(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_end) {
   for (int m = 0; m < e->M; ++m) {
     rsfg_ctx* h = e->members[m];
+    h->cont_ok = false;
     DevCtl* c = h->h_ctl;
     memset(c, 0, sizeof(*c));
     c->t = h->t;
@@ -881,8 +885,8 @@ static int ens_start(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_end) 
     c->hmax = (o->dt_max < o->dt_monitor) ? o->dt_max : o->dt_monitor;
     c->max_steps = o->max_steps;
     for (int i = 0; i < 8; ++i) c->kslot[i] = i;
-    c->max_event = kMaxEv;
-    c->max_mon = kMaxMon;
+    c->max_event = std::min(o->max_events > 0 ? o->max_events : kMaxEv, h->ev_cap);
+    c->max_mon = std::min(o->max_monitor > 0 ? o->max_monitor : kMaxMon, h->mon_cap);
     c->xcur = h->xcur;
     RSFG_HIP(hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, e->stream));
     RSFG_HIP(hipMemsetAsync(h->d_rupt, 0, sizeof(int) * h->n, e->stream));
@@ -1058,12 +1062,24 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
   const int per = o->steps_per_launch > 0 ? o->steps_per_launch : 32;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  RSFG_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < per; ++k) ens_attempt(e, tb.s);
-  hipLaunchKernelGGL(k_halted_b, dim3((e->M + 255) / 256), dim3(256), 0, e->stream,
-                     ens_args(e, 0), e->M, e->d_halted);
-  RSFG_HIP(hipStreamEndCapture(e->stream, &graph));
-  RSFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  {
+    // the capture is ended whatever the launches returned, so the stream
+    // never stays in capture mode
+    hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
+    if (he == hipSuccess) {
+      for (int k = 0; k < per; ++k) ens_attempt(e, tb.s);
+      hipLaunchKernelGGL(k_halted_b, dim3((e->M + 255) / 256), dim3(256), 0, e->stream,
+                         ens_args(e, 0), e->M, e->d_halted);
+      const hipError_t el = hipGetLastError();
+      he = hipStreamEndCapture(e->stream, &graph);
+      if (he == hipSuccess) he = el;
+    }
+    if (he == hipSuccess) he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (he != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      return hip_fail(he, "rsfg_ensemble_solve: capture", __FILE__, __LINE__);
+    }
+  }
   std::vector<long long> hs(2 * e->M, 0), prev(e->M, -1);
   std::vector<char> running(e->ngeo, 1);
   const bool sk = kEnsCfgs[e->cfg].sk > 0;
@@ -1114,7 +1130,9 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
           for (int k = 0; k < per; ++k) ens_attempt(e, tb.s);
           hipLaunchKernelGGL(k_halted_b, dim3((e->M + 255) / 256), dim3(256), 0, e->stream,
                              ens_args(e, 0), e->M, e->d_halted);
+          const hipError_t el = hipGetLastError();
           he = hipStreamEndCapture(e->stream, &graph);
+          if (he == hipSuccess) he = el;
         }
         if (he == hipSuccess) he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (he != hipSuccess) {
@@ -1155,6 +1173,8 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
       q.status = c->status == 2 ? RSFG_OK : c->status;
       q.nevent = std::min(c->nevent, c->max_event);
       q.nmonitor = std::min(c->nmon, c->max_mon);
+      q.events_dropped = c->ev_drop;
+      q.monitor_dropped = c->mon_drop;
       q.wall_s = wall;
     }
   }

@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/rsfg.h"
 #include "rsf_dev.h"
@@ -68,8 +69,9 @@ struct DevCtl {
   int rupt_update;      // next prep marks ruptured cells
   int storm;
   int imex_g1;          // additive pair: the explicit part at X is stale
-  int nevent, nmon;
+  int nevent, nmon;     // rows stored (at most max_event, max_mon)
   int max_event, max_mon;
+  int ev_drop, mon_drop;  // rows past the capacity: counted, not stored
   double* hlog;         // accepted step sizes, in order (optional)
   long long hlog_cap;
 };
@@ -149,6 +151,16 @@ struct rsfg_ctx {
   rsfg::DevCtl* d_ctl = nullptr;
   rsfg::DevCtl* h_ctl = nullptr;  // pinned
   int nevent = 0, nmon = 0;
+  int ev_cap = 0, mon_cap = 0;    // rows allocated in d_ev, d_mon
+  // the last solve can be continued (rsfg_solve_continue) with this method
+  // and operator precision; cleared by every call that touches the buffers
+  bool cont_ok = false;
+  int cont_method = -1, cont_prec = 0;
+  // captured graph of a batch of step attempts, reused while its argument
+  // block (gkey) and batch size are unchanged
+  hipGraph_t ggraph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> gkey;
   // accepted step sizes of the last solve (rsfg_set_step_log)
   double* d_hlog = nullptr;
   long long hlog_cap = 0;
@@ -176,4 +188,8 @@ enum {
 
 int ensure_tmp(rsfg_ctx* h, size_t bytes);
 int alloc_state(rsfg_ctx* h);
+// event and monitor logs of at least ev_rows and mon_rows rows
+int ensure_logs(rsfg_ctx* h, int ev_rows, int mon_rows);
+// drops the cached attempt graph (buffers moved or handle destroyed)
+void drop_graph(rsfg_ctx* h);
 }  // namespace rsfg

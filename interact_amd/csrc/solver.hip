@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -566,8 +567,8 @@ static int start_solve(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, Sta
   c->hmax = (o->dt_max < o->dt_monitor) ? o->dt_max : o->dt_monitor;
   c->max_steps = o->max_steps;
   for (int i = 0; i < 8; ++i) c->kslot[i] = i;
-  c->max_event = kMaxEv;
-  c->max_mon = kMaxMon;
+  c->max_event = o->max_events > 0 ? o->max_events : kMaxEv;
+  c->max_mon = o->max_monitor > 0 ? o->max_monitor : kMaxMon;
   c->xcur = h->xcur;
   c->hlog = h->d_hlog;
   c->hlog_cap = h->hlog_cap;
@@ -603,7 +604,51 @@ static int start_solve(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, Sta
   return RSFG_OK;
 }
 
-int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st) {
+void drop_graph(rsfg_ctx* h) {
+  if (h->gexec) hipGraphExecDestroy(h->gexec);
+  if (h->ggraph) hipGraphDestroy(h->ggraph);
+  h->gexec = nullptr;
+  h->ggraph = nullptr;
+  h->gkey.clear();
+}
+
+// Captures one batch of step attempts into h's cached graph unless the
+// cached one was captured from the same argument block and batch size.  On
+// a failure the capture is ended and nothing is cached.
+template <typename F>
+static int cached_graph(rsfg_ctx* h, const StageArgs& a, int per, F enqueue) {
+  std::vector<unsigned char> key(sizeof(StageArgs) + sizeof(int));
+  memcpy(key.data(), &a, sizeof(StageArgs));
+  memcpy(key.data() + sizeof(StageArgs), &per, sizeof(int));
+  if (h->gexec && key == h->gkey) return RSFG_OK;
+  drop_graph(h);
+  hipError_t e = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(e, "rsfg_solve: begin capture", __FILE__, __LINE__);
+  enqueue();
+  hipGraph_t g = nullptr;
+  // end the capture whatever the launches returned, so the stream leaves
+  // capture mode
+  const hipError_t el = hipGetLastError();
+  e = hipStreamEndCapture(h->stream, &g);
+  if (e == hipSuccess) e = el;
+  hipGraphExec_t x = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (g) hipGraphDestroy(g);
+    return hip_fail(e, "rsfg_solve: capture of the attempt graph", __FILE__, __LINE__);
+  }
+  h->ggraph = g;
+  h->gexec = x;
+  h->gkey.swap(key);
+  return RSFG_OK;
+}
+
+// cont = false: a solve from the handle's current (t, X); cont = true: the
+// continuation of the last solve (rsfg_solve_continue), which keeps the
+// controller block, the FSAL derivative, the monitor reference and the
+// event state and appends to the logs
+int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stats* st,
+                 bool cont) {
   const auto wall0 = std::chrono::steady_clock::now();
   const void* K = operator_ptr(h, o->prec);
   RSFG_CHECK(K != nullptr, RSFG_ESTATE, "rsfg_solve: operator precision not assembled");
@@ -613,6 +658,24 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
   RSFG_CHECK(o->method != RSFG_ARKIMEX_L2 || h->ncomp == 1, RSFG_EINVAL,
              "rsfg_solve: the IMEX method is defined for the scalar slip layout "
              "(RSFG_SLIP_STRIKE / DIP / RAKE), as in the reference");
+  RSFG_CHECK(o->max_events <= (1 << 24) && o->max_monitor <= (1 << 26), RSFG_EINVAL,
+             "rsfg_solve: max_events at most 2^24 and max_monitor at most 2^26 rows");
+  if (cont) {
+    RSFG_CHECK(h->cont_ok, RSFG_ESTATE,
+               "rsfg_solve_continue: no solve to continue (none yet, or the state, "
+               "friction, loading or solver buffers changed since)");
+    RSFG_CHECK(o->method == h->cont_method && o->prec == h->cont_prec, RSFG_EINVAL,
+               "rsfg_solve_continue: method and precision must be those of the solve "
+               "being continued");
+  }
+  h->cont_ok = false;
+  const int ev_rows = o->max_events > 0 ? o->max_events : kMaxEv;
+  const int mon_rows = o->max_monitor > 0 ? o->max_monitor : kMaxMon;
+  {
+    // a continuation keeps the rows already logged (ensure_logs copies them)
+    const int rc0 = ensure_logs(h, ev_rows, mon_rows);
+    if (rc0 != RSFG_OK) return rc0;
+  }
   Launcher* L = make_launcher(o->prec, h->ncomp, h->G);
   if ((size_t)L->wg_count(h->n) * kNRed > h->partial_count) {
     delete L;
@@ -622,7 +685,27 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
   StageArgs a;
   fill_stage_args(h, o, K, a);
   DevCtl* c = h->h_ctl;
-  {
+  if (cont) {
+    // h_ctl holds the block as the last batch left it (the halted solve
+    // changes nothing after halting)
+    RSFG_CHECK(c->max_event <= ev_rows && c->max_mon <= mon_rows, RSFG_EINVAL,
+               "rsfg_solve_continue: the logs cannot shrink below the continued solve's");
+    c->t_end = t_end;
+    c->hmax = (o->dt_max < o->dt_monitor) ? o->dt_max : o->dt_monitor;
+    c->max_event = ev_rows;
+    c->max_mon = mon_rows;
+    c->max_steps = (o->max_steps > LLONG_MAX - c->steps) ? LLONG_MAX : c->steps + o->max_steps;
+    c->halted = (c->t < t_end) ? 0 : 1;
+    c->status = 0;
+    c->hlog = h->d_hlog;
+    c->hlog_cap = h->hlog_cap;
+    const hipError_t e =
+        hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, h->stream);
+    if (e != hipSuccess) {
+      delete L;
+      return hip_fail(e, "rsfg_solve_continue: control block", __FILE__, __LINE__);
+    }
+  } else {
     const int rc0 = start_solve(h, o, t_end, a, L);
     if (rc0 != RSFG_OK) {
       delete L;
@@ -666,19 +749,12 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
       L->ctrl(b, h->stream);
     }
   };
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  if (use_graph) {
-    RSFG_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_batch();
-    RSFG_HIP(hipStreamEndCapture(h->stream, &graph));
-    RSFG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  }
-  long launches = 0;
   int rc = RSFG_OK;
-  while (!c->halted) {
+  if (use_graph && !c->halted) rc = cached_graph(h, a, per, enqueue_batch);
+  long launches = 0;
+  while (rc == RSFG_OK && !c->halted) {
     hipError_t e = hipSuccess;
-    if (use_graph) e = hipGraphLaunch(exec, h->stream);
+    if (use_graph) e = hipGraphLaunch(h->gexec, h->stream);
     else {
       enqueue_batch();
       e = hipGetLastError();
@@ -691,8 +767,6 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
     }
     ++launches;
   }
-  if (exec) hipGraphExecDestroy(exec);
-  if (graph) hipGraphDestroy(graph);
   delete L;
   if (rc != RSFG_OK) return rc;
   h->xcur = c->xcur;
@@ -700,6 +774,7 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
   h->nevent = c->nevent;
   h->nmon = c->nmon;
   if (st) {
+    memset(st, 0, sizeof(*st));
     st->steps = c->steps;
     st->rejects = c->rejects;
     st->domain_rejects = c->domain_rejects;
@@ -708,6 +783,11 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
     st->t = c->t;
     st->dt = c->h;
     st->status = c->status == 2 ? RSFG_OK : c->status;
+    st->nevent = c->nevent;
+    st->nmonitor = c->nmon;
+    st->events_dropped = c->ev_drop;
+    st->monitor_dropped = c->mon_drop;
+    st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
   }
   if (c->status == RSFG_ESTORM) {
     set_error("rsfg_solve: rejection storm (consecutive out-of-domain trial steps)");
@@ -717,11 +797,10 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
     set_error("rsfg_solve: step size underflow or non-finite time");
     return RSFG_EDOMAIN;
   }
-  if (st) {
-    st->nevent = c->nevent;
-    st->nmonitor = c->nmon;
-    st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
-  }
+  // a solve that ended at t_end or at max_steps can be continued
+  h->cont_ok = true;
+  h->cont_method = o->method;
+  h->cont_prec = o->prec;
   return RSFG_OK;
 }
 
@@ -735,6 +814,7 @@ int stage_bench(rsfg_ctx* h, int prec, int nrep, double* ms) {
   rsfg_solver_opts o;
   rsfg_solver_defaults(&o);
   o.prec = prec;
+  h->cont_ok = false;
   Launcher* L = make_launcher(prec, h->ncomp, h->G);
   StageArgs a;
   fill_stage_args(h, &o, K, a);

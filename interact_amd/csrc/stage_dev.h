@@ -220,6 +220,8 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
         const double mv = red[RED_VSUM] / n;
         row[9] = log10(mv > 1e-300 ? mv : 1e-300);
         c->nmon++;
+      } else {
+        c->mon_drop++;
       }
       c->t_mon = c->t;
       c->mon_copy = 1;
@@ -383,6 +385,8 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
         row[6] = cnt > 0.0 ? drop / cnt : 0.0;
         row[7] = cnt;
         c->nevent++;
+      } else {
+        c->ev_drop++;
       }
       c->vprev = vm;
     }
@@ -429,6 +433,8 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
         const double mv = red[RED_VSUM] / n;
         row[9] = log10(mv > 1e-300 ? mv : 1e-300);
         c->nmon++;
+      } else {
+        c->mon_drop++;
       }
       c->t_mon = c->t;
       c->mon_copy = 1;

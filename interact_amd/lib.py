@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "librsfg.so")
 
 RSFG_OK = 0
+RSFG_EINVAL, RSFG_ENOMEM, RSFG_EHIP, RSFG_ESTATE, RSFG_ESTORM, RSFG_EDOMAIN = -1, -2, -3, -4, -5, -6
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ESTATE", -5: "ESTORM",
           -6: "EDOMAIN"}
 
@@ -33,6 +34,7 @@ EXPORTS = [
     "rsfg_matvec", "rsfg_matvec_dev", "rsfg_matvec_t", "rsfg_matvec_bench",
     "rsfg_set_friction", "rsfg_set_loading", "rsfg_rhs",
     "rsfg_solver_defaults", "rsfg_set_state", "rsfg_get_state", "rsfg_solve",
+    "rsfg_solve_continue",
     "rsfg_events", "rsfg_monitor", "rsfg_stage_bench",
     "rsfg_read_table", "rsfg_write_checkpoint", "rsfg_read_checkpoint",
     "rsfg_create_member", "rsfg_ensemble_create", "rsfg_ensemble_member",
@@ -72,7 +74,8 @@ class SolverOpts(C.Structure):
                 ("vel_event", C.c_double), ("vel_event_hyst", C.c_double),
                 ("rupture_vth", C.c_double), ("max_steps", C.c_long),
                 ("max_domain_reject", C.c_int),
-                ("steps_per_launch", C.c_int)]
+                ("steps_per_launch", C.c_int), ("max_events", C.c_int),
+                ("max_monitor", C.c_int)]
 
 
 class AdjointProblem(C.Structure):
@@ -86,7 +89,8 @@ class Stats(C.Structure):
                 ("domain_rejects", C.c_long), ("rhs_evals", C.c_long),
                 ("launches", C.c_long), ("t", C.c_double), ("dt", C.c_double),
                 ("status", C.c_int), ("nevent", C.c_int),
-                ("nmonitor", C.c_int), ("wall_s", C.c_double)]
+                ("nmonitor", C.c_int), ("wall_s", C.c_double),
+                ("events_dropped", C.c_int), ("monitor_dropped", C.c_int)]
 
 
 _LIB = None

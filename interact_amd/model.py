@@ -115,6 +115,12 @@ class FaultSystem:
         r, c, _ = self.dims()
         return self.entries(np.arange(r), np.arange(c))
 
+    def dense_normal(self):
+        """the assembled normal-traction ("In") operator, n x cols (needs
+        with_normal at assembly)"""
+        r, c, _ = self.dims()
+        return self.entries(r + np.arange(self.n), np.arange(c))
+
     def traction_pairs(self, recv, src, full_space=False):
         recv = np.ascontiguousarray(recv, np.int32)
         src = np.ascontiguousarray(src, np.int32)
@@ -295,6 +301,16 @@ class FaultSystem:
                 "rsfg_solve")
         return {k: getattr(st, k) for k, _ in L.Stats._fields_}
 
+    def solve_continue(self, t_end, **kw):
+        """Continues the last solve to t_end (rsfg_solve_continue): the same
+        accepted steps as one solve to the final t_end; method and prec must
+        be those of the continued solve."""
+        o = self.solver_options(**kw)
+        st = L.Stats()
+        L.check(self._lib.rsfg_solve_continue(self.h, C.byref(o), t_end, C.byref(st)),
+                "rsfg_solve_continue")
+        return {k: getattr(st, k) for k, _ in L.Stats._fields_}
+
     def stage_bench(self, prec=L.PREC_FP64, nrep=50):
         ms = C.c_double()
         L.check(self._lib.rsfg_stage_bench(self.h, prec, nrep, C.byref(ms)),
@@ -398,6 +414,26 @@ class Ensemble:
             self.close()
         except Exception:
             pass
+
+
+def read_patch_file(fname):
+    """Patches of a reference geometry file (rsfg_read_patch_file): a dict
+    of arrays xyz, strike, dip (float32, as read), l, w, group, vert (nine
+    vertex coordinates of triangles, zeros for rectangles)."""
+    lib = L.load()
+    n = C.c_int(0)
+    L.check(lib.rsfg_read_patch_file(os.fsencode(fname), None, C.byref(n)),
+            "rsfg_read_patch_file")
+    arr = (L.Patch * max(n.value, 1))()
+    L.check(lib.rsfg_read_patch_file(os.fsencode(fname), arr, C.byref(n)),
+            "rsfg_read_patch_file")
+    k = n.value
+    return dict(xyz=np.array([[p.x, p.y, p.z] for p in arr[:k]]).reshape(k, 3),
+                strike=np.array([p.strike for p in arr[:k]], np.float32),
+                dip=np.array([p.dip for p in arr[:k]], np.float32),
+                l=np.array([p.l for p in arr[:k]]), w=np.array([p.w for p in arr[:k]]),
+                group=np.array([p.group for p in arr[:k]], np.int32),
+                vert=np.array([list(p.vert) for p in arr[:k]]).reshape(k, 9))
 
 
 def read_table(fname, n, ncols):

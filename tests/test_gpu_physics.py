@@ -1,0 +1,258 @@
+"""GPU parity of the physics branches beyond the default aging-law path,
+through the C ABI against the oracle: the state laws 2-5 (src/rsf_engine.c:
+44-132), the normal-traction "In" rows (src/rsf_solve.c:245-247) against the
+reference-built golden tractions, the normal-stress evolution with its
+limiter (src/rsf_engine.c:264-275), the monitor rows (src/rsf_output.c:
+779-864), the event/monitor log capacities, and the continuation of a solve.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN
+from helpers import SEC_PER_YEAR, geom_from_text, initial_state, row_rel
+
+pytestmark = pytest.mark.gpu
+
+G_BP5 = 3.204e10
+
+
+def _L():
+    from interact_amd import lib
+    return lib
+
+
+def _bp5(ds, law=1, prestress="make_bp5"):
+    from interact_amd import faults
+    from interact_amd.model import FaultSystem
+    L = _L()
+    g = faults.planar_grid(ds)
+    f = faults.bp5_fields(g, prestress=prestress)
+    fs = FaultSystem(g)
+    fs.assemble(L.SLIP_STRIKE, shear_modulus=G_BP5)
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=1e-6, f0=0.6, eta=f["eta"], law=law)
+    fs.set_loading(1e-9)
+    X0 = initial_state(f["tau"], 25e6, f["vel"], f["a"], 1e-6)
+    return g, f, fs, X0
+
+
+def _oracle(g, f, law=1, **kw):
+    K = oracle.traction_tensor(g, 0, 16)[:, :, 0, 0] * (G_BP5 / 1e4)
+    return oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ np.full(len(f["a"]), -1e-9),
+                             eta=f["eta"], law=law, nthreads=16, **kw), K
+
+
+# ------------------------------------------------------------------ operator
+def test_normal_rows_match_reference_tractions():
+    """The In rows equal -scale times the reference's normal tractions on
+    the golden grid, scalar and two-component layouts (row-relative 1e-13)."""
+    L = _L()
+    from interact_amd.model import FaultSystem
+    z = np.load(os.path.join(GOLDEN, "ktensor_grid.npz"))
+    g = geom_from_text(str(z["geom"]))
+    T = z["T"]
+    n = T.shape[0]
+    sc = G_BP5 / 1e4
+    fs = FaultSystem(g)
+    fs.assemble(L.SLIP_STRIKE, shear_modulus=G_BP5, with_normal=True)
+    assert row_rel(fs.dense_normal(), -sc * T[:, :, 0, 2]) < 1e-13
+    assert row_rel(fs.dense(), sc * T[:, :, 0, 0]) < 1e-13
+    fs.assemble(L.SLIP_VECTOR, shear_modulus=G_BP5, with_normal=True)
+    In = np.zeros((n, 2 * n))
+    In[:, 0::2] = -sc * T[:, :, 0, 2]
+    In[:, 1::2] = -sc * T[:, :, 1, 2]
+    assert row_rel(fs.dense_normal(), In) < 1e-13
+    fs.close()
+
+
+# ------------------------------------------------------------------ state laws
+@pytest.mark.parametrize("law", [2, 3, 4, 5])
+def test_rhs_state_laws_match_oracle(law):
+    """Device RHS under each state law vs the oracle (BP5 2 km fields, state
+    off steady state, slip rates over seven decades)."""
+    g, f, fs, X0 = _bp5(2.0, law=law)
+    P, K = _oracle(g, f, law=law)
+    rng = np.random.default_rng(law)
+    X = X0.reshape(-1, 4).copy()
+    X[:, 0] += rng.normal(0, 0.02, len(X))
+    X = X.ravel()
+    Fg, ok = fs.rhs(X)
+    Fo = P.rhs(X)
+    assert ok
+    Fg, Fo = Fg.reshape(-1, 4), Fo.reshape(-1, 4)
+    for k in range(4):
+        assert np.abs(Fg[:, k] - Fo[:, k]).max() <= 1e-10 * np.abs(Fo[:, k]).max(), (law, k)
+    fs.close()
+
+
+@pytest.mark.parametrize("law", [2, 3, 4, 5])
+def test_solve_state_laws_match_oracle(law):
+    """rk5dp at rtol 1e-6 over the first 0.02 yr of the seeded BP5 2 km
+    nucleation under each law: the device takes the oracle's steps and
+    rejections and ends at its state."""
+    g, f, fs, X0 = _bp5(2.0, law=law)
+    P, K = _oracle(g, f, law=law)
+    t_end = 0.02 * SEC_PER_YEAR
+    fs.set_state(X0)
+    st = fs.solve(t_end, rtol=1e-6)
+    ro = P.solve(X0, t_end, rtol=1e-6)
+    assert st["status"] == 0 and ro["status"] == 0
+    assert (st["steps"], st["rejects"]) == (ro["steps"], ro["rejects"])
+    X, t = fs.get_state()
+    assert abs(t - ro["t_final"]) <= 1e-9 * t
+    assert np.abs(X - ro["X"]).max() / np.abs(ro["X"]).max() < 1e-8
+    fs.close()
+
+
+# ------------------------------------------------------------------ normal stress
+def _dipping_grid(ds_km, nx=40, nz=12, dip=30.0, top_km=1.0):
+    """planar fault of nx x nz square patches, strike 0 (+y), dipping dip
+    degrees, top edge top_km deep: strike slip on it changes the normal
+    stress on the fault (zero on a vertical strike-slip fault)."""
+    d = np.radians(dip)
+    i, j = np.meshgrid(np.arange(nx), np.arange(nz), indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    ds = ds_km * 1e3
+    s = (j + 0.5) * ds                           # down-dip distance
+    xyz = np.stack([s * np.cos(d), (i + 0.5 - nx / 2) * ds, -top_km * 1e3 - s * np.sin(d)], 1)
+    n = nx * nz
+    return dict(xyz=xyz, strike=np.zeros(n, np.float32), dip=np.full(n, dip, np.float32),
+                l=np.full(n, ds / 2), w=np.full(n, ds / 2),
+                along=(i + 0.5 - nx / 2) * ds_km, depth=top_km + s / 1e3 * np.sin(d))
+
+
+def _sigma_setup(limit):
+    from interact_amd import faults
+    from interact_amd.model import FaultSystem
+    L = _L()
+    g = _dipping_grid(2.0)
+    f = faults.bp5_fields(g)
+    n = len(g["l"])
+    sc = G_BP5 / 1e4
+    T = oracle.traction_tensor(g, 0, 16)
+    K, Kn = sc * T[:, :, 0, 0], -sc * T[:, :, 0, 2]
+    assert np.abs(Kn).max() > 0.05 * np.abs(K).max()
+    fs = FaultSystem(g)
+    fs.assemble(L.SLIP_STRIKE, shear_modulus=G_BP5, with_normal=True)
+    assert row_rel(fs.dense_normal(), Kn) < 1e-12
+    kw = dict(calc_sigma_dot=True, limit_sigma=limit, min_sigma=24.9e6, max_sigma=25.1e6)
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=1e-6, f0=0.6, eta=f["eta"], **kw)
+    fs.set_loading(1e-9)
+    vpl = np.full(n, -1e-9)
+    P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ vpl, Kn=Kn, sinc_n=Kn @ vpl,
+                          eta=f["eta"], nthreads=16, **kw)
+    X0 = initial_state(f["tau"], 25e6, f["vel"], f["a"], 1e-6)
+    return g, f, fs, P, X0
+
+
+@pytest.mark.parametrize("limit", [False, True])
+def test_sigma_evolution_matches_oracle(limit):
+    """calc_sigma_dot (sigma_dot = K_n v + sinc_n, with and without the
+    limiter) on a 30-degree dipping fault: the device RHS against the oracle,
+    then rk5dp over the seeded nucleation with the oracle's steps, sigma
+    changing and, with the limiter, held where it left [24.9, 25.1] MPa."""
+    g, f, fs, P, X0 = _sigma_setup(limit)
+    rng = np.random.default_rng(9)
+    X = X0.reshape(-1, 4).copy()
+    X[:, 0] += rng.normal(0, 0.01, len(X))
+    X[:, 2] *= 1 + rng.normal(0, 0.01, len(X))
+    X = X.ravel()
+    Fg, ok = fs.rhs(X)
+    Fo = P.rhs(X)
+    assert ok
+    Fg, Fo = Fg.reshape(-1, 4), Fo.reshape(-1, 4)
+    for k in range(4):
+        assert np.abs(Fg[:, k] - Fo[:, k]).max() <= 1e-10 * np.abs(Fo[:, k]).max(), k
+    assert np.abs(Fo[:, 2]).max() > 0
+    if limit:
+        sig = X.reshape(-1, 4)[:, 2]
+        out = (sig < 24.9e6) | (sig > 25.1e6)
+        assert out.any() and (Fg[out, 2] == 0).all()
+    t_end = 0.02 * SEC_PER_YEAR
+    fs.set_state(X0)
+    st = fs.solve(t_end, rtol=1e-6)
+    ro = P.solve(X0, t_end, rtol=1e-6)
+    assert (st["steps"], st["rejects"]) == (ro["steps"], ro["rejects"])
+    Xg, t = fs.get_state()
+    assert np.abs(Xg - ro["X"]).max() / np.abs(ro["X"]).max() < 1e-8
+    dsig = np.abs(Xg.reshape(-1, 4)[:, 2] - 25e6).max()
+    assert dsig > 1e3
+    fs.close()
+
+
+# ------------------------------------------------------------------ monitor
+def test_monitor_rows_match_oracle():
+    """rsf_monitor rows (step, t, t [yr], dt, log10 max|v|, mean slip, mean
+    mu, sigma max, sigma min, log10 mean|v|) of the device controller against
+    the oracle's, BP5 2 km through the seeded event at rtol 1e-6."""
+    g, f, fs, X0 = _bp5(2.0)
+    P, K = _oracle(g, f)
+    t_end = 0.05 * SEC_PER_YEAR
+    fs.set_state(X0)
+    st = fs.solve(t_end, rtol=1e-6)
+    ro = P.solve(X0, t_end, rtol=1e-6)
+    mg, mo = fs.monitor(), ro["monitor"]
+    assert st["nmonitor"] == len(mg) == len(mo) > 10
+    assert np.array_equal(mg[:, 0], mo[:, 0])           # same step numbers
+    for c in range(1, 10):
+        scale = np.abs(mo[:, c]).max()
+        assert np.abs(mg[:, c] - mo[:, c]).max() <= 1e-9 * scale + 1e-300, c
+    fs.close()
+
+
+def test_event_and_monitor_logs_report_overflow():
+    """Logs capped through the solver options keep their first rows and
+    count the rest as dropped (no silent truncation)."""
+    g, f, fs, X0 = _bp5(2.0)
+    t_end = 0.05 * SEC_PER_YEAR
+    fs.set_state(X0)
+    full = fs.solve(t_end, rtol=1e-6)
+    ev, mon = fs.events(), fs.monitor()
+    assert full["events_dropped"] == 0 and full["monitor_dropped"] == 0
+    assert len(ev) >= 2 and len(mon) > 5
+    fs.set_state(X0)
+    st = fs.solve(t_end, rtol=1e-6, max_events=1, max_monitor=5)
+    assert st["nevent"] == 1 and st["events_dropped"] == len(ev) - 1
+    assert st["nmonitor"] == 5 and st["monitor_dropped"] == len(mon) - 5
+    assert np.array_equal(fs.events(), ev[:1])
+    assert np.array_equal(fs.monitor(), mon[:5])
+    # a later solve with the default capacity stores every row again
+    fs.set_state(X0)
+    again = fs.solve(t_end, rtol=1e-6)
+    assert again["events_dropped"] == 0 and np.array_equal(fs.events(), ev)
+    fs.close()
+
+
+# ------------------------------------------------------------------ continuation
+def test_solve_continue_equals_one_solve():
+    """A solve chopped into three continued segments takes exactly the steps
+    of one solve to the final time: same state bits, events and monitor rows
+    (BP5 2 km through the seeded event, rk5dp rtol 1e-6)."""
+    L = _L()
+    g, f, fs, X0 = _bp5(2.0)
+    T = 0.05 * SEC_PER_YEAR
+    fs.set_state(X0)
+    one = fs.solve(T, rtol=1e-6)
+    X1, t1 = fs.get_state()
+    ev1, mon1 = fs.events(), fs.monitor()
+    fs.set_state(X0)
+    fs.solve(T / 3, rtol=1e-6)
+    fs.solve_continue(2 * T / 3, rtol=1e-6)
+    seg = fs.solve_continue(T, rtol=1e-6)
+    X2, t2 = fs.get_state()
+    assert (seg["steps"], seg["rejects"], seg["rhs_evals"]) == \
+        (one["steps"], one["rejects"], one["rhs_evals"])
+    assert t1 == t2 and np.array_equal(X1, X2)
+    assert np.array_equal(fs.events(), ev1) and np.array_equal(fs.monitor(), mon1)
+    # continuing to a time already reached does nothing
+    same = fs.solve_continue(T / 2, rtol=1e-6)
+    assert same["steps"] == one["steps"] and np.array_equal(fs.get_state()[0], X1)
+    # a changed state, or another operator precision, cannot be continued
+    with pytest.raises(L.RsfgError, match="precision"):
+        fs.solve_continue(2 * T, rtol=1e-6, prec=L.PREC_FP32)
+    fs.set_state(X0)
+    with pytest.raises(L.RsfgError, match="no solve to continue"):
+        fs.solve_continue(2 * T, rtol=1e-6)
+    fs.close()

@@ -69,3 +69,49 @@ def test_read_table_matches_the_reference_inputs(tmp_path):
     import pytest
     with pytest.raises(lib.RsfgError, match="row 1 column 1"):
         read_table(str(short), 2, 2)
+
+
+def test_read_patch_file_matches_the_oracle_reader(tmp_path):
+    """The reference geometry format through the library's reader (host
+    code, no GPU), src/read_geometry.c:115-117 and :148-215: the BP5 2 km
+    rectangles as the oracle's reader parses them, and a triangle file (the
+    rough Hurst-0.8 mesh) with its vertices; a capacity below the file's
+    patch count is an error that reports the full count."""
+    import ctypes as C
+    import numpy as np
+    import oracle
+    from conftest import GOLDEN
+    from interact_amd import faults, lib
+    from interact_amd.model import read_patch_file
+    import pytest
+    f = os.path.join(GOLDEN, "bp5_2km", "geom_bp5_2km.in")
+    got, ref = read_patch_file(f), oracle.read_patches(f)
+    assert len(got["l"]) == len(ref["l"]) == 1000
+    for k in ("xyz", "strike", "dip", "l", "w", "group"):
+        assert np.array_equal(got[k], ref[k]), k
+    assert not got["vert"].any()
+    nx, nz = 6, 4
+    off = faults.rough_offset(nx, nz, 1.0)
+    tg = faults.planar_tri_grid(1.0, nx, nz, offset=off)
+    tf = tmp_path / "tri.in"
+    faults.write_patch_file(str(tf), tg)
+    got = read_patch_file(str(tf))
+    assert len(got["l"]) == 2 * nx * nz
+    assert (got["l"] < 0).all() and (got["w"] < 0).all()
+    assert np.array_equal(got["vert"], tg["vert"])
+    L = lib.load()
+    arr = (lib.Patch * 10)()
+    n = C.c_int(10)
+    assert L.rsfg_read_patch_file(os.fsencode(f), arr, C.byref(n)) == lib.RSFG_EINVAL
+    assert n.value == 1000
+    assert b"more than the capacity" in L.rsfg_last_error()
+    assert arr[9].y == ref["xyz"][9, 1]
+    # reading ends at the first line that does not parse as a patch, as the
+    # reference's while (fscanf(...) == 8) loop does
+    short = tmp_path / "short.in"
+    short.write_text("0 0 -1000 0 90 500 500 0\n0 0 -2000 0 90 500\n")
+    assert len(read_patch_file(str(short))["l"]) == 1
+    tri = tmp_path / "tri_short.in"
+    tri.write_text("0 0 -1000 0 90 -1 -1 0 0 0 -1 0 1 -1\n")
+    with pytest.raises(lib.RsfgError, match="truncated triangle"):
+        read_patch_file(str(tri))
