@@ -31,7 +31,8 @@ EXPORTS = [
     "rsfg_last_error", "rsfg_version", "rsfg_device_count", "rsfg_create",
     "rsfg_destroy", "rsfg_read_patch_file", "rsfg_assemble",
     "rsfg_operator_dims", "rsfg_operator_bytes", "rsfg_entries", "rsfg_traction_pairs",
-    "rsfg_matvec", "rsfg_matvec_dev", "rsfg_matvec_t", "rsfg_matvec_bench",
+    "rsfg_matvec", "rsfg_matvec_dev", "rsfg_matvec_t",
+    "rsfg_matvec_bench",
     "rsfg_set_friction", "rsfg_set_loading", "rsfg_rhs",
     "rsfg_solver_defaults", "rsfg_set_state", "rsfg_get_state", "rsfg_solve",
     "rsfg_solve_continue",
@@ -169,6 +170,8 @@ def load(path=None):
                                        C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.rsfg_solve.argtypes = [h, C.POINTER(SolverOpts), C.c_double,
                              C.POINTER(Stats)]
+    L.rsfg_solve_continue.argtypes = [h, C.POINTER(SolverOpts), C.c_double,
+                                      C.POINTER(Stats)]
     L.rsfg_events.argtypes = [h, _dp, C.POINTER(C.c_int)]
     L.rsfg_monitor.argtypes = [h, _dp, C.POINTER(C.c_int)]
     L.rsfg_stage_bench.argtypes = [h, C.c_int, C.c_int, C.POINTER(C.c_double)]
@@ -191,12 +194,14 @@ def device_count():
     return c.value
 
 
-def read_patch_file(fname, capacity=None):
-    if capacity is None:
-        capacity = sum(1 for _ in open(fname))
-    arr = (Patch * max(capacity, 1))()
-    n = C.c_int(capacity)
-    check(load().rsfg_read_patch_file(fname.encode(), arr, C.byref(n)),
+def read_patch_file(fname):
+    """(ctypes Patch array, count) of a reference geometry file: counts the
+    patches, then reads them (rsfg_read_patch_file)."""
+    n = C.c_int(0)
+    check(load().rsfg_read_patch_file(os.fsencode(fname), None, C.byref(n)),
+          "rsfg_read_patch_file")
+    arr = (Patch * max(n.value, 1))()
+    check(load().rsfg_read_patch_file(os.fsencode(fname), arr, C.byref(n)),
           "rsfg_read_patch_file")
     return arr, n.value
 

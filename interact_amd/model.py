@@ -420,14 +420,7 @@ def read_patch_file(fname):
     """Patches of a reference geometry file (rsfg_read_patch_file): a dict
     of arrays xyz, strike, dip (float32, as read), l, w, group, vert (nine
     vertex coordinates of triangles, zeros for rectangles)."""
-    lib = L.load()
-    n = C.c_int(0)
-    L.check(lib.rsfg_read_patch_file(os.fsencode(fname), None, C.byref(n)),
-            "rsfg_read_patch_file")
-    arr = (L.Patch * max(n.value, 1))()
-    L.check(lib.rsfg_read_patch_file(os.fsencode(fname), arr, C.byref(n)),
-            "rsfg_read_patch_file")
-    k = n.value
+    arr, k = L.read_patch_file(fname)
     return dict(xyz=np.array([[p.x, p.y, p.z] for p in arr[:k]]).reshape(k, 3),
                 strike=np.array([p.strike for p in arr[:k]], np.float32),
                 dip=np.array([p.dip for p in arr[:k]], np.float32),
