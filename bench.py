@@ -86,6 +86,13 @@ DTYPES = {"fp64": "f64 K.v + f64 state",
 # (two DC3D evaluations + stress + tractions, half space, vertical coplanar
 # patches), counted on the oracle by oracle/opcount.cpp
 F_PAIR = 6205
+# the same for a Nikkhoo-Walter triangle pair (two half-space triangle
+# stresses for strike and dip slip, each the main and image dislocations and
+# the free-surface correction of three edges, plus tractions), counted by
+# oracle/opcount_tri.cpp on receivers off the source triangle of a rough
+# fault; it includes the per-triangle frame and edge angles that the oracle,
+# like the reference, evaluates within every pair
+F_PAIR_TRI = 30559
 FP64_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 METRIC = "sim-years/hour + K·v HBM GB/s, mixed-prec vs FP64 GPU at matched event times"
@@ -897,8 +904,11 @@ def rough_triangle_assembly(ds_km):
     pairs = n * n
     log(f"assembly, rough triangles: {n} triangles, {pairs} pairs in {ms:.1f} ms = "
         f"{pairs / ms * 1e3:.3e} pairs/s")
-    return dict(triangles=n, pairs=pairs, ms=ms, pairs_per_s=pairs / ms * 1e3,
-                entries_per_s=4 * pairs / ms * 1e3, dof=2 * n)
+    rate = pairs / ms * 1e3
+    return dict(triangles=n, pairs=pairs, ms=ms, pairs_per_s=rate,
+                entries_per_s=4 * rate, dof=2 * n, f_pair=F_PAIR_TRI,
+                tflops=rate * F_PAIR_TRI / 1e12,
+                frac_fp64_peak=rate * F_PAIR_TRI / 1e12 / FP64_PEAK_TFLOPS)
 
 
 if __name__ == "__main__":

@@ -154,9 +154,12 @@ template <int R>
 struct GemvUnroll<double, R> {
   static constexpr int U = (R == 1) ? 6 : (R == 2 ? 3 : 2);
 };
+#ifndef RSFG_BF16_U
+#define RSFG_BF16_U 2
+#endif
 template <int R>
 struct GemvUnroll<bf16_t, R> {
-  static constexpr int U = (R == 1) ? 4 : 2;
+  static constexpr int U = (R == 1) ? 4 : RSFG_BF16_U;
 };
 
 // Lane partial sums of rows [row0, row0 + R) over all columns, streamed by

@@ -66,10 +66,13 @@ __global__ __launch_bounds__(256) void k_init_vel(StageArgs a) {
 #define RSFG_STAGE_PPB 8
 #endif
 constexpr int kStagePPB = RSFG_STAGE_PPB;  // patches (= waves) per workgroup
+#ifndef RSFG_STAGE_MINW
+#define RSFG_STAGE_MINW 4
+#endif
 static_assert(kStagePPB >= 1 && kStagePPB <= 16, "one wave per patch, at most 1024 threads");
 
 template <typename TK, int NC, int G>
-__global__ __launch_bounds__(64 * kStagePPB, 4) void k_stage(StageArgs a) {
+__global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(StageArgs a) {
   using VT = typename KTraits<TK>::VT;
   constexpr int D = Dim<NC>::v;
   constexpr int PD = PreDim<NC>::v;

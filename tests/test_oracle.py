@@ -100,6 +100,29 @@ def test_triangle_free_surface_is_traction_free():
             assert np.abs(S[:, 2]).max() <= 1e-12 * np.abs(Sb).max()
 
 
+def test_triangle_operation_count_runs_the_oracle_arithmetic():
+    """oracle/opcount_tri (the oracle's triangle pair compiled with an
+    operation-counting scalar) gives F_pair of triangle assembly, the
+    constant bench.py prices rough-mesh assembly with, and tractions bitwise
+    equal to the plain oracle's on the same three-triangle mesh."""
+    import subprocess
+    from conftest import ROOT
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "opcount_tri"], check=True)
+    out = subprocess.run([os.path.join(ROOT, "oracle", "opcount_tri")], capture_output=True,
+                         text=True, check=True).stdout
+    f_pair = int(out.split("F_pair_tri")[1].split()[0])
+    assert f_pair == 30559
+    T = np.array([float(x) for x in out.split("tractions")[1].split()]).reshape(3, 3, 2, 3)
+    v = np.array([1.2, -250.0, -10500.0, -0.7, 250.0, -10500.0, 0.4, -250.0, -10000.0,
+                  -0.7, 250.0, -10500.0, 2.1, 250.0, -10000.0, 0.4, -250.0, -10000.0,
+                  3.0, 2750.0, -12500.0, -1.5, 3250.0, -12500.0, 0.8, 2750.0, -12000.0]
+                 ).reshape(3, 9)
+    g = dict(xyz=(v[:, 0:3] + v[:, 3:6] + v[:, 6:9]) / 3, strike=np.zeros(3, np.float32),
+             dip=np.full(3, 90, np.float32), l=-np.ones(3), w=-np.ones(3), vert=v)
+    To = oracle.mesh_traction_tensor(g, 0, 1)
+    assert np.array_equal(To, T)
+
+
 def test_unit_slip_known_answers():
     """|K 1|_2 of scaling_tests/full_vs_half.csv:2,5 on the makefault mesh
     (strike slip -> strike traction, internal units, scale 1)."""
