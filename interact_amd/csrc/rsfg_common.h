@@ -72,8 +72,6 @@ struct DevCtl {
   int nevent, nmon;     // rows stored (at most max_event, max_mon)
   int max_event, max_mon;
   int ev_drop, mon_drop;  // rows past the capacity: counted, not stored
-  double* hlog;         // accepted step sizes, in order (optional)
-  long long hlog_cap;
 };
 
 // Butcher tableau and integrator settings (kernel argument, by value)

@@ -546,6 +546,8 @@ void fill_stage_args(rsfg_ctx* h, const rsfg_solver_opts* o, const void* K, Stag
   a.slip0 = h->d_cat + n;
   a.rupt = h->d_rupt;
   a.ctl = h->d_ctl;
+  a.hlog = h->d_hlog;
+  a.hlog_cap = h->hlog_cap;
   fill_tableau(a.tb, o->method);
   a.so.rtol = o->rtol;
   for (int f = 0; f < 6; ++f) a.so.atol[f] = o->atol[f];
@@ -573,8 +575,6 @@ static int start_solve(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, Sta
   c->max_event = o->max_events > 0 ? o->max_events : kMaxEv;
   c->max_mon = o->max_monitor > 0 ? o->max_monitor : kMaxMon;
   c->xcur = h->xcur;
-  c->hlog = h->d_hlog;
-  c->hlog_cap = h->hlog_cap;
   RSFG_HIP(hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, h->stream));
   // the GEMV reads the input vector up to a whole chunk past the last
   // column: its padding must be zero in this precision's element type
@@ -700,8 +700,6 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
     c->max_steps = (o->max_steps > LLONG_MAX - c->steps) ? LLONG_MAX : c->steps + o->max_steps;
     c->halted = (c->t < t_end) ? 0 : 1;
     c->status = 0;
-    c->hlog = h->d_hlog;
-    c->hlog_cap = h->hlog_cap;
     const hipError_t e =
         hipMemcpyAsync(h->d_ctl, c, sizeof(DevCtl), hipMemcpyHostToDevice, h->stream);
     if (e != hipSuccess) {

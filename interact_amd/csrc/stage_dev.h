@@ -40,6 +40,8 @@ struct StageArgs {
   double* slip0;
   int* rupt;
   DevCtl* ctl;
+  double* hlog;        // accepted step sizes, in order (optional; the control
+  long long hlog_cap;  // block the host copies holds no device address)
   Tableau tb;
   SolveOpts so;
   int s;       // stage index; 0 = evaluation of K_0 at the initial state
@@ -400,7 +402,7 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
   __syncthreads();
   // ---- advance
   if (tid == 0) {
-    if (c->hlog && c->steps < c->hlog_cap) c->hlog[c->steps] = c->h;
+    if (a.hlog && c->steps < a.hlog_cap) a.hlog[c->steps] = c->h;
     c->t += c->h;
     c->h = s_par[0];
     c->xcur = xo ^ 1;
