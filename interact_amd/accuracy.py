@@ -37,6 +37,24 @@ def trace_deviation(ref_mon, run_mon):
     return float(dv.max()), float(np.sqrt(np.mean(dv ** 2)))
 
 
+def trace_deviation_uniform(ref_mon, run_mon, npts=2000):
+    """(max, rms) of the same difference on npts equally spaced times of the
+    common span.  The monitor writes its rows on state change, so most of
+    them fall inside events, where log10 max|v| moves by decades within
+    seconds and a small onset shift dominates the row-sampled metric; on a
+    uniform time grid the interseismic trajectory carries the weight."""
+    tr, lr = _trace(ref_mon)
+    tx, lx = _trace(run_mon)
+    if len(tr) < 2 or len(tx) < 2:
+        return float("nan"), float("nan")
+    lo, hi = max(tr[0], tx[0]), min(tr[-1], tx[-1])
+    if not hi > lo:
+        return float("nan"), float("nan")
+    tt = np.linspace(lo, hi, npts)
+    dv = np.abs(np.interp(tt, tx, lx) - np.interp(tt, tr, lr))
+    return float(dv.max()), float(np.sqrt(np.mean(dv ** 2)))
+
+
 def onsets(ev):
     ev = np.asarray(ev, np.float64).reshape(-1, 8)
     return np.sort(ev[ev[:, 1] > 0, 0]) / SEC_PER_YEAR
@@ -56,11 +74,13 @@ def event_peaks(ev):
 
 
 def score(ref_mon, ref_ev, run_mon, run_ev):
-    """dict of maxdev, rmsdev (log10 units), dt1 (first-onset difference
-    [yr]), drec (mean-recurrence difference [yr]), nev_ref, nev_run,
+    """dict of maxdev, rmsdev (log10 units, on the reference's monitor
+    rows), maxdev_uniform, rmsdev_uniform (on a uniform time grid), dt1
+    (first-onset difference [yr]), drec (mean-recurrence difference [yr]), nev_ref, nev_run,
     rec_ref [yr], donset (onset differences of the events both runs have
     [yr]) and dpeak (relative peak slip-rate differences of those events)."""
     maxdev, rmsdev = trace_deviation(ref_mon, run_mon)
+    umax, urms = trace_deviation_uniform(ref_mon, run_mon)
     er, ex = onsets(ref_ev), onsets(run_ev)
     dt1 = float(ex[0] - er[0]) if len(er) and len(ex) else float("nan")
     rec_ref = float(np.mean(np.diff(er))) if len(er) >= 2 else float("nan")
@@ -68,7 +88,8 @@ def score(ref_mon, ref_ev, run_mon, run_ev):
     k = min(len(er), len(ex))
     pr, px = event_peaks(ref_ev), event_peaks(run_ev)
     kp = min(len(pr), len(px))
-    return dict(maxdev=maxdev, rmsdev=rmsdev, dt1=dt1, drec=rec_run - rec_ref,
+    return dict(maxdev=maxdev, rmsdev=rmsdev, maxdev_uniform=umax, rmsdev_uniform=urms,
+                dt1=dt1, drec=rec_run - rec_ref,
                 nev_ref=int(len(er)), nev_run=int(len(ex)), rec_ref=rec_ref,
                 donset=[float(d) for d in ex[:k] - er[:k]],
                 dpeak=[float(d) for d in (px[:kp, 1] - pr[:kp, 1]) / pr[:kp, 1]])

@@ -150,8 +150,11 @@ __device__ __forceinline__ void store_entry(const AsmArgs& a, size_t idx,
   }
 }
 
+#ifndef RSFG_ASM_MINW
+#define RSFG_ASM_MINW 1
+#endif
 template <bool TRI>
-__global__ __launch_bounds__(256) void k_assemble(AsmArgs a) {
+__global__ __launch_bounds__(256, RSFG_ASM_MINW) void k_assemble(AsmArgs a) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= a.n) return;

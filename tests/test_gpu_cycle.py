@@ -72,7 +72,8 @@ def test_c1_cycle_trajectories_match_the_oracle():
         assert max(abs(d) for d in s["donset"]) < tol_on, (name, s)
         assert max(abs(d) for d in s["dpeak"]) < tol_pk, (name, s)
     assert scores["fp64"]["rmsdev"] < 0.01, scores["fp64"]
-    print({k: {q: v[q] for q in ("maxdev", "rmsdev", "dt1", "drec", "donset", "dpeak")}
+    print({k: {q: v[q] for q in ("maxdev", "rmsdev", "maxdev_uniform", "rmsdev_uniform", "dt1",
+                                 "drec", "donset", "dpeak")}
            for k, v in scores.items()})
     fs.close()
 
