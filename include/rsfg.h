@@ -171,6 +171,20 @@ int rsfg_traction_pairs(rsfg_handle *h, long npairs, const int *recv,
 int rsfg_matvec(rsfg_handle *h, int prec, const double *x, double *y);
 int rsfg_matvec_dev(rsfg_handle *h, int prec, const void *x_dev,
                     void *y_dev);
+/* rsfg_matvec_dev without any allocation.  No counterpart in the
+   reference: for callers that cannot call hipMalloc (code under HIP graph
+   capture, tight serving loops, harnesses that forbid allocation).  The
+   padded product input and, with normal rows, the product over all rows are
+   staged in a static device array of 512 KiB per GPU (65536 doubles, held
+   until the process ends) instead of the handle's allocated scratch, so the
+   padded columns (cols + 1024 rounded up to a multiple of 512) plus, with
+   normal rows, all rows must fit in 65536 doubles (e.g. 32000 DOF without
+   normal rows); a larger operator returns RSFG_EINVAL.  Same
+   kernels and results as rsfg_matvec_dev, asynchronous on the handle's
+   stream.  Every call on a GPU uses that GPU's one array: calls that
+   overlap on one GPU (other handles or threads) race. */
+int rsfg_matvec_dev_noalloc(rsfg_handle *h, int prec, const void *x_dev,
+                            void *y_dev);
 /* transposed product K^T x (adjoint) */
 int rsfg_matvec_t(rsfg_handle *h, int prec, const double *x, double *y);
 /* time nrep back-to-back products on device-resident data; returns the

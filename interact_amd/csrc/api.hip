@@ -7,6 +7,7 @@
 #include <type_traits>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -437,6 +438,34 @@ static int prec_ok(rsfg_ctx* h, int prec) {
   return operator_ptr(h, prec) != nullptr;
 }
 
+// Staging buffer of rsfg_matvec_dev_noalloc: the padded product input and,
+// with normal rows, the product over all rows.  One copy per GPU, held until
+// the process ends; every call on a GPU shares it.  In host code the name
+// `scratch` is only a stand-in for the symbol: the device address of each
+// GPU's copy comes from hipGetSymbolAddress (scratch_ptr below) and only that
+// address is passed to kernels.
+constexpr size_t kScratchDoubles = 65536;   // 512 KiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch[kMaxDevices] = {nullptr};
+
+// the device address of `scratch` on `device`, looked up once per device
+// with that device current (the caller has made it current)
+static int scratch_ptr(int device, double** out) {
+  RSFG_CHECK(device >= 0 && device < kMaxDevices, RSFG_EINVAL, "scratch: device index");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+    g_scratch[device] = static_cast<double*>(p);
+  }
+  *out = g_scratch[device];
+  return RSFG_OK;
+}
+
 // Near-field chunk lists of the bf16 + lo format from the assembled FP64
 // operator: per patch, chunks in decreasing energy until the chunks left
 // out hold at most tol^2 of the patch rows' energy; then the lo blocks.
@@ -581,6 +610,12 @@ int rsfg_create(int device, const rsfg_patch* pt, int n, rsfg_handle** out) {
     }
   }
   RSFG_HIP(hipSetDevice(device));
+  {
+    // the static staging array of rsfg_matvec_dev_noalloc on this device
+    double* sp = nullptr;
+    const int rc = scratch_ptr(device, &sp);
+    if (rc != RSFG_OK) return rc;
+  }
   rsfg_ctx* h = new rsfg_ctx();
   h->device = device;
   h->n = n;
@@ -889,6 +924,37 @@ int rsfg_matvec_dev(rsfg_handle* h, int prec, const void* x_dev, void* y_dev) {
                        h->ncomp, h->G, static_cast<double*>(y_dev));
     RSFG_HIP(hipGetLastError());
   }
+  return RSFG_OK;
+}
+
+int rsfg_matvec_dev_noalloc(rsfg_handle* h, int prec, const void* x_dev, void* y_dev) {
+  RSFG_CHECK(h && x_dev && y_dev, RSFG_EINVAL, "rsfg_matvec_dev_noalloc: null argument");
+  RSFG_CHECK(prec_ok(h, prec), RSFG_ESTATE, "rsfg_matvec_dev_noalloc: precision not assembled");
+  const size_t need = vin_elems_for(h->cols);
+  const size_t yall = (h->G == h->ncomp) ? 0 : (size_t)h->rows_all;
+  RSFG_CHECK(need + yall <= kScratchDoubles, RSFG_EINVAL,
+             "rsfg_matvec_dev_noalloc: operator too large for the static staging buffer ("
+             "padded columns " + std::to_string(need) + " + rows " + std::to_string(yall) +
+                 " > " + std::to_string(kScratchDoubles) + " doubles); use rsfg_matvec_dev");
+  RSFG_HIP(hipSetDevice(h->device));
+  double* sp = nullptr;
+  int rc = scratch_ptr(h->device, &sp);
+  if (rc) return rc;
+  // the staging kernel writes every padded input element the product reads,
+  // so nothing of an earlier call's contents is used
+  const size_t saved = h->vin_elems;
+  h->vin_elems = need;
+  rc = stage_input(h, prec, static_cast<const double*>(x_dev), sp);
+  h->vin_elems = saved;
+  if (rc) return rc;
+  if (yall == 0) return matvec_any(h, prec, sp, static_cast<double*>(y_dev));
+  double* ya = sp + need;
+  rc = matvec_any(h, prec, sp, ya);
+  if (rc) return rc;
+  const int nr = h->n * h->ncomp;  // drop the normal rows
+  hipLaunchKernelGGL(k_pick_shear, dim3((nr + 255) / 256), dim3(256), 0, h->stream, ya, h->n,
+                     h->ncomp, h->G, static_cast<double*>(y_dev));
+  RSFG_HIP(hipGetLastError());
   return RSFG_OK;
 }
 
