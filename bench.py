@@ -51,15 +51,18 @@ sys.path.insert(0, ROOT)
 
 SEC_PER_YEAR = 31557600.0
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused stage kernel on the 1 km, 8000-DOF
-# workload: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors that a plain
-# 256 MiB device copy reads on the same counters
-# (profiles/pmc_traffic_planar1km_8000dof.md)
+# HBM bytes per launch of the fused stage kernel on the 1 km (8000 DOF) and
+# 0.5 km (32000 DOF) workloads: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
+# factors that a plain 256 MiB device copy reads on the same counters
+# (profiles/pmc_traffic_planar1km_8000dof.md,
+# profiles/kernels_planar0.5km_32000dof.md)
 STAGE_TRAFFIC = {"c2": {"fp64": 517.6e6, "fp32": 262.1e6, "bf16x2": 262.2e6, "bf16": 132.6e6,
                         "bf16nf": 145.9e6},
-                 "c3": {}}
-# HBM bytes per launch of the FP32 product k_matvec<float,2> (same counters)
-MATVEC_TRAFFIC = {"c2": {}, "c3": {}}
+                 "c3": {"fp64": 8.2051e9, "fp32": 4.1082e9, "bf16x2": 4.1082e9,
+                        "bf16": 2.0746e9, "bf16nf": 2.1486e9}}
+# HBM bytes per launch of the FP32 product k_matvec<float,2> (same counters;
+# profiles/kernels_planar0.5km_32000dof.md for the 0.5 km workload)
+MATVEC_TRAFFIC = {"c2": {"fp32": 259.8e6}, "c3": {"fp32": 4.0974e9}}
 # HBM bytes per launch of the c4 batched product by unit shape (the shapes
 # the tuner picks on these boxes), from the same two counter runs of
 # `--config c4 --kernels-only` with the copy-kernel factors above
