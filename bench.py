@@ -333,8 +333,9 @@ def main():
     for name in ("fp64", "fp32", "bf16x2", "bf16", "bf16nf"):
         p = lib.PREC_BY_NAME[name]
         y = fs.matvec(xa, p)
-        mv_ms = fs.matvec_bench(p, 50)
-        st_ms = fs.stage_bench(p, 50)
+        # median of three measurements of 50 launches each (HIP events)
+        mv_ms = float(np.median([fs.matvec_bench(p, 50) for _ in range(3)]))
+        st_ms = float(np.median([fs.stage_bench(p, 50) for _ in range(3)]))
         bytes_alg = fs.operator_bytes(p) + 16.0 * D
         kv[name] = dict(relerr=float(np.linalg.norm(y - y64) / np.linalg.norm(y64)),
                         matvec_ms=mv_ms, stage_ms=st_ms,
