@@ -183,7 +183,7 @@ int alloc_state(rsfg_ctx* h) {
   for (int i = 0; i < 2; ++i) { h->Pb[i] = p; p += pre; }
   for (int i = 0; i < 2; ++i) { h->vin[i] = p; p += h->vin_elems; }
   for (int i = 0; i < 2; ++i) { h->vxb[i] = p; p += h->vin_elems; }
-  h->partial_count = (size_t)(n + 1) * kNRed;
+  h->partial_count = (size_t)(n + 64) * kNRed;   // per-patch or per-wave columns
   RSFG_HIP(hipMalloc(&h->partials, h->partial_count * sizeof(double)));
   {
     const int rc = ensure_logs(h, kMaxEv, kMaxMon);

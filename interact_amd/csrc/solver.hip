@@ -69,13 +69,157 @@ constexpr int kStagePPB = RSFG_STAGE_PPB;  // patches (= waves) per workgroup
 #ifndef RSFG_STAGE_MINW
 #define RSFG_STAGE_MINW 4
 #endif
+#ifndef RSFG_STAGE_SPLIT
+#define RSFG_STAGE_SPLIT 0
+#endif
 static_assert(kStagePPB >= 1 && kStagePPB <= 16, "one wave per patch, at most 1024 threads");
+
+// Epilogue operands of one patch whose addresses are known at launch: the
+// state-only record of Y_s, the accepted state X, the tableau-weighted sum
+// of the earlier stage derivatives and the stressing-rate loading.  The
+// stage kernels request them before streaming their rows, so the epilogue
+// starts on data already in registers.
+template <int NC>
+struct StageOperands {
+  double P[PreDim<NC>::v], sn[NC], snn, x0[Dim<NC>::v], ws[Dim<NC>::v];
+};
+
+template <int NC>
+__device__ __forceinline__ void stage_operands(const StageArgs& a, const int* ks, size_t sst,
+                                               const double* X, int p, bool own,
+                                               StageOperands<NC>& o) {
+  constexpr int D = Dim<NC>::v;
+  constexpr int PD = PreDim<NC>::v;
+  const int n = a.n, S = a.tb.s, s = a.s;
+  const bool mid = (s > 0) && (s < S - 1);
+#pragma unroll
+  for (int k = 0; k < PD; ++k) o.P[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < NC; ++k) o.sn[k] = 0.0;
+  o.snn = 0.0;
+#pragma unroll
+  for (int f = 0; f < D; ++f) o.x0[f] = o.ws[f] = 0.0;
+  if (!own) return;
+  if (s > 0) {
+    const double* Ps = a.Pb[s & 1];
+#pragma unroll
+    for (int k = 0; k < PD; ++k) o.P[k] = Ps[k * n + p];
+#pragma unroll
+    for (int f = 0; f < D; ++f) o.x0[f] = X[f * n + p];
+    // sum_j A[s+1][j] K_j over j < s (middle stages), or
+    // sum_j bh[j] K_j over j < S - 1 (final stage)
+    const int nj = mid ? s : S - 1;
+#pragma unroll
+    for (int j = 0; j < kMaxStages; ++j) {
+      if (j >= nj) break;
+      const double w = mid ? a.tb.A[s + 1][j] : a.tb.bh[j];
+      if (w == 0.0) continue;
+      const double* Kj = a.Stg[0] + (size_t)ks[j] * sst;
+#pragma unroll
+      for (int f = 0; f < D; ++f) o.ws[f] += w * Kj[f * n + p];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) o.sn[k] = a.sinc[NC * p + k];
+  o.snn = a.sinc[NC * n + p];
+}
+
+__device__ __forceinline__ void r9_neutral(double* r9) {
+  r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
+  r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
+  r9[RED_DX2] = 0.0;
+}
+
+// Stage epilogue of patch p from its G operator row sums rsum (shear rows,
+// then the normal row when present): K_s = F(Y_s, rsum) into its stage
+// slot, then Y_{s+1}, its state-only record and v(Y_{s+1}) (middle stages),
+// or the patch's terms of the launch's reductions r9 (initial evaluation
+// and final stage: error norm, slip-rate and stress extremes, ...).
+template <typename VT, int NC, int G>
+__device__ __forceinline__ void stage_epilogue(const StageArgs& a, DevCtl* c, const int* ks,
+                                               size_t sst, double hstep, const double* X,
+                                               double* Xn, int p, const double* rsum,
+                                               StageOperands<NC>& o, double* r9) {
+  constexpr int D = Dim<NC>::v;
+  constexpr int PD = PreDim<NC>::v;
+  const int n = a.n, S = a.tb.s, s = a.s;
+  const bool reduce = (s == 0) || (s == S - 1);
+  const bool mid = (s > 0) && (s < S - 1);
+  const PatchCoef cf = load_coef(a.coef, n, p);
+  double y[D], kf[D], td[NC];
+  if (reduce) {
+    const double* Yc = (s == 0) ? X : Xn;
+#pragma unroll
+    for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
+  }
+  if (s == 0) rsf_patch_pre<NC>(a.fp, cf, y, o.P);
+#pragma unroll
+  for (int k = 0; k < NC; ++k) td[k] = rsum[k];
+  const double sdn = (G > NC) ? rsum[NC] : 0.0;
+  rsf_patch_finish<NC>(a.fp, o.P, td, sdn, o.sn, o.snn, kf);
+  int kss = ks[0];
+#pragma unroll
+  for (int j = 1; j < kMaxStages; ++j) kss = (j == s) ? ks[j] : kss;
+  double* Ks = a.Stg[0] + (size_t)kss * sst;
+#pragma unroll
+  for (int f = 0; f < D; ++f) Ks[f * n + p] = kf[f];
+  if (mid) {
+    // next stage state Y_{s+1} = X + h (sum_{j<s} A[s+1][j] K_j + A[s+1][s] K_s)
+    // and its state-only record
+    const double w = a.tb.A[s + 1][s];
+    double* Yn = (s + 1 == S - 1) ? Xn : a.Yb[(s + 1) & 1];
+    double yn[D];
+#pragma unroll
+    for (int f = 0; f < D; ++f) {
+      yn[f] = o.x0[f] + hstep * (o.ws[f] + w * kf[f]);
+      Yn[f * n + p] = yn[f];
+    }
+    double Pn[PD];
+    rsf_patch_pre<NC>(a.fp, cf, yn, Pn);
+    store_pre<NC>(Pn, a.Pb[(s + 1) & 1], n, p);
+    write_vel_pre<VT, NC>(Pn, static_cast<VT*>(a.vout), p, a.vpl);
+    if (!rsf_patch_ok<NC>(a.fp, cf, yn)) atomicOr(&c->stage_fail, 1);
+  } else {
+    // reductions of the state y (initial state or the new solution)
+    if (s == S - 1) {
+      double en = 0.0;
+#pragma unroll
+      for (int f = 0; f < D; ++f) {
+        const double e = o.ws[f] + a.tb.bh[S - 1] * kf[f];
+        const double ye = o.x0[f] + hstep * e;
+        const double tol = a.so.atol[f] + a.so.rtol * fmax(fabs(y[f]), fabs(ye));
+        if (tol > 0.0) en = fmax(en, fabs(y[f] - ye) / tol);
+        const double dm = a.Xmon[f * n + p] - y[f];
+        r9[RED_DX2] += dm * dm;
+      }
+      r9[RED_ERR] = en;
+    }
+    const double vm = pre_vmag<NC>(o.P);
+    r9[RED_VMAX] = vm;
+    r9[RED_VSUM] = vm;
+    r9[RED_SLIP] = y[NC + 2];
+    r9[RED_MU] = y[1] / y[NC + 1];
+    r9[RED_SMAX] = y[NC + 1];
+    r9[RED_SMIN] = y[NC + 1];
+#pragma unroll
+    for (int f = 0; f < D; ++f) r9[RED_XN2] += y[f] * y[f];
+  }
+}
+
+// one wave's reductions into column col of partials[kNRed][ncol]; lanes
+// without a patch hold the neutral values
+__device__ __forceinline__ void write_partials(const StageArgs& a, const double* r9, int col,
+                                               int ncol) {
+#pragma unroll
+  for (int k = 0; k < kNRed; ++k) {
+    const double q = wave_reduce(r9[k], red_kind(k));
+    if ((threadIdx.x & 63) == 0) a.partials[(size_t)k * ncol + col] = q;
+  }
+}
 
 template <typename TK, int NC, int G>
 __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(StageArgs a) {
   using VT = typename KTraits<TK>::VT;
-  constexpr int D = Dim<NC>::v;
-  constexpr int PD = PreDim<NC>::v;
   __shared__ double rs[kStagePPB][G];
   DevCtl* c = a.ctl;
   // The control fields the launch needs are read with the halted flag, in
@@ -97,46 +241,56 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
   const double* X = xc ? a.Xb[1] : a.Xb[0];
   double* Xn = xc ? a.Xb[0] : a.Xb[1];
   const bool reduce = (s == 0) || (s == S - 1);
-  const bool mid = (s > 0) && (s < S - 1);
-  // Epilogue operands with addresses known at launch: the state-only
-  // record of Y_s, the loading rates, the accepted state X and the
-  // tableau-weighted sum of the earlier stage derivatives.  Wave 0 requests
-  // them before it streams its rows, so the epilogue starts on data already
-  // in registers.  (Fetching the remaining operands as well was measured
-  // slower: the two workgroups sharing a CU then stream in lockstep
-  // instead of one after the other.)
-  double P[PD], sn[NC], snn = 0.0, x0[D], ws[D];
+  // Wave 0 requests the epilogue operands before it streams its rows.
+  // (Fetching the remaining operands as well was measured slower: the two
+  // workgroups sharing a CU then stream in lockstep instead of one after
+  // the other.)
+  StageOperands<NC> op;
+  stage_operands<NC>(a, ks, sst, X, p, wave == 0 && t < kStagePPB && p < n, op);
+  // Split layout (RSFG_STAGE_SPLIT, for at most 16 rows per workgroup):
+  // the workgroup's kStagePPB G rows streamed by all its waves together
+  // (wave w takes every kStagePPB-th column chunk); part[w][j G + k] is
+  // wave w's share of row k of patch j.  Otherwise each wave streams the
+  // G rows of its own patch.
+  constexpr bool kSplit = RSFG_STAGE_SPLIT && kStagePPB * G <= 16;
+  constexpr int RW = kSplit ? kStagePPB * G : 1;
+  __shared__ double part[kStagePPB][RW];
+  if constexpr (kSplit) {
+    const int w = __builtin_amdgcn_readfirstlane(wave);
+    const int p0 = blockIdx.x * kStagePPB;
+    double acc[RW];
 #pragma unroll
-  for (int k = 0; k < PD; ++k) P[k] = 0.0;
-#pragma unroll
-  for (int k = 0; k < NC; ++k) sn[k] = 0.0;
-#pragma unroll
-  for (int f = 0; f < D; ++f) x0[f] = ws[f] = 0.0;
-  if (wave == 0 && t < kStagePPB && p < n) {
-    if (s > 0) {
-      const double* Ps = a.Pb[s & 1];
-#pragma unroll
-      for (int k = 0; k < PD; ++k) P[k] = Ps[k * n + p];
-#pragma unroll
-      for (int f = 0; f < D; ++f) x0[f] = X[f * n + p];
-      // sum_j A[s+1][j] K_j over j < s (middle stages), or
-      // sum_j bh[j] K_j over j < S - 1 (final stage)
-      const int nj = mid ? s : S - 1;
-#pragma unroll
-      for (int j = 0; j < kMaxStages; ++j) {
-        if (j >= nj) break;
-        const double w = mid ? a.tb.A[s + 1][j] : a.tb.bh[j];
-        if (w == 0.0) continue;
-        const double* Kj = a.Stg[0] + (size_t)ks[j] * sst;
-#pragma unroll
-        for (int f = 0; f < D; ++f) ws[f] += w * Kj[f * n + p];
-      }
+    for (int r = 0; r < RW; ++r) acc[r] = 0.0;
+    if constexpr (std::is_same<TK, bf16nf_t>::value) {
+      gemv_split_acc<bf16_t, RW, kStagePPB>(static_cast<const bf16_t*>(a.K), a.ld, p0 * G,
+                                            a.rows_all, a.cols, w,
+                                            static_cast<const float*>(a.vin), acc);
+      gemv_nf_lo_split<G, kStagePPB>(a.Knf_lo, a.nf_off, a.nf_list, p0, n, w,
+                                     static_cast<const float*>(a.vin), acc);
+    } else {
+      gemv_split_acc<TK, RW, kStagePPB>(static_cast<const TK*>(a.K), a.ld, p0 * G, a.rows_all,
+                                        a.cols, w, static_cast<const VT*>(a.vin), acc);
     }
 #pragma unroll
-    for (int k = 0; k < NC; ++k) sn[k] = a.sinc[NC * p + k];
-    snn = a.sinc[NC * n + p];
-  }
-  {
+    for (int r = 0; r < RW; ++r) acc[r] = wave_sum(acc[r]);
+    if (lane == 0) {
+#pragma unroll
+      for (int r = 0; r < RW; ++r) part[w][r] = acc[r];
+    }
+    __syncthreads();
+    if (wave != 0) return;
+    // lane t < kStagePPB combines the waves' shares of its own patch in a
+    // fixed order
+    if (t < kStagePPB) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) {
+        double v = 0.0;
+#pragma unroll
+        for (int q = 0; q < kStagePPB; ++q) v += part[q][t * G + k];
+        rs[t][k] = v;
+      }
+    }
+  } else {
     const int pw = min(blockIdx.x * kStagePPB + wave, n - 1);
     double rowsum[G];
     if constexpr (std::is_same<TK, bf16nf_t>::value) {
@@ -152,84 +306,83 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
 #pragma unroll
       for (int k = 0; k < G; ++k) rs[wave][k] = rowsum[k];
     }
+    __syncthreads();
   }
-  __syncthreads();
   if (wave != 0) return;
-  {
-    double r9[kNRed];
-    r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
-    r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
-    r9[RED_DX2] = 0.0;
-    if (t < kStagePPB && p < n) {
-      const PatchCoef cf = load_coef(a.coef, n, p);
-      double y[D], kf[D], td[NC];
-      if (reduce) {
-        const double* Yc = (s == 0) ? X : Xn;
+  double r9[kNRed];
+  r9_neutral(r9);
+  if (t < kStagePPB && p < n) {
+    double rsum[G];
 #pragma unroll
-        for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
-      }
-      if (s == 0) rsf_patch_pre<NC>(a.fp, cf, y, P);
+    for (int k = 0; k < G; ++k) rsum[k] = rs[t][k];
+    stage_epilogue<VT, NC, G>(a, c, ks, sst, hstep, X, Xn, p, rsum, op, r9);
+  }
+  if (reduce) write_partials(a, r9, blockIdx.x, gridDim.x);
+}
+
+// Persistent stage kernel of the FP32-input formats (FP32, bf16 hi+lo,
+// bf16, bf16 near-field): one workgroup of kPW waves per CU, x copied into
+// LDS once per launch.  Wave g of the grid owns the ppw consecutive patches
+// [g ppw, (g + 1) ppw) (ppw <= 64): it streams their G rows one patch after
+// the other against the LDS copy of x, leaves patch j's row sums on lane j,
+// then runs the epilogue of all its patches at once, one lane each, on
+// operands it requested before streaming.  Each wave writes its own column
+// of the reductions.  Against the per-patch kernel this removes the x
+// slices' L2 traffic and the per-workgroup barrier, launch and epilogue
+// gaps; every sum is formed in the same order (same bits).
+constexpr int kPW = 8;          // waves per workgroup (two per SIMD)
+constexpr int kXL = 36864;      // floats of x in LDS: 144 KiB of the CU's 160 KiB
+
+template <typename TK, int NC, int G>
+__global__ __launch_bounds__(64 * kPW, 1) void k_stage_p(StageArgs a, int ppw) {
+  using VT = typename KTraits<TK>::VT;
+  using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
+  static_assert(std::is_same<VT, float>::value, "FP32-input formats only");
+  __shared__ __attribute__((aligned(16))) float xs[kXL];
+  DevCtl* c = a.ctl;
+  const int halted = c->halted, xc = c->xcur;
+  const double hstep = c->h;
+  int ks[kMaxStages];
 #pragma unroll
-      for (int k = 0; k < NC; ++k) td[k] = rs[t][k];
-      const double sdn = (G > NC) ? rs[t][NC] : 0.0;
-      rsf_patch_finish<NC>(a.fp, P, td, sdn, sn, snn, kf);
-      int kss = ks[0];
+  for (int j = 0; j < kMaxStages; ++j) ks[j] = c->kslot[j];
+  if (halted) return;
+  const size_t sst = a.Stg[1] - a.Stg[0];
+  const int n = a.n, S = a.tb.s, s = a.s;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int g = blockIdx.x * kPW + wave;
+  const int pb = min(g * ppw, n), pe = min(pb + ppw, n);
+  const int p = pb + lane;
+  const bool own = p < pe;
+  const double* X = xc ? a.Xb[1] : a.Xb[0];
+  double* Xn = xc ? a.Xb[0] : a.Xb[1];
+  const bool reduce = (s == 0) || (s == S - 1);
+  StageOperands<NC> op;
+  stage_operands<NC>(a, ks, sst, X, p, own, op);
+  x_to_lds<TH>(static_cast<const float*>(a.vin), a.cols, xs);
+  __syncthreads();
+  double rsum[G];
 #pragma unroll
-      for (int j = 1; j < kMaxStages; ++j) kss = (j == s) ? ks[j] : kss;
-      double* Ks = a.Stg[0] + (size_t)kss * sst;
+  for (int k = 0; k < G; ++k) rsum[k] = 0.0;
+  for (int q = pb; q < pe; ++q) {
+    double acc[G];
 #pragma unroll
-      for (int f = 0; f < D; ++f) Ks[f * n + p] = kf[f];
-      if (mid) {
-        // next stage state Y_{s+1} = X + h (sum_{j<s} A[s+1][j] K_j + A[s+1][s] K_s)
-        // and its state-only record
-        const double w = a.tb.A[s + 1][s];
-        double* Yn = (s + 1 == S - 1) ? Xn : a.Yb[(s + 1) & 1];
-        double yn[D];
-#pragma unroll
-        for (int f = 0; f < D; ++f) {
-          yn[f] = x0[f] + hstep * (ws[f] + w * kf[f]);
-          Yn[f * n + p] = yn[f];
-        }
-        double Pn[PD];
-        rsf_patch_pre<NC>(a.fp, cf, yn, Pn);
-        store_pre<NC>(Pn, a.Pb[(s + 1) & 1], n, p);
-        write_vel_pre<VT, NC>(Pn, static_cast<VT*>(a.vout), p, a.vpl);
-        if (!rsf_patch_ok<NC>(a.fp, cf, yn)) atomicOr(&c->stage_fail, 1);
-      } else {
-        // reductions of the state y (initial state or the new solution)
-        if (s == S - 1) {
-          double en = 0.0;
-#pragma unroll
-          for (int f = 0; f < D; ++f) {
-            const double e = ws[f] + a.tb.bh[S - 1] * kf[f];
-            const double ye = x0[f] + hstep * e;
-            const double tol = a.so.atol[f] + a.so.rtol * fmax(fabs(y[f]), fabs(ye));
-            if (tol > 0.0) en = fmax(en, fabs(y[f] - ye) / tol);
-            const double dm = a.Xmon[f * n + p] - y[f];
-            r9[RED_DX2] += dm * dm;
-          }
-          r9[RED_ERR] = en;
-        }
-        const double vm = pre_vmag<NC>(P);
-        r9[RED_VMAX] = vm;
-        r9[RED_VSUM] = vm;
-        r9[RED_SLIP] = y[NC + 2];
-        r9[RED_MU] = y[1] / y[NC + 1];
-        r9[RED_SMAX] = y[NC + 1];
-        r9[RED_SMIN] = y[NC + 1];
-#pragma unroll
-        for (int f = 0; f < D; ++f) r9[RED_XN2] += y[f] * y[f];
-      }
+    for (int k = 0; k < G; ++k) acc[k] = 0.0;
+    gemv_lds_acc<TH, G, 2>(static_cast<const TH*>(a.K), a.ld, q * G, a.cols, xs, acc);
+    if constexpr (std::is_same<TK, bf16nf_t>::value) {
+      const int o = a.nf_off[q];
+      gemv_nf_lo_lds<G>(a.Knf_lo + (size_t)o * G * kNfChunk, a.nf_list + o,
+                        a.nf_off[q + 1] - o, xs, acc);
     }
-    if (reduce) {
-      // lanes without a patch hold the neutral values set above
 #pragma unroll
-      for (int k = 0; k < kNRed; ++k) {
-        const double q = wave_reduce(r9[k], red_kind(k));
-        if (t == 0) a.partials[(size_t)k * gridDim.x + blockIdx.x] = q;
-      }
+    for (int k = 0; k < G; ++k) {
+      const double tot = wave_sum(acc[k]);
+      rsum[k] = (lane == q - pb) ? tot : rsum[k];
     }
   }
+  double r9[kNRed];
+  r9_neutral(r9);
+  if (own) stage_epilogue<VT, NC, G>(a, c, ks, sst, hstep, X, Xn, p, rsum, op, r9);
+  if (reduce) write_partials(a, r9, g, gridDim.x * kPW);
 }
 
 // Additive (IMEX) step of the scalar layout, one launch per stage with the
@@ -461,12 +614,46 @@ struct Launcher {
   virtual void stage(const StageArgs& a, hipStream_t st) = 0;
   virtual void ctrl(const StageArgs& a, hipStream_t st) = 0;
   virtual void imex(const StageArgs& a, hipStream_t st) = 0;
-  virtual int wg_count(int n) const = 0;
+  // columns of partials[kNRed][.] the reducing stages write (k_ctrl reads
+  // that many)
+  virtual int red_cols() const = 0;
 };
+
+// Persistent stage kernel geometry for n patches and cols columns: ppw
+// patches per wave and the workgroup count, or nwg = 0 where the per-patch
+// kernel runs (FP64 entries, x larger than the LDS budget, more than 64
+// patches per wave, or RSFG_STAGE_PERSIST=0 in the environment).
+template <typename TK>
+static void persist_geometry(int n, int cols, bool allow, int* nwg, int* ppw) {
+  *nwg = 0;
+  *ppw = 0;
+  using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
+  if constexpr (!std::is_same<typename KTraits<TK>::VT, float>::value) {
+    return;
+  } else {
+    const char* e = getenv("RSFG_STAGE_PERSIST");
+    if (!allow || (e && e[0] == '0') || n <= 0) return;
+    if (lds_x_floats<TH>(cols) > kXL) return;
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        ncu <= 0)
+      return;
+    const long waves = (long)ncu * kPW;
+    const int q = (int)((n + waves - 1) / waves);
+    if (q > 64) return;
+    *ppw = q;
+    *nwg = (n + kPW * q - 1) / (kPW * q);
+  }
+}
 
 template <typename TK, int NC, int G>
 struct LauncherT : Launcher {
   using VT = typename KTraits<TK>::VT;
+  int nwg_p = 0, ppw_p = 0, n_ = 0;
+  LauncherT(int n, int cols, bool allow_persist) : n_(n) {
+    persist_geometry<TK>(n, cols, allow_persist, &nwg_p, &ppw_p);
+  }
   void init_vel(const StageArgs& a, hipStream_t st) override {
     hipLaunchKernelGGL((k_init_vel<VT, NC>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);
   }
@@ -474,40 +661,52 @@ struct LauncherT : Launcher {
     hipLaunchKernelGGL((k_prep<VT, NC>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);
   }
   void stage(const StageArgs& a, hipStream_t st) override {
-    hipLaunchKernelGGL((k_stage<TK, NC, G>), dim3(wg_count(a.n)), dim3(64 * kStagePPB), 0, st,
-                       a);
+    if constexpr (std::is_same<VT, float>::value) {
+      if (nwg_p > 0) {
+        hipLaunchKernelGGL((k_stage_p<TK, NC, G>), dim3(nwg_p), dim3(64 * kPW), 0, st, a, ppw_p);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_stage<TK, NC, G>), dim3(wg_per_patch(a.n)), dim3(64 * kStagePPB), 0,
+                       st, a);
   }
   void ctrl(const StageArgs& a, hipStream_t st) override {
-    hipLaunchKernelGGL((k_ctrl<NC>), dim3(1), dim3(kCtrlThreads), 0, st, a, wg_count(a.n));
+    hipLaunchKernelGGL((k_ctrl<NC>), dim3(1), dim3(kCtrlThreads), 0, st, a, red_cols());
   }
   // additive stages exist for the scalar layout only (the solve rejects
-  // the method otherwise)
+  // the method otherwise); their launcher is built without the persistent
+  // kernel, so k_ctrl reads the per-patch kernel's columns
   void imex(const StageArgs& a, hipStream_t st) override {
     if constexpr (NC == 1) {
-      hipLaunchKernelGGL((k_imex<TK, G>), dim3(wg_count(a.n)), dim3(64 * kStagePPB), 0, st, a);
+      hipLaunchKernelGGL((k_imex<TK, G>), dim3(wg_per_patch(a.n)), dim3(64 * kStagePPB), 0, st,
+                         a);
     }
   }
   // one patch per wave, kStagePPB waves per workgroup
-  int wg_count(int n) const override { return (n + kStagePPB - 1) / kStagePPB; }
+  static int wg_per_patch(int n) { return (n + kStagePPB - 1) / kStagePPB; }
+  int red_cols() const override { return nwg_p > 0 ? nwg_p * kPW : wg_per_patch(n_); }
 };
 
 template <typename TK>
-static Launcher* make_launcher_t(int ncomp, int G) {
+static Launcher* make_launcher_t(int ncomp, int G, int n, int cols, bool allow) {
   if (ncomp == 1) {
-    if (G == 1) return new LauncherT<TK, 1, 1>();
-    return new LauncherT<TK, 1, 2>();
+    if (G == 1) return new LauncherT<TK, 1, 1>(n, cols, allow);
+    return new LauncherT<TK, 1, 2>(n, cols, allow);
   }
-  if (G == 2) return new LauncherT<TK, 2, 2>();
-  return new LauncherT<TK, 2, 3>();
+  if (G == 2) return new LauncherT<TK, 2, 2>(n, cols, allow);
+  return new LauncherT<TK, 2, 3>(n, cols, allow);
 }
 
-static Launcher* make_launcher(int prec, int ncomp, int G) {
+// allow_persist: false for the additive (IMEX) method, whose stage kernel
+// is per-patch
+static Launcher* make_launcher(const rsfg_ctx* h, int prec, bool allow_persist) {
+  const int nc = h->ncomp, G = h->G, n = h->n, cols = h->cols;
   switch (prec) {
-    case RSFG_PREC_FP32: return make_launcher_t<float>(ncomp, G);
-    case RSFG_PREC_BF16X2: return make_launcher_t<bf16x2_t>(ncomp, G);
-    case RSFG_PREC_BF16: return make_launcher_t<bf16_t>(ncomp, G);
-    case RSFG_PREC_BF16NF: return make_launcher_t<bf16nf_t>(ncomp, G);
-    default: return make_launcher_t<double>(ncomp, G);
+    case RSFG_PREC_FP32: return make_launcher_t<float>(nc, G, n, cols, allow_persist);
+    case RSFG_PREC_BF16X2: return make_launcher_t<bf16x2_t>(nc, G, n, cols, allow_persist);
+    case RSFG_PREC_BF16: return make_launcher_t<bf16_t>(nc, G, n, cols, allow_persist);
+    case RSFG_PREC_BF16NF: return make_launcher_t<bf16nf_t>(nc, G, n, cols, allow_persist);
+    default: return make_launcher_t<double>(nc, G, n, cols, allow_persist);
   }
 }
 
@@ -679,8 +878,8 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
     const int rc0 = ensure_logs(h, ev_rows, mon_rows);
     if (rc0 != RSFG_OK) return rc0;
   }
-  Launcher* L = make_launcher(o->prec, h->ncomp, h->G);
-  if ((size_t)L->wg_count(h->n) * kNRed > h->partial_count) {
+  Launcher* L = make_launcher(h, o->prec, o->method != RSFG_ARKIMEX_L2);
+  if ((size_t)L->red_cols() * kNRed > h->partial_count) {
     delete L;
     set_error("rsfg_solve: partial buffer too small");
     return RSFG_ESTATE;
@@ -816,7 +1015,7 @@ int stage_bench(rsfg_ctx* h, int prec, int nrep, double* ms) {
   rsfg_solver_defaults(&o);
   o.prec = prec;
   h->cont_ok = false;
-  Launcher* L = make_launcher(prec, h->ncomp, h->G);
+  Launcher* L = make_launcher(h, prec, true);
   StageArgs a;
   fill_stage_args(h, &o, K, a);
   int rc = start_solve(h, &o, h->t + 1e30, a, L);
