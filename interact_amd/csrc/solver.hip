@@ -76,18 +76,22 @@ static_assert(kStagePPB >= 1 && kStagePPB <= 16, "one wave per patch, at most 10
 
 // Epilogue operands of one patch whose addresses are known at launch: the
 // state-only record of Y_s, the accepted state X, the tableau-weighted sum
-// of the earlier stage derivatives and the stressing-rate loading.  The
-// stage kernels request them before streaming their rows, so the epilogue
-// starts on data already in registers.
+// of the earlier stage derivatives and the stressing-rate loading; with
+// kAll also the friction coefficients, the state itself (reducing stages)
+// and the monitor snapshot (final stage).  The stage kernels request them
+// before streaming their rows, so the epilogue starts on data already in
+// registers.
 template <int NC>
 struct StageOperands {
   double P[PreDim<NC>::v], sn[NC], snn, x0[Dim<NC>::v], ws[Dim<NC>::v];
+  PatchCoef cf;
+  double y[Dim<NC>::v], xm[Dim<NC>::v];
 };
 
-template <int NC>
+template <int NC, bool kAll>
 __device__ __forceinline__ void stage_operands(const StageArgs& a, const int* ks, size_t sst,
-                                               const double* X, int p, bool own,
-                                               StageOperands<NC>& o) {
+                                               const double* X, const double* Xn, int p,
+                                               bool own, StageOperands<NC>& o) {
   constexpr int D = Dim<NC>::v;
   constexpr int PD = PreDim<NC>::v;
   const int n = a.n, S = a.tb.s, s = a.s;
@@ -98,7 +102,8 @@ __device__ __forceinline__ void stage_operands(const StageArgs& a, const int* ks
   for (int k = 0; k < NC; ++k) o.sn[k] = 0.0;
   o.snn = 0.0;
 #pragma unroll
-  for (int f = 0; f < D; ++f) o.x0[f] = o.ws[f] = 0.0;
+  for (int f = 0; f < D; ++f) o.x0[f] = o.ws[f] = o.y[f] = o.xm[f] = 0.0;
+  o.cf.a = o.cf.b = o.cf.dc = 0.0;
   if (!own) return;
   if (s > 0) {
     const double* Ps = a.Pb[s & 1];
@@ -108,20 +113,33 @@ __device__ __forceinline__ void stage_operands(const StageArgs& a, const int* ks
     for (int f = 0; f < D; ++f) o.x0[f] = X[f * n + p];
     // sum_j A[s+1][j] K_j over j < s (middle stages), or
     // sum_j bh[j] K_j over j < S - 1 (final stage)
+    // (a fully unrolled loop: ks stays in registers, every index constant)
     const int nj = mid ? s : S - 1;
 #pragma unroll
     for (int j = 0; j < kMaxStages; ++j) {
-      if (j >= nj) break;
-      const double w = mid ? a.tb.A[s + 1][j] : a.tb.bh[j];
-      if (w == 0.0) continue;
-      const double* Kj = a.Stg[0] + (size_t)ks[j] * sst;
+      const double w = (j < nj) ? (mid ? a.tb.A[s + 1][j] : a.tb.bh[j]) : 0.0;
+      if (w != 0.0) {
+        const double* Kj = a.Stg[0] + (size_t)ks[j] * sst;
 #pragma unroll
-      for (int f = 0; f < D; ++f) o.ws[f] += w * Kj[f * n + p];
+        for (int f = 0; f < D; ++f) o.ws[f] += w * Kj[f * n + p];
+      }
     }
   }
 #pragma unroll
   for (int k = 0; k < NC; ++k) o.sn[k] = a.sinc[NC * p + k];
   o.snn = a.sinc[NC * n + p];
+  if constexpr (kAll) {
+    o.cf = load_coef(a.coef, n, p);
+    if (!mid) {
+      const double* Yc = (s == 0) ? X : Xn;
+#pragma unroll
+      for (int f = 0; f < D; ++f) o.y[f] = Yc[f * n + p];
+    }
+    if (s == S - 1) {
+#pragma unroll
+      for (int f = 0; f < D; ++f) o.xm[f] = a.Xmon[f * n + p];
+    }
+  }
 }
 
 __device__ __forceinline__ void r9_neutral(double* r9) {
@@ -135,8 +153,8 @@ __device__ __forceinline__ void r9_neutral(double* r9) {
 // slot, then Y_{s+1}, its state-only record and v(Y_{s+1}) (middle stages),
 // or the patch's terms of the launch's reductions r9 (initial evaluation
 // and final stage: error norm, slip-rate and stress extremes, ...).
-template <typename VT, int NC, int G>
-__device__ __forceinline__ void stage_epilogue(const StageArgs& a, DevCtl* c, const int* ks,
+template <typename VT, int NC, int G, bool kAll>
+__device__ __forceinline__ void stage_epilogue(const StageArgs& a, DevCtl* c, int kss,
                                                size_t sst, double hstep, const double* X,
                                                double* Xn, int p, const double* rsum,
                                                StageOperands<NC>& o, double* r9) {
@@ -145,21 +163,18 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs& a, DevCtl* c, co
   const int n = a.n, S = a.tb.s, s = a.s;
   const bool reduce = (s == 0) || (s == S - 1);
   const bool mid = (s > 0) && (s < S - 1);
-  const PatchCoef cf = load_coef(a.coef, n, p);
+  const PatchCoef cf = kAll ? o.cf : load_coef(a.coef, n, p);
   double y[D], kf[D], td[NC];
   if (reduce) {
     const double* Yc = (s == 0) ? X : Xn;
 #pragma unroll
-    for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
+    for (int f = 0; f < D; ++f) y[f] = kAll ? o.y[f] : Yc[f * n + p];
   }
   if (s == 0) rsf_patch_pre<NC>(a.fp, cf, y, o.P);
 #pragma unroll
   for (int k = 0; k < NC; ++k) td[k] = rsum[k];
   const double sdn = (G > NC) ? rsum[NC] : 0.0;
   rsf_patch_finish<NC>(a.fp, o.P, td, sdn, o.sn, o.snn, kf);
-  int kss = ks[0];
-#pragma unroll
-  for (int j = 1; j < kMaxStages; ++j) kss = (j == s) ? ks[j] : kss;
   double* Ks = a.Stg[0] + (size_t)kss * sst;
 #pragma unroll
   for (int f = 0; f < D; ++f) Ks[f * n + p] = kf[f];
@@ -189,7 +204,7 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs& a, DevCtl* c, co
         const double ye = o.x0[f] + hstep * e;
         const double tol = a.so.atol[f] + a.so.rtol * fmax(fabs(y[f]), fabs(ye));
         if (tol > 0.0) en = fmax(en, fabs(y[f] - ye) / tol);
-        const double dm = a.Xmon[f * n + p] - y[f];
+        const double dm = (kAll ? o.xm[f] : a.Xmon[f * n + p]) - y[f];
         r9[RED_DX2] += dm * dm;
       }
       r9[RED_ERR] = en;
@@ -232,6 +247,7 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
   int ks[kMaxStages];
 #pragma unroll
   for (int j = 0; j < kMaxStages; ++j) ks[j] = c->kslot[j];
+  const int kss = c->kslot[a.s];   // this stage's derivative slot
   if (halted) return;
   const size_t sst = a.Stg[1] - a.Stg[0];
   const int n = a.n, S = a.tb.s, s = a.s;
@@ -246,7 +262,7 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
   // workgroups sharing a CU then stream in lockstep instead of one after
   // the other.)
   StageOperands<NC> op;
-  stage_operands<NC>(a, ks, sst, X, p, wave == 0 && t < kStagePPB && p < n, op);
+  stage_operands<NC, false>(a, ks, sst, X, Xn, p, wave == 0 && t < kStagePPB && p < n, op);
   // Split layout (RSFG_STAGE_SPLIT, for at most 16 rows per workgroup):
   // the workgroup's kStagePPB G rows streamed by all its waves together
   // (wave w takes every kStagePPB-th column chunk); part[w][j G + k] is
@@ -315,22 +331,28 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
     double rsum[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) rsum[k] = rs[t][k];
-    stage_epilogue<VT, NC, G>(a, c, ks, sst, hstep, X, Xn, p, rsum, op, r9);
+    stage_epilogue<VT, NC, G, false>(a, c, kss, sst, hstep, X, Xn, p, rsum, op, r9);
   }
   if (reduce) write_partials(a, r9, blockIdx.x, gridDim.x);
 }
 
 // Persistent stage kernel of the FP32-input formats (FP32, bf16 hi+lo,
 // bf16, bf16 near-field): one workgroup of kPW waves per CU, x copied into
-// LDS once per launch.  Wave g of the grid owns the ppw consecutive patches
-// [g ppw, (g + 1) ppw) (ppw <= 64): it streams their G rows one patch after
-// the other against the LDS copy of x, leaves patch j's row sums on lane j,
-// then runs the epilogue of all its patches at once, one lane each, on
-// operands it requested before streaming.  Each wave writes its own column
-// of the reductions.  Against the per-patch kernel this removes the x
-// slices' L2 traffic and the per-workgroup barrier, launch and epilogue
-// gaps; every sum is formed in the same order (same bits).
+// LDS once per launch.  Wave w of workgroup b owns the ppw consecutive
+// patches starting at (kPW b + w) ppw (ppw <= kMaxPPW) and streams their G
+// rows one patch after the other against the LDS copy of x.  The row sums
+// of the workgroup's kPW ppw patches then meet in LDS and kEW waves, one
+// per SIMD, run the epilogue with one lane per patch: the epilogue is a
+// long FP64 chain at the end of the launch, when no stream is left to hide
+// it, and one wave per SIMD halves its issue time against two.  The
+// epilogue waves request their patches' operands before streaming.  Each
+// epilogue wave writes its own column of the reductions.  Against the
+// per-patch kernel this removes the x slices' L2 traffic and the
+// per-workgroup launch, barrier and epilogue gaps; every sum is formed in
+// the same order.
 constexpr int kPW = 8;          // waves per workgroup (two per SIMD)
+constexpr int kEW = 4;          // epilogue waves per workgroup (one per SIMD)
+constexpr int kMaxPPW = 32;     // patches per wave: kPW kMaxPPW / kEW <= 64 lanes
 constexpr int kXL = 36864;      // floats of x in LDS: 144 KiB of the CU's 160 KiB
 
 template <typename TK, int NC, int G>
@@ -338,31 +360,32 @@ __global__ __launch_bounds__(64 * kPW, 1) void k_stage_p(StageArgs a, int ppw) {
   using VT = typename KTraits<TK>::VT;
   using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
   static_assert(std::is_same<VT, float>::value, "FP32-input formats only");
+  static_assert(kPW * kMaxPPW / kEW <= 64, "one lane per patch in the epilogue waves");
   __shared__ __attribute__((aligned(16))) float xs[kXL];
+  __shared__ double red[kPW * kMaxPPW][G];
   DevCtl* c = a.ctl;
   const int halted = c->halted, xc = c->xcur;
   const double hstep = c->h;
   int ks[kMaxStages];
 #pragma unroll
   for (int j = 0; j < kMaxStages; ++j) ks[j] = c->kslot[j];
+  const int kss = c->kslot[a.s];   // this stage's derivative slot
   if (halted) return;
   const size_t sst = a.Stg[1] - a.Stg[0];
   const int n = a.n, S = a.tb.s, s = a.s;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int g = blockIdx.x * kPW + wave;
-  const int pb = min(g * ppw, n), pe = min(pb + ppw, n);
-  const int p = pb + lane;
-  const bool own = p < pe;
+  const int p0 = blockIdx.x * kPW * ppw;              // the workgroup's first patch
+  const int pb = min(p0 + wave * ppw, n), pe = min(pb + ppw, n);
+  const int ne = kPW * ppw / kEW;                     // patches per epilogue wave
+  const int pp = p0 + wave * ne + lane;               // this lane's epilogue patch
+  const bool own = wave < kEW && lane < ne && pp < n;
   const double* X = xc ? a.Xb[1] : a.Xb[0];
   double* Xn = xc ? a.Xb[0] : a.Xb[1];
   const bool reduce = (s == 0) || (s == S - 1);
   StageOperands<NC> op;
-  stage_operands<NC>(a, ks, sst, X, p, own, op);
+  stage_operands<NC, true>(a, ks, sst, X, Xn, pp, own, op);
   x_to_lds<TH>(static_cast<const float*>(a.vin), a.cols, xs);
   __syncthreads();
-  double rsum[G];
-#pragma unroll
-  for (int k = 0; k < G; ++k) rsum[k] = 0.0;
   for (int q = pb; q < pe; ++q) {
     double acc[G];
 #pragma unroll
@@ -376,13 +399,20 @@ __global__ __launch_bounds__(64 * kPW, 1) void k_stage_p(StageArgs a, int ppw) {
 #pragma unroll
     for (int k = 0; k < G; ++k) {
       const double tot = wave_sum(acc[k]);
-      rsum[k] = (lane == q - pb) ? tot : rsum[k];
+      if (lane == 0) red[q - p0][k] = tot;
     }
   }
+  __syncthreads();
+  if (wave >= kEW) return;
   double r9[kNRed];
   r9_neutral(r9);
-  if (own) stage_epilogue<VT, NC, G>(a, c, ks, sst, hstep, X, Xn, p, rsum, op, r9);
-  if (reduce) write_partials(a, r9, g, gridDim.x * kPW);
+  if (own) {
+    double rsum[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) rsum[k] = red[pp - p0][k];
+    stage_epilogue<VT, NC, G, true>(a, c, kss, sst, hstep, X, Xn, pp, rsum, op, r9);
+  }
+  if (reduce) write_partials(a, r9, blockIdx.x * kEW + wave, gridDim.x * kEW);
 }
 
 // Additive (IMEX) step of the scalar layout, one launch per stage with the
@@ -621,8 +651,8 @@ struct Launcher {
 
 // Persistent stage kernel geometry for n patches and cols columns: ppw
 // patches per wave and the workgroup count, or nwg = 0 where the per-patch
-// kernel runs (FP64 entries, x larger than the LDS budget, more than 64
-// patches per wave, or RSFG_STAGE_PERSIST=0 in the environment).
+// kernel runs (FP64 entries, x larger than the LDS budget, more than
+// kMaxPPW patches per wave, or RSFG_STAGE_PERSIST=0 in the environment).
 template <typename TK>
 static void persist_geometry(int n, int cols, bool allow, int* nwg, int* ppw) {
   *nwg = 0;
@@ -641,7 +671,7 @@ static void persist_geometry(int n, int cols, bool allow, int* nwg, int* ppw) {
       return;
     const long waves = (long)ncu * kPW;
     const int q = (int)((n + waves - 1) / waves);
-    if (q > 64) return;
+    if (q > kMaxPPW) return;
     *ppw = q;
     *nwg = (n + kPW * q - 1) / (kPW * q);
   }
@@ -684,7 +714,7 @@ struct LauncherT : Launcher {
   }
   // one patch per wave, kStagePPB waves per workgroup
   static int wg_per_patch(int n) { return (n + kStagePPB - 1) / kStagePPB; }
-  int red_cols() const override { return nwg_p > 0 ? nwg_p * kPW : wg_per_patch(n_); }
+  int red_cols() const override { return nwg_p > 0 ? nwg_p * kEW : wg_per_patch(n_); }
 };
 
 template <typename TK>
