@@ -386,10 +386,67 @@ static int matvec_t_launch(rsfg_ctx* h, const TK* K, const double* d_x, double* 
   return RSFG_OK;
 }
 
+// Persistent product of the FP32-input formats: one workgroup of kPW waves
+// per CU copies x into LDS once, then each wave streams upw consecutive
+// units against that copy (a unit is R = 2 rows of the plain formats, or
+// one patch's G rows and its near-field lo chunks for bf16nf).  Every row
+// sum is formed in the order of k_matvec / k_matvec_nf (same bits).
+template <typename TK, int R>
+__global__ __launch_bounds__(64 * kPW, 1) void k_matvec_p(const void* __restrict__ Kv,
+                                                         const bf16_t* __restrict__ lo,
+                                                         const int* __restrict__ off,
+                                                         const uint16_t* __restrict__ list,
+                                                         long ld, int rows, int cols,
+                                                         const float* __restrict__ x,
+                                                         double* __restrict__ y, int upw) {
+  using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
+  __shared__ __attribute__((aligned(16))) float xs[kXL];
+  x_to_lds<TH>(x, cols, xs);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nunit = (rows + R - 1) / R;
+  const int ub = min((blockIdx.x * kPW + wave) * upw, nunit), ue = min(ub + upw, nunit);
+  for (int u = ub; u < ue; ++u) {
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    gemv_lds_acc<TH, R, 2>(static_cast<const TH*>(Kv), ld, u * R, rows, cols, xs, acc);
+    if constexpr (std::is_same<TK, bf16nf_t>::value)
+      gemv_nf_lo_lds<R>(lo + (size_t)off[u] * R * kNfChunk, list + off[u], off[u + 1] - off[u],
+                        xs, acc);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const double tot = wave_sum(acc[r]);
+      if (lane == 0 && u * R + r < rows) y[u * R + r] = tot;
+    }
+  }
+}
+
+// Launches k_matvec_p when the format takes FP32 x, x fits the LDS copy and
+// the units spread over the CUs; returns false (nothing launched) otherwise.
+template <typename TK, int R>
+static bool matvec_persistent(rsfg_ctx* h, const void* K, const float* x, double* y) {
+  using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
+  if (lds_x_floats<TH>(h->cols) > kXL) return false;
+  const char* e = getenv("RSFG_STAGE_PERSIST");
+  if (e && e[0] == '0') return false;
+  int nwg = 0, upw = 0;
+  if (!persist_split(h->ncu, (h->rows_all + R - 1) / R, 64, &nwg, &upw)) return false;
+  hipLaunchKernelGGL((k_matvec_p<TK, R>), dim3(nwg), dim3(64 * kPW), 0, h->stream, K, h->Knf_lo,
+                     h->nf_off, h->nf_list, (long)h->ld, h->rows_all, h->cols, x, y, upw);
+  return true;
+}
+
 template <typename TK>
 static int matvec_launch(rsfg_ctx* h, const TK* K, const void* d_xvt, double* d_y) {
   constexpr int R = 2;
   using VT = typename KTraits<TK>::VT;
+  if constexpr (std::is_same<VT, float>::value) {
+    if (matvec_persistent<TK, R>(h, K, static_cast<const float*>(d_xvt), d_y)) {
+      RSFG_HIP(hipGetLastError());
+      return RSFG_OK;
+    }
+  }
   const int nb = (h->rows_all + 4 * R - 1) / (4 * R);
   hipLaunchKernelGGL((k_matvec<TK, R>), dim3(nb), dim3(256), 0, h->stream, K, (long)h->ld,
                      h->rows_all, h->cols, static_cast<const VT*>(d_xvt), d_y);
@@ -406,6 +463,14 @@ static int matvec_any(rsfg_ctx* h, int prec, const void* d_xvt, double* d_y) {
     case RSFG_PREC_BF16NF: {
       const dim3 grid((h->n + 3) / 4), block(256);
       const float* x = static_cast<const float*>(d_xvt);
+      bool done = false;
+      if (h->G == 1) done = matvec_persistent<bf16nf_t, 1>(h, h->Kb16, x, d_y);
+      else if (h->G == 2) done = matvec_persistent<bf16nf_t, 2>(h, h->Kb16, x, d_y);
+      else done = matvec_persistent<bf16nf_t, 3>(h, h->Kb16, x, d_y);
+      if (done) {
+        RSFG_HIP(hipGetLastError());
+        return RSFG_OK;
+      }
 #define RSFG_NF(GG)                                                                            \
   hipLaunchKernelGGL(k_matvec_nf<GG>, grid, block, 0, h->stream, h->Kb16, h->Knf_lo, h->nf_off, \
                      h->nf_list, (long)h->ld, h->n, h->cols, x, d_y)
@@ -618,6 +683,8 @@ int rsfg_create(int device, const rsfg_patch* pt, int n, rsfg_handle** out) {
   }
   rsfg_ctx* h = new rsfg_ctx();
   h->device = device;
+  if (hipDeviceGetAttribute(&h->ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+    h->ncu = 0;   // the persistent kernels stay off
   h->n = n;
   h->h_geo = new double[(size_t)kGeoFields * n];
   double* g = h->h_geo;
@@ -693,6 +760,7 @@ int rsfg_create_member(rsfg_handle* geom, rsfg_handle** out) {
   RSFG_HIP(hipSetDevice(geom->device));
   rsfg_ctx* h = new rsfg_ctx();
   h->device = geom->device;
+  h->ncu = geom->ncu;
   h->n = geom->n;
   h->borrowed = true;
   h->d_geo = geom->d_geo;

@@ -22,10 +22,6 @@
 
 #include "rsfg_common.h"
 
-#ifndef RSFG_GEMV_PIPE
-#define RSFG_GEMV_PIPE 0
-#endif
-
 namespace rsfg {
 
 // native clang vector types (the nontemporal builtin needs them)
@@ -142,10 +138,9 @@ struct ChunkFma<bf16_t, R> {
   }
 };
 
-// One 64-lane chunk of R rows held in registers: the loads of a chunk are
-// issued by load() and consumed by fma(), so a loop can keep the next
-// batch of chunks in flight while it computes the current one.  fma() does
-// the arithmetic of ChunkFma in the same order (same bits).
+// One 64-lane chunk of R rows held in registers (operator slices k, x
+// slices x); fma() does the arithmetic of ChunkFma in the same order (same
+// bits) for x slices taken from anywhere (the LDS copy of x below).
 template <typename TK>
 struct SliceT;
 template <>
@@ -180,13 +175,6 @@ struct Slice {
   static constexpr int NX = SliceT<TK>::NX;
   KV k[R];
   XV x[NX];
-  __device__ __forceinline__ void load(const TK* const* rp, int col,
-                                       const typename KTraits<TK>::VT* __restrict__ xp) {
-#pragma unroll
-    for (int i = 0; i < NX; ++i) x[i] = *reinterpret_cast<const XV*>(xp + col + 4 * i);
-#pragma unroll
-    for (int r = 0; r < R; ++r) k[r] = __builtin_nontemporal_load(reinterpret_cast<const KV*>(rp[r] + col));
-  }
   __device__ __forceinline__ void fma(double* acc) const {
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -250,15 +238,6 @@ template <int R>
 struct GemvUnroll<bf16_t, R> {
   static constexpr int U = (R == 1) ? 4 : RSFG_BF16_U;
 };
-#ifndef RSFG_PIPE_U
-#define RSFG_PIPE_U 1
-#endif
-// chunks per batch of the pipelined loop (two batches in registers)
-template <typename TK, int R>
-struct PipeUnroll {
-  static constexpr int U = (R == 1) ? GemvUnroll<TK, R>::U : RSFG_PIPE_U;
-};
-
 // Lane partial sums of rows [row0, row0 + R) over all columns, streamed by
 // ONE wave (rows past nrows - 1 are clamped to it; the caller discards
 // them), added into acc.
@@ -275,45 +254,10 @@ __device__ __forceinline__ void gemv_acc(const TK* __restrict__ K, long ld, int 
 #pragma unroll
   for (int r = 0; r < R; ++r) rp[r] = K + (size_t)min(row0 + r, nrows - 1) * ld;
   int c = 0;
-#if RSFG_GEMV_PIPE
-  // batches of PU chunks, two in registers: batch k + 1 is requested before
-  // batch k is computed, so a wave never waits with no load in flight
-  constexpr int PU = PipeUnroll<TK, R>::U;
-  const int nb = nch / PU;
-  if (nb > 0) {
-    Slice<TK, R> sa[PU], sb[PU];
-#pragma unroll
-    for (int u = 0; u < PU; ++u) sa[u].load(rp, u * CH + lo, x);
-    int k = 1;
-    for (; k + 1 < nb; k += 2) {
-#pragma unroll
-      for (int u = 0; u < PU; ++u) sb[u].load(rp, (k * PU + u) * CH + lo, x);
-#pragma unroll
-      for (int u = 0; u < PU; ++u) sa[u].fma(acc);
-#pragma unroll
-      for (int u = 0; u < PU; ++u) sa[u].load(rp, ((k + 1) * PU + u) * CH + lo, x);
-#pragma unroll
-      for (int u = 0; u < PU; ++u) sb[u].fma(acc);
-    }
-    if (k < nb) {
-#pragma unroll
-      for (int u = 0; u < PU; ++u) sb[u].load(rp, (k * PU + u) * CH + lo, x);
-#pragma unroll
-      for (int u = 0; u < PU; ++u) sa[u].fma(acc);
-#pragma unroll
-      for (int u = 0; u < PU; ++u) sb[u].fma(acc);
-    } else {
-#pragma unroll
-      for (int u = 0; u < PU; ++u) sa[u].fma(acc);
-    }
-  }
-  c = nb * PU;
-#else
   for (; c + U <= nch; c += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) ChunkFma<TK, R>::run(rp, (c + u) * CH + lo, x, acc);
   }
-#endif
   for (; c < nch; ++c) ChunkFma<TK, R>::run(rp, c * CH + lo, x, acc);
 }
 
@@ -331,28 +275,6 @@ __device__ __forceinline__ void gemv_wave(const TK* __restrict__ K, long ld, int
   gemv_acc<TK, R>(K, ld, row0, nrows, cols, x, acc);
 #pragma unroll
   for (int r = 0; r < R; ++r) out[r] = wave_sum(acc[r]);
-}
-
-// ---- workgroup-split row block
-// The NW waves of one workgroup stream the same RW consecutive rows; wave w
-// takes the column chunks w, w + NW, w + 2 NW, ...  Each slice of x is then
-// loaded once per workgroup instead of once per wave (for R = 2 rows of
-// two-byte entries per wave the x slices are as many bytes as the operator
-// rows), and the workgroup's loads of one row cover NW consecutive chunks
-// at a time.  Lane partial sums of the RW rows are added into acc; the
-// caller combines the waves.  wave must be wave-uniform.
-template <typename TK, int RW, int NW>
-__device__ __forceinline__ void gemv_split_acc(const TK* __restrict__ K, long ld, int row0,
-                                               int nrows, int cols, int wave,
-                                               const typename KTraits<TK>::VT* __restrict__ x,
-                                               double* acc) {
-  constexpr int CH = chunk_elems<TK>();
-  const int lo = (threadIdx.x & 63) * KTraits<TK>::VEC;
-  const int nch = (cols + CH - 1) / CH;
-  const TK* rp[RW];
-#pragma unroll
-  for (int r = 0; r < RW; ++r) rp[r] = K + (size_t)min(row0 + r, nrows - 1) * ld;
-  for (int c = wave; c < nch; c += NW) ChunkFma<TK, RW>::run(rp, c * CH + lo, x, acc);
 }
 
 // ---- bf16 hi everywhere + bf16 lo in the near-field chunks ("nf")
@@ -411,6 +333,22 @@ __device__ __forceinline__ void gemv_nf_wave(const bf16_t* __restrict__ Khi,
 // lane-interleaved: lane l's entries 8l .. 8l+3 at 4l and 8l+4 .. 8l+7 at
 // 256 + 4l, so both 16-byte LDS reads of a wave are contiguous.  The
 // arithmetic is that of ChunkFma, in the same order.
+constexpr int kPW = 8;       // waves per workgroup of the persistent kernels (two per SIMD)
+constexpr int kXL = 36864;   // floats of x in LDS: 144 KiB of the CU's 160 KiB
+
+// Persistent launch over ncu CUs for `units` work units: upw consecutive
+// units per wave and nwg workgroups of kPW waves.  False (no launch) when
+// a wave would get more than max_upw units.
+inline bool persist_split(int ncu, long units, int max_upw, int* nwg, int* upw) {
+  if (ncu <= 0 || units <= 0) return false;
+  const long waves = (long)ncu * kPW;
+  const long q = (units + waves - 1) / waves;
+  if (q > max_upw) return false;
+  *upw = (int)q;
+  *nwg = (int)((units + kPW * q - 1) / (kPW * q));
+  return true;
+}
+
 template <typename TK>
 constexpr int lds_x_floats(int cols) {
   return (cols + chunk_elems<TK>() - 1) / chunk_elems<TK>() * chunk_elems<TK>();
@@ -447,17 +385,19 @@ __device__ __forceinline__ void lds_x_slice(const float* xs, int c, Slice<TK, R>
   }
 }
 
-// Lane partial sums of rows [row0, row0 + R) (all valid) against the LDS
-// copy of x, U chunks per loop trip, added into acc.
+// Lane partial sums of rows [row0, row0 + R) against the LDS copy of x,
+// U chunks per loop trip, added into acc (rows past nrows - 1 are clamped
+// to it; the caller discards them).
 template <typename TK, int R, int U>
 __device__ __forceinline__ void gemv_lds_acc(const TK* __restrict__ K, long ld, int row0,
-                                             int cols, const float* xs, double* acc) {
+                                             int nrows, int cols, const float* xs,
+                                             double* acc) {
   constexpr int CH = chunk_elems<TK>();
   const int lo = (threadIdx.x & 63) * KTraits<TK>::VEC;
   const int nch = (cols + CH - 1) / CH;
   const TK* rp[R];
 #pragma unroll
-  for (int r = 0; r < R; ++r) rp[r] = K + (size_t)(row0 + r) * ld;
+  for (int r = 0; r < R; ++r) rp[r] = K + (size_t)min(row0 + r, nrows - 1) * ld;
   auto chunk = [&](int c) {
     Slice<TK, R> sl;
 #pragma unroll
@@ -490,43 +430,6 @@ __device__ __forceinline__ void gemv_nf_lo_lds(const bf16_t* __restrict__ lo,
           reinterpret_cast<const v4u*>(lo + ((size_t)k * G + r) * kNfChunk + l8));
     lds_x_slice<bf16_t, G>(xs, (int)list[k], sl);
     sl.fma(acc);
-  }
-}
-
-// Near-field lo chunks of the patches [p0, p0 + PPB) (G rows each) shared
-// among the PPB waves of a workgroup: list entry k of patch p0 + j goes to
-// wave (k + j) mod PPB, so the first entries of the patches land on
-// different waves.  Adds into acc[j G + r].
-template <int G, int PPB>
-__device__ __forceinline__ void gemv_nf_lo_split(const bf16_t* __restrict__ lo,
-                                                 const int* __restrict__ off,
-                                                 const uint16_t* __restrict__ list, int p0, int n,
-                                                 int wave, const float* __restrict__ x,
-                                                 double* acc) {
-  const int l8 = (threadIdx.x & 63) * 8;
-#pragma unroll
-  for (int j = 0; j < PPB; ++j) {
-    if (p0 + j >= n) break;
-    const int o = off[p0 + j], nl = off[p0 + j + 1] - o;
-    for (int k = (wave - j + PPB) % PPB; k < nl; k += PPB) {
-      const int col = (int)list[o + k] * kNfChunk + l8;
-      const v4f x0 = *reinterpret_cast<const v4f*>(x + col);
-      const v4f x1 = *reinterpret_cast<const v4f*>(x + col + 4);
-#pragma unroll
-      for (int r = 0; r < G; ++r) {
-        const v4u kv = __builtin_nontemporal_load(
-            reinterpret_cast<const v4u*>(lo + ((size_t)(o + k) * G + r) * kNfChunk + l8));
-        float t = bf_lo(kv.x) * x0.x;
-        t = fmaf(bf_hi(kv.x), x0.y, t);
-        t = fmaf(bf_lo(kv.y), x0.z, t);
-        t = fmaf(bf_hi(kv.y), x0.w, t);
-        t = fmaf(bf_lo(kv.z), x1.x, t);
-        t = fmaf(bf_hi(kv.z), x1.y, t);
-        t = fmaf(bf_lo(kv.w), x1.z, t);
-        t = fmaf(bf_hi(kv.w), x1.w, t);
-        acc[j * G + r] += static_cast<double>(t);
-      }
-    }
   }
 }
 

@@ -95,6 +95,7 @@ struct SolveOpts {
 // the opaque handle of the C ABI
 struct rsfg_ctx {
   int device = 0;
+  int ncu = 0;                // compute units of the device (persistent kernels' grid)
   int n = 0;                  // patches
   bool borrowed = false;      // geometry and operator belong to another handle
   hipStream_t stream = nullptr;

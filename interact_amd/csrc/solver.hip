@@ -69,9 +69,6 @@ constexpr int kStagePPB = RSFG_STAGE_PPB;  // patches (= waves) per workgroup
 #ifndef RSFG_STAGE_MINW
 #define RSFG_STAGE_MINW 4
 #endif
-#ifndef RSFG_STAGE_SPLIT
-#define RSFG_STAGE_SPLIT 0
-#endif
 static_assert(kStagePPB >= 1 && kStagePPB <= 16, "one wave per patch, at most 1024 threads");
 
 // Epilogue operands of one patch whose addresses are known at launch: the
@@ -263,50 +260,7 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
   // the other.)
   StageOperands<NC> op;
   stage_operands<NC, false>(a, ks, sst, X, Xn, p, wave == 0 && t < kStagePPB && p < n, op);
-  // Split layout (RSFG_STAGE_SPLIT, for at most 16 rows per workgroup):
-  // the workgroup's kStagePPB G rows streamed by all its waves together
-  // (wave w takes every kStagePPB-th column chunk); part[w][j G + k] is
-  // wave w's share of row k of patch j.  Otherwise each wave streams the
-  // G rows of its own patch.
-  constexpr bool kSplit = RSFG_STAGE_SPLIT && kStagePPB * G <= 16;
-  constexpr int RW = kSplit ? kStagePPB * G : 1;
-  __shared__ double part[kStagePPB][RW];
-  if constexpr (kSplit) {
-    const int w = __builtin_amdgcn_readfirstlane(wave);
-    const int p0 = blockIdx.x * kStagePPB;
-    double acc[RW];
-#pragma unroll
-    for (int r = 0; r < RW; ++r) acc[r] = 0.0;
-    if constexpr (std::is_same<TK, bf16nf_t>::value) {
-      gemv_split_acc<bf16_t, RW, kStagePPB>(static_cast<const bf16_t*>(a.K), a.ld, p0 * G,
-                                            a.rows_all, a.cols, w,
-                                            static_cast<const float*>(a.vin), acc);
-      gemv_nf_lo_split<G, kStagePPB>(a.Knf_lo, a.nf_off, a.nf_list, p0, n, w,
-                                     static_cast<const float*>(a.vin), acc);
-    } else {
-      gemv_split_acc<TK, RW, kStagePPB>(static_cast<const TK*>(a.K), a.ld, p0 * G, a.rows_all,
-                                        a.cols, w, static_cast<const VT*>(a.vin), acc);
-    }
-#pragma unroll
-    for (int r = 0; r < RW; ++r) acc[r] = wave_sum(acc[r]);
-    if (lane == 0) {
-#pragma unroll
-      for (int r = 0; r < RW; ++r) part[w][r] = acc[r];
-    }
-    __syncthreads();
-    if (wave != 0) return;
-    // lane t < kStagePPB combines the waves' shares of its own patch in a
-    // fixed order
-    if (t < kStagePPB) {
-#pragma unroll
-      for (int k = 0; k < G; ++k) {
-        double v = 0.0;
-#pragma unroll
-        for (int q = 0; q < kStagePPB; ++q) v += part[q][t * G + k];
-        rs[t][k] = v;
-      }
-    }
-  } else {
+  {
     const int pw = min(blockIdx.x * kStagePPB + wave, n - 1);
     double rowsum[G];
     if constexpr (std::is_same<TK, bf16nf_t>::value) {
@@ -322,8 +276,8 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
 #pragma unroll
       for (int k = 0; k < G; ++k) rs[wave][k] = rowsum[k];
     }
-    __syncthreads();
   }
+  __syncthreads();
   if (wave != 0) return;
   double r9[kNRed];
   r9_neutral(r9);
@@ -350,10 +304,8 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
 // per-patch kernel this removes the x slices' L2 traffic and the
 // per-workgroup launch, barrier and epilogue gaps; every sum is formed in
 // the same order.
-constexpr int kPW = 8;          // waves per workgroup (two per SIMD)
 constexpr int kEW = 4;          // epilogue waves per workgroup (one per SIMD)
 constexpr int kMaxPPW = 32;     // patches per wave: kPW kMaxPPW / kEW <= 64 lanes
-constexpr int kXL = 36864;      // floats of x in LDS: 144 KiB of the CU's 160 KiB
 
 template <typename TK, int NC, int G>
 __global__ __launch_bounds__(64 * kPW, 1) void k_stage_p(StageArgs a, int ppw) {
@@ -390,7 +342,8 @@ __global__ __launch_bounds__(64 * kPW, 1) void k_stage_p(StageArgs a, int ppw) {
     double acc[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) acc[k] = 0.0;
-    gemv_lds_acc<TH, G, 2>(static_cast<const TH*>(a.K), a.ld, q * G, a.cols, xs, acc);
+    gemv_lds_acc<TH, G, 2>(static_cast<const TH*>(a.K), a.ld, q * G, a.rows_all, a.cols, xs,
+                           acc);
     if constexpr (std::is_same<TK, bf16nf_t>::value) {
       const int o = a.nf_off[q];
       gemv_nf_lo_lds<G>(a.Knf_lo + (size_t)o * G * kNfChunk, a.nf_list + o,
@@ -654,26 +607,15 @@ struct Launcher {
 // kernel runs (FP64 entries, x larger than the LDS budget, more than
 // kMaxPPW patches per wave, or RSFG_STAGE_PERSIST=0 in the environment).
 template <typename TK>
-static void persist_geometry(int n, int cols, bool allow, int* nwg, int* ppw) {
+static void persist_geometry(int ncu, int n, int cols, bool allow, int* nwg, int* ppw) {
   *nwg = 0;
   *ppw = 0;
   using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
-  if constexpr (!std::is_same<typename KTraits<TK>::VT, float>::value) {
-    return;
-  } else {
+  if constexpr (std::is_same<typename KTraits<TK>::VT, float>::value) {
     const char* e = getenv("RSFG_STAGE_PERSIST");
-    if (!allow || (e && e[0] == '0') || n <= 0) return;
+    if (!allow || (e && e[0] == '0')) return;
     if (lds_x_floats<TH>(cols) > kXL) return;
-    int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        ncu <= 0)
-      return;
-    const long waves = (long)ncu * kPW;
-    const int q = (int)((n + waves - 1) / waves);
-    if (q > kMaxPPW) return;
-    *ppw = q;
-    *nwg = (n + kPW * q - 1) / (kPW * q);
+    if (!persist_split(ncu, n, kMaxPPW, nwg, ppw)) *nwg = *ppw = 0;
   }
 }
 
@@ -681,8 +623,8 @@ template <typename TK, int NC, int G>
 struct LauncherT : Launcher {
   using VT = typename KTraits<TK>::VT;
   int nwg_p = 0, ppw_p = 0, n_ = 0;
-  LauncherT(int n, int cols, bool allow_persist) : n_(n) {
-    persist_geometry<TK>(n, cols, allow_persist, &nwg_p, &ppw_p);
+  LauncherT(int ncu, int n, int cols, bool allow_persist) : n_(n) {
+    persist_geometry<TK>(ncu, n, cols, allow_persist, &nwg_p, &ppw_p);
   }
   void init_vel(const StageArgs& a, hipStream_t st) override {
     hipLaunchKernelGGL((k_init_vel<VT, NC>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);
@@ -718,25 +660,25 @@ struct LauncherT : Launcher {
 };
 
 template <typename TK>
-static Launcher* make_launcher_t(int ncomp, int G, int n, int cols, bool allow) {
+static Launcher* make_launcher_t(int ncomp, int G, int ncu, int n, int cols, bool allow) {
   if (ncomp == 1) {
-    if (G == 1) return new LauncherT<TK, 1, 1>(n, cols, allow);
-    return new LauncherT<TK, 1, 2>(n, cols, allow);
+    if (G == 1) return new LauncherT<TK, 1, 1>(ncu, n, cols, allow);
+    return new LauncherT<TK, 1, 2>(ncu, n, cols, allow);
   }
-  if (G == 2) return new LauncherT<TK, 2, 2>(n, cols, allow);
-  return new LauncherT<TK, 2, 3>(n, cols, allow);
+  if (G == 2) return new LauncherT<TK, 2, 2>(ncu, n, cols, allow);
+  return new LauncherT<TK, 2, 3>(ncu, n, cols, allow);
 }
 
 // allow_persist: false for the additive (IMEX) method, whose stage kernel
 // is per-patch
 static Launcher* make_launcher(const rsfg_ctx* h, int prec, bool allow_persist) {
-  const int nc = h->ncomp, G = h->G, n = h->n, cols = h->cols;
+  const int nc = h->ncomp, G = h->G, n = h->n, cols = h->cols, cu = h->ncu;
   switch (prec) {
-    case RSFG_PREC_FP32: return make_launcher_t<float>(nc, G, n, cols, allow_persist);
-    case RSFG_PREC_BF16X2: return make_launcher_t<bf16x2_t>(nc, G, n, cols, allow_persist);
-    case RSFG_PREC_BF16: return make_launcher_t<bf16_t>(nc, G, n, cols, allow_persist);
-    case RSFG_PREC_BF16NF: return make_launcher_t<bf16nf_t>(nc, G, n, cols, allow_persist);
-    default: return make_launcher_t<double>(nc, G, n, cols, allow_persist);
+    case RSFG_PREC_FP32: return make_launcher_t<float>(nc, G, cu, n, cols, allow_persist);
+    case RSFG_PREC_BF16X2: return make_launcher_t<bf16x2_t>(nc, G, cu, n, cols, allow_persist);
+    case RSFG_PREC_BF16: return make_launcher_t<bf16_t>(nc, G, cu, n, cols, allow_persist);
+    case RSFG_PREC_BF16NF: return make_launcher_t<bf16nf_t>(nc, G, cu, n, cols, allow_persist);
+    default: return make_launcher_t<double>(nc, G, cu, n, cols, allow_persist);
   }
 }
 
