@@ -51,18 +51,18 @@ sys.path.insert(0, ROOT)
 
 SEC_PER_YEAR = 31557600.0
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused stage kernel on the 1 km (8000 DOF) and
-# 0.5 km (32000 DOF) workloads: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the
-# factors that a plain 256 MiB device copy reads on the same counters
+# HBM bytes per launch of the stage kernel on the 1 km (8000 DOF: per-patch
+# kernel) and 0.5 km (32000 DOF: persistent kernel, FP64 per-patch)
+# workloads: FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors that a plain
+# 256 MiB device copy reads on the same counters
 # (profiles/pmc_traffic_planar1km_8000dof.md,
 # profiles/kernels_planar0.5km_32000dof.md)
 STAGE_TRAFFIC = {"c2": {"fp64": 517.6e6, "fp32": 262.1e6, "bf16x2": 262.2e6, "bf16": 132.6e6,
                         "bf16nf": 145.9e6},
-                 "c3": {"fp64": 8.2051e9, "fp32": 4.1082e9, "bf16x2": 4.1082e9,
-                        "bf16": 2.0746e9, "bf16nf": 2.1486e9}}
-# HBM bytes per launch of the FP32 product k_matvec<float,2> (same counters;
-# profiles/kernels_planar0.5km_32000dof.md for the 0.5 km workload)
-MATVEC_TRAFFIC = {"c2": {"fp32": 259.8e6}, "c3": {"fp32": 4.0974e9}}
+                 "c3": {"fp64": 8.2031e9, "fp32": 4.1035e9, "bf16x2": 4.1035e9,
+                        "bf16": 2.0694e9, "bf16nf": 2.1443e9}}
+# HBM bytes per launch of the FP32 product k_matvec_p<float,2> (same counters)
+MATVEC_TRAFFIC = {"c2": {"fp32": 260.1e6}, "c3": {"fp32": 4.0965e9}}
 # HBM bytes per launch of the c4 batched product by unit shape (the shapes
 # the tuner picks on these boxes), from the same two counter runs of
 # `--config c4 --kernels-only` with the copy-kernel factors above
@@ -99,6 +99,32 @@ F_PAIR_TRI = 30559
 FP64_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 METRIC = "sim-years/hour + K·v HBM GB/s, mixed-prec vs FP64 GPU at matched event times"
+
+
+# the persistent kernels (x copied into LDS, one workgroup per CU) run for
+# the FP32-input formats while x fits in their 144 KiB LDS copy: 36864 FP32
+# values, counted in whole chunks of 512 (two-byte formats) or 256 columns;
+# the stage kernel only from 3 patches per wave up (more than 4096 patches
+# on the 256 CUs of an MI355X), below which the per-patch kernel is faster
+LDS_X_FLOATS = 36864
+PERSIST_MIN_PATCHES = 2 * 256 * 8 + 1
+
+
+def persistent(name, D):
+    chunk = 512 if name in ("bf16", "bf16nf") else 256
+    return name != "fp64" and -(-D // chunk) * chunk <= LDS_X_FLOATS
+
+
+def stage_kernel(name, D, n):
+    if persistent(name, D) and n >= PERSIST_MIN_PATCHES:
+        return f"k_stage_p<{STAGE_TYPES[name]},2,2> (persistent: x in LDS, fused K v + RHS epilogue)"
+    return f"k_stage<{STAGE_TYPES[name]},2,2> (fused K v + RHS epilogue)"
+
+
+def matvec_kernel(name, D):
+    if persistent(name, D):
+        return f"k_matvec_p<{STAGE_TYPES[name]},2> (persistent: x in LDS)"
+    return f"k_matvec<{STAGE_TYPES[name]},2>"
 
 
 def log(*a):
@@ -398,14 +424,14 @@ def main():
     head = runs[hname]
     stg = kv[hname]
     traffic = STAGE_TRAFFIC[args.config].get(hname)
-    roof = dict(kernel=f"k_stage<{STAGE_TYPES[hname]},2,2> (fused K v + RHS epilogue)", bound="hbm",
+    roof = dict(kernel=stage_kernel(hname, D, n), bound="hbm",
                 achieved=stg["stage_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=stg["stage_gbs"] / HBM_PEAK_GBS, traffic=traffic,
                 bytes_per_launch=stg["bytes"], launch_us=stg["stage_ms"] * 1e3)
     # the FP32 product K v on its own (the storage the north-star K v figure
     # names), HIP events on the handle's stream
     k32 = kv["fp32"]
-    roof_kv32 = dict(kernel="k_matvec<float,2> (K v, FP32 operator)", bound="hbm",
+    roof_kv32 = dict(kernel=matvec_kernel("fp32", D) + ", K v of the FP32 operator", bound="hbm",
                      achieved=k32["matvec_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                      frac=k32["matvec_gbs"] / HBM_PEAK_GBS,
                      traffic=MATVEC_TRAFFIC[args.config].get("fp32"),
