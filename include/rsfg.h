@@ -286,6 +286,8 @@ typedef struct {
   double wall_s;        /* host wall time of the solve call */
   int events_dropped;   /* event / monitor rows past the logs' capacity */
   int monitor_dropped;  /*   (max_events, max_monitor): counted, not stored */
+  int ncatalog;         /* catalog rows available */
+  int catalog_dropped;  /* catalog rows past the capacity (max_events) */
 } rsfg_stats;
 
 /* integrates from the current state to t_end (STEPOVER semantics) */
@@ -362,6 +364,18 @@ int rsfg_stage_bench(rsfg_handle *h, int prec, int nrep, double *ms);
    mean mu, peak |v| since onset, mean stress drop over ruptured cells
    [Pa], ruptured cell count.  *n in: capacity, out: rows copied. */
 int rsfg_events(rsfg_handle *h, double *rows, int *n);
+/* catalog rows (rsf_catalog.dat, rsf_finalize_event, src/rsf_output.c:
+   1078-1180), one per closed event that ruptured at least one cell (|v| >=
+   rupture_vth at an accepted step between its onset and arrest): 11 doubles
+   each: onset t [s], arrest t [s], ruptured cells, their area (4 l w per
+   cell; a triangle counts 4 times its area, as in the reference, whose
+   triangles carry l = w = sqrt(area)), mean and max |slip change| [m],
+   mean and max stress drop |tau_onset| - |tau_arrest| [Pa], peak |v|,
+   M0 = G sum area |slip change| [N m] with G = scale 1e4, and
+   Mw = 2/3 (log10 M0 - 9.05) (-99 for M0 <= 0).  In the two-component
+   layout tau and slip are the strike components, as in the event rows.
+   *n in: capacity, out: rows copied. */
+int rsfg_catalog(rsfg_handle *h, double *rows, int *n);
 /* monitor rows (rsf_monitor.dat, src/rsf_output.c:832-842): 10 doubles
    each: step, t [s], t [yr], dt, log10 max|v|, mean slip, mean mu,
    max sigma, min sigma, log10 mean|v| */

@@ -130,10 +130,11 @@ static void free_state(rsfg_ctx* h) {
   hipFree(h->d_ev);
   hipFree(h->d_mon);
   hipFree(h->d_cat);
+  hipFree(h->d_catr);
   hipFree(h->d_rupt);
   h->d_state = nullptr;
   h->partials = nullptr;
-  h->d_ev = h->d_mon = h->d_cat = nullptr;
+  h->d_ev = h->d_mon = h->d_cat = h->d_catr = nullptr;
   h->d_rupt = nullptr;
   h->ev_cap = h->mon_cap = 0;
 }
@@ -149,6 +150,15 @@ int ensure_logs(rsfg_ctx* h, int ev_rows, int mon_rows) {
     RSFG_HIP(hipStreamSynchronize(h->stream));
     hipFree(h->d_ev);
     h->d_ev = p;
+    // catalog rows: at most one per event row, grown with it
+    double* q = nullptr;
+    RSFG_HIP(hipMalloc(&q, (size_t)ev_rows * kCatCols * sizeof(double)));
+    if (h->d_catr && h->ev_cap > 0)
+      RSFG_HIP(hipMemcpyAsync(q, h->d_catr, (size_t)h->ev_cap * kCatCols * sizeof(double),
+                              hipMemcpyDeviceToDevice, h->stream));
+    RSFG_HIP(hipStreamSynchronize(h->stream));
+    hipFree(h->d_catr);
+    h->d_catr = q;
     h->ev_cap = ev_rows;
   }
   if (mon_rows > h->mon_cap) {
@@ -735,7 +745,27 @@ int rsfg_create(int device, const rsfg_patch* pt, int n, rsfg_handle** out) {
     g[kGeoR0 * n + i] = q.rake[0];
     g[kGeoR1 * n + i] = q.rake[1];
   }
+  // catalog area per patch, 4 l w as the reference sums it
+  // (src/rsf_output.c:1117); a triangle carries l = w = sqrt(its area)
+  // (src/green/eval_triangle_nw.c:120-121), so it counts 4 times its area
+  std::vector<double> area(n);
+  for (int i = 0; i < n; ++i) {
+    const rsfg_patch& q = pt[i];
+    if (q.l < 0 && q.w < 0) {
+      const double* v = q.vert;
+      const double e1[3] = {v[3] - v[0], v[4] - v[1], v[5] - v[2]};
+      const double e2[3] = {v[6] - v[0], v[7] - v[1], v[8] - v[2]};
+      const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                   cz = e1[0] * e2[1] - e1[1] * e2[0];
+      area[i] = 4.0 * 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+    } else {
+      area[i] = 4.0 * q.l * q.w;
+    }
+  }
   hipError_t e = hipStreamCreate(&h->stream);
+  if (e == hipSuccess) e = hipMalloc(&h->d_area, sizeof(double) * n);
+  if (e == hipSuccess)
+    e = hipMemcpy(h->d_area, area.data(), sizeof(double) * n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&h->d_geo, sizeof(double) * kGeoFields * n);
   if (e == hipSuccess)
     e = hipMemcpy(h->d_geo, g, sizeof(double) * kGeoFields * n, hipMemcpyHostToDevice);
@@ -764,6 +794,7 @@ int rsfg_create_member(rsfg_handle* geom, rsfg_handle** out) {
   h->n = geom->n;
   h->borrowed = true;
   h->d_geo = geom->d_geo;
+  h->d_area = geom->d_area;
   h->assembled = true;
   h->has_tri = geom->has_tri;
   h->slip_mode = geom->slip_mode;
@@ -815,10 +846,12 @@ int rsfg_destroy(rsfg_handle* h) {
     h->nf_off = nullptr;
     h->nf_list = nullptr;
     h->d_geo = nullptr;
+    h->d_area = nullptr;
   }
   free_operator(h);
   free_state(h);
   hipFree(h->d_geo);
+  hipFree(h->d_area);
   hipFree(h->d_coef);
   hipFree(h->d_sinc);
   hipFree(h->d_ctl);
@@ -1495,6 +1528,15 @@ int rsfg_events(rsfg_handle* h, double* rows, int* n) {
   const int k = std::min(*n, h->nevent);
   if (k > 0)
     RSFG_HIP(hipMemcpy(rows, h->d_ev, sizeof(double) * 8 * k, hipMemcpyDeviceToHost));
+  *n = k;
+  return RSFG_OK;
+}
+
+int rsfg_catalog(rsfg_handle* h, double* rows, int* n) {
+  RSFG_CHECK(h && rows && n, RSFG_EINVAL, "rsfg_catalog: null argument");
+  const int k = std::min(*n, h->ncat);
+  if (k > 0)
+    RSFG_HIP(hipMemcpy(rows, h->d_catr, sizeof(double) * kCatCols * k, hipMemcpyDeviceToHost));
   *n = k;
   return RSFG_OK;
 }

@@ -1160,6 +1160,7 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
     h->t = c->t;
     h->nevent = c->nevent;
     h->nmon = c->nmon;
+    h->ncat = c->ncat;
     if (st) {
       rsfg_stats& q = st[m];
       memset(&q, 0, sizeof q);
@@ -1175,6 +1176,8 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
       q.nmonitor = std::min(c->nmon, c->max_mon);
       q.events_dropped = c->ev_drop;
       q.monitor_dropped = c->mon_drop;
+      q.ncatalog = c->ncat;
+      q.catalog_dropped = c->cat_drop;
       q.wall_s = wall;
     }
   }

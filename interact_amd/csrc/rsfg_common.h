@@ -47,6 +47,8 @@ constexpr int kMaxMon = 1 << 18;
 constexpr int kMaxStages = 7;
 // reductions per workgroup in the final stage
 constexpr int kNRed = 9;
+constexpr int kCatCols = 11;   // doubles per catalog row
+
 enum { RED_ERR = 0, RED_VMAX, RED_VSUM, RED_SLIP, RED_MU, RED_SMAX, RED_SMIN, RED_XN2, RED_DX2 };
 
 // Controller state, resident in device memory; the single-workgroup
@@ -72,6 +74,9 @@ struct DevCtl {
   int nevent, nmon;     // rows stored (at most max_event, max_mon)
   int max_event, max_mon;
   int ev_drop, mon_drop;  // rows past the capacity: counted, not stored
+  double t_onset;       // time of the open event's onset
+  int ev_open;          // an onset was logged and its arrest not yet
+  int ncat, cat_drop;   // catalog rows stored (at most max_event) / past it
 };
 
 // Butcher tableau and integrator settings (kernel argument, by value)
@@ -87,6 +92,7 @@ struct SolveOpts {
   double rtol, atol[6];
   double dt_monitor, rdx, adx;
   double vel_event, hyst, rupture_vth;
+  double shear_mu;      // shear modulus [Pa] of the catalog's seismic moment
   int max_domain_reject;
 };
 
@@ -146,11 +152,13 @@ struct rsfg_ctx {
   double* d_ev = nullptr;
   double* d_mon = nullptr;
   double* d_cat = nullptr;    // tau0, slip0 per patch
+  double* d_catr = nullptr;   // catalog rows (kCatCols each, ev_cap rows)
+  double* d_area = nullptr;   // per-patch catalog area 4 l w (members borrow it)
   int* d_rupt = nullptr;
   rsfg::DevCtl* d_ctl = nullptr;
   rsfg::DevCtl* h_ctl = nullptr;  // pinned
-  int nevent = 0, nmon = 0;
-  int ev_cap = 0, mon_cap = 0;    // rows allocated in d_ev, d_mon
+  int nevent = 0, nmon = 0, ncat = 0;
+  int ev_cap = 0, mon_cap = 0;    // rows allocated in d_ev (and d_catr), d_mon
   // the last solve can be continued (rsfg_solve_continue) with this method
   // and operator precision; cleared by every call that touches the buffers
   bool cont_ok = false;

@@ -724,6 +724,8 @@ void fill_stage_args(rsfg_ctx* h, const rsfg_solver_opts* o, const void* K, Stag
   a.tau0 = h->d_cat;
   a.slip0 = h->d_cat + n;
   a.rupt = h->d_rupt;
+  a.catr = h->d_catr;
+  a.area = h->d_area;
   a.ctl = h->d_ctl;
   a.hlog = h->d_hlog;
   a.hlog_cap = h->hlog_cap;
@@ -736,6 +738,7 @@ void fill_stage_args(rsfg_ctx* h, const rsfg_solver_opts* o, const void* K, Stag
   a.so.vel_event = o->vel_event;
   a.so.hyst = o->vel_event_hyst;
   a.so.rupture_vth = o->rupture_vth;
+  a.so.shear_mu = h->scale * 1e4;   // scale = G_SI / 1e4 (src/rsf_solve.c:235-237)
   a.so.max_domain_reject = o->max_domain_reject;
 }
 
@@ -953,6 +956,7 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
   h->t = c->t;
   h->nevent = c->nevent;
   h->nmon = c->nmon;
+  h->ncat = c->ncat;
   if (st) {
     memset(st, 0, sizeof(*st));
     st->steps = c->steps;
@@ -967,6 +971,8 @@ int solve_device(rsfg_ctx* h, const rsfg_solver_opts* o, double t_end, rsfg_stat
     st->nmonitor = c->nmon;
     st->events_dropped = c->ev_drop;
     st->monitor_dropped = c->mon_drop;
+    st->ncatalog = c->ncat;
+    st->catalog_dropped = c->cat_drop;
     st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
   }
   if (c->status == RSFG_ESTORM) {

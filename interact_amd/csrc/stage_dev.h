@@ -39,6 +39,8 @@ struct StageArgs {
   double* tau0;
   double* slip0;
   int* rupt;
+  double* catr;        // catalog rows (kCatCols each, capacity max_event)
+  const double* area;  // per-patch catalog area 4 l w
   DevCtl* ctl;
   double* hlog;        // accepted step sizes, in order (optional; the control
   long long hlog_cap;  // block the host copies holds no device address)
@@ -230,6 +232,7 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
       c->rupt_update = 0;
       c->stage_fail = 0;
       c->attempt_failed = 0;
+      c->ev_open = 0;
       if (!(c->t < c->t_end)) c->halted = 1;
     }
     return;
@@ -351,9 +354,13 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
       if (s_int[2]) break;
     }
     const double th = s_par[3];
-    // state at the crossing: statistics, onset snapshots, arrest drop
+    // state at the crossing: statistics, onset snapshots, arrest drop and
+    // the catalog sums over the ruptured cells (rsf_finalize_event,
+    // src/rsf_output.c:1090-1135: |slip change|, change of |tau|, area
+    // 4 l w, moment density area |slip change|)
     const int onset = !c->slipping;  // toggles to slipping
     double vm = 0.0, ss = 0.0, sm = 0.0, drop = 0.0, cnt = 0.0, y[D];
+    double sds = 0.0, mds = 0.0, mdd = 0.0, sar = 0.0, smo = 0.0;
     for (int p = tid; p < n; p += blockDim.x) {
       dense_patch<NC>(a, X0, X1, Kp, hs, th, p, y);
       vm = fmax(vm, vel_mag<NC>(a.fp, a.coef[p], y));
@@ -364,8 +371,16 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
         a.slip0[p] = y[NC + 2];
         a.rupt[p] = 0;
       } else if (a.rupt[p]) {
-        drop += fabs(a.tau0[p]) - fabs(y[1]);
+        const double dd = fabs(a.tau0[p]) - fabs(y[1]);
+        const double ds = fabs(y[NC + 2] - a.slip0[p]);
+        const double ar = a.area ? a.area[p] : 0.0;
+        drop += dd;
         cnt += 1.0;
+        sds += ds;
+        mds = fmax(mds, ds);
+        mdd = fmax(mdd, dd);
+        sar += ar;
+        smo += ar * ds;
       }
     }
     vm = block_reduce(vm, 1, sh);
@@ -373,23 +388,60 @@ __device__ void controller(const StageArgs& a, int s, const double* red, DevCtl*
     sm = block_reduce(sm, 0, sh);
     drop = block_reduce(drop, 0, sh);
     cnt = block_reduce(cnt, 0, sh);
+    if (!onset) {
+      sds = block_reduce(sds, 0, sh);
+      mds = block_reduce(mds, 1, sh);
+      mdd = block_reduce(mdd, 1, sh);
+      sar = block_reduce(sar, 0, sh);
+      smo = block_reduce(smo, 0, sh);
+    }
     if (tid == 0) {
+      const double tc = c->t + th * hs;
+      const double peak = onset ? vm : fmax(c->peakv, vm);
       c->slipping = onset;
       if (onset) c->peakv = vm;
       if (c->nevent < c->max_event) {
         double* row = a.ev + 8 * c->nevent;
-        row[0] = c->t + th * hs;
+        row[0] = tc;
         row[1] = onset ? 1.0 : -1.0;
         row[2] = log10(vm > 1e-300 ? vm : 1e-300);
         row[3] = ss / n;
         row[4] = sm / n;
-        row[5] = onset ? vm : fmax(c->peakv, vm);
+        row[5] = peak;
         row[6] = cnt > 0.0 ? drop / cnt : 0.0;
         row[7] = cnt;
         c->nevent++;
       } else {
         c->ev_drop++;
       }
+      // catalog row of a closed event that ruptured at least one cell
+      // (src/rsf_output.c:1150-1178): onset and arrest [s], ruptured
+      // cells, their area, mean and max |slip change|, mean and max
+      // stress drop [Pa], peak |v|, M0 = mu sum area |slip change|,
+      // Mw = 2/3 (log10 M0 - 9.05) (-99 for M0 <= 0,
+      // src/geometry/llgeo.c:757-760)
+      if (!onset && c->ev_open && cnt > 0.0) {
+        if (a.catr && c->ncat < c->max_event) {
+          double* row = a.catr + kCatCols * c->ncat;
+          const double m0 = o.shear_mu * smo;
+          row[0] = c->t_onset;
+          row[1] = tc;
+          row[2] = cnt;
+          row[3] = sar;
+          row[4] = sds / cnt;
+          row[5] = mds;
+          row[6] = drop / cnt;
+          row[7] = mdd;
+          row[8] = peak;
+          row[9] = m0;
+          row[10] = (m0 > 0.0) ? (2.0 / 3.0) * (log10(m0) - 9.05) : -99.0;
+          c->ncat++;
+        } else {
+          c->cat_drop++;
+        }
+      }
+      if (onset) c->t_onset = tc;
+      c->ev_open = onset;
       c->vprev = vm;
     }
     __syncthreads();

@@ -36,7 +36,7 @@ EXPORTS = [
     "rsfg_set_friction", "rsfg_set_loading", "rsfg_rhs",
     "rsfg_solver_defaults", "rsfg_set_state", "rsfg_get_state", "rsfg_solve",
     "rsfg_solve_continue",
-    "rsfg_events", "rsfg_monitor", "rsfg_stage_bench",
+    "rsfg_events", "rsfg_monitor", "rsfg_catalog", "rsfg_stage_bench",
     "rsfg_read_table", "rsfg_write_checkpoint", "rsfg_read_checkpoint",
     "rsfg_create_member", "rsfg_ensemble_create", "rsfg_ensemble_member",
     "rsfg_ensemble_solve", "rsfg_ensemble_tune", "rsfg_ensemble_config", "rsfg_ensemble_select",
@@ -91,7 +91,8 @@ class Stats(C.Structure):
                 ("launches", C.c_long), ("t", C.c_double), ("dt", C.c_double),
                 ("status", C.c_int), ("nevent", C.c_int),
                 ("nmonitor", C.c_int), ("wall_s", C.c_double),
-                ("events_dropped", C.c_int), ("monitor_dropped", C.c_int)]
+                ("events_dropped", C.c_int), ("monitor_dropped", C.c_int),
+                ("ncatalog", C.c_int), ("catalog_dropped", C.c_int)]
 
 
 _LIB = None
@@ -175,6 +176,7 @@ def load(path=None):
                                       C.POINTER(Stats)]
     L.rsfg_events.argtypes = [h, _dp, C.POINTER(C.c_int)]
     L.rsfg_monitor.argtypes = [h, _dp, C.POINTER(C.c_int)]
+    L.rsfg_catalog.argtypes = [h, _dp, C.POINTER(C.c_int)]
     L.rsfg_stage_bench.argtypes = [h, C.c_int, C.c_int, C.POINTER(C.c_double)]
     for name in EXPORTS:
         if name not in ("rsfg_last_error", "rsfg_version"):

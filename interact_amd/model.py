@@ -323,6 +323,15 @@ class FaultSystem:
         L.check(self._lib.rsfg_events(self.h, rows, C.byref(n)), "events")
         return rows[:8 * n.value].reshape(-1, 8).copy()
 
+    def catalog(self, cap=4096):
+        """rows of rsfg_catalog (11 columns: onset t [s], arrest t [s],
+        ruptured cells, area, mean and max |slip change|, mean and max
+        stress drop [Pa], peak |v|, M0, Mw)"""
+        rows = np.zeros(11 * cap)
+        n = C.c_int(cap)
+        L.check(self._lib.rsfg_catalog(self.h, rows, C.byref(n)), "catalog")
+        return rows[:11 * n.value].reshape(-1, 11).copy()
+
     def monitor(self, cap=1 << 18):
         rows = np.zeros(10 * cap)
         n = C.c_int(cap)

@@ -75,7 +75,9 @@ class Result(C.Structure):
                 ("t_final", C.c_double), ("dt_final", C.c_double),
                 ("status", C.c_int), ("nevent", C.c_int),
                 ("ev", C.c_void_p), ("nmon", C.c_int), ("mon", C.c_void_p),
-                ("hlog", C.c_void_p), ("hlog_cap", C.c_long)]
+                ("hlog", C.c_void_p), ("hlog_cap", C.c_long),
+                ("ncat", C.c_int), ("cat", C.c_void_p), ("cat_area", C.c_void_p),
+                ("cat_mu", C.c_double)]
 
 
 def _setup(L):
@@ -330,7 +332,7 @@ class RSFProblem:
               rdx_monitor=1e-4, adx_monitor=0.0, vel_event=1e-3,
               vel_event_hyst=0.5, rupture_vth=None, max_steps=10_000_000,
               max_domain_reject=5000, max_events=4096, max_monitor=200000,
-              log_steps=0):
+              log_steps=0, cat_area=None, cat_mu=0.0):
         X = np.array(X0, np.float64, copy=True)
         at = (C.c_double * 6)(*(list(atol) if atol is not None else
                                 [1e-12, 1e-3, 1e-3, 1e-3, 1e-3, 1e-3][:self.dim]
@@ -344,6 +346,13 @@ class RSFProblem:
         r = Result()
         r.ev = ev.ctypes.data
         r.mon = mon.ctypes.data
+        cat = np.zeros(max_events * 11)
+        r.cat = cat.ctypes.data
+        area = None
+        if cat_area is not None:
+            area = np.ascontiguousarray(cat_area, np.float64)
+            r.cat_area = area.ctypes.data
+        r.cat_mu = cat_mu
         hlog = np.zeros(max(int(log_steps), 1))
         if log_steps:
             r.hlog = hlog.ctypes.data
@@ -353,7 +362,8 @@ class RSFProblem:
                    domain_rejects=r.domain_rejects, rhs_evals=r.rhs_evals,
                    t_final=r.t_final, dt_final=r.dt_final, status=r.status,
                    events=ev[:8 * r.nevent].reshape(-1, 8).copy(),
-                   monitor=mon[:10 * r.nmon].reshape(-1, 10).copy())
+                   monitor=mon[:10 * r.nmon].reshape(-1, 10).copy(),
+                   catalog=cat[:11 * r.ncat].reshape(-1, 11).copy())
         if log_steps:
             out["hlog"] = hlog[:min(r.steps, int(log_steps))].copy()
         return out

@@ -155,6 +155,15 @@ typedef struct {
   double *mon;                /* max_monitor x 10 monitor columns */
   double *hlog;               /* accepted step sizes in order, or NULL */
   long hlog_cap;
+  /* catalog (rsf_finalize_event, src/rsf_output.c:1078-1180), or cat NULL:
+     one row of 11 per closed event with ruptured cells, at most max_events:
+     onset t, arrest t, ruptured cells, their area, mean and max |slip
+     change|, mean and max stress drop, peak |v|, M0 = cat_mu sum area
+     |slip change|, Mw; cat_area: per-patch area 4 l w (NULL: zero) */
+  int ncat;
+  double *cat;
+  const double *cat_area;
+  double cat_mu;
 } rsfo_result;
 
 /* integrates X in place; X is AoS as for rsfo_rhs */

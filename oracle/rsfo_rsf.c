@@ -499,8 +499,8 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
   double *K[7], *Y, *Xn, *Xmon, *Xe, *tau0, *slip0, *work;
   unsigned char *rupt;
   double t = o->t0, h = o->dt_init, hmax, t_old_mon, thr;
-  int slipping, storm = 0, nrup;
-  double fprev, peakv = 0.0;
+  int slipping, storm = 0, nrup, ev_open = 0;
+  double fprev, peakv = 0.0, t_onset = 0.0;
   field_stats st;
   tab_init(&tb, o->method);
   if (tb.imex && nc != 1) {       /* the reference's IMEX is scalar-slip only */
@@ -524,6 +524,7 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
   r->status = 0;
   r->nevent = 0;
   r->nmon = 0;
+  r->ncat = 0;
   hmax = (o->dt_max < o->dt_monitor) ? o->dt_max : o->dt_monitor;
   if (tb.imex) rsfo_imex_parts(p, X, K[0], K[3]);
   else rsfo_rhs(p, X, K[0], NULL, NULL);
@@ -730,6 +731,44 @@ int rsfo_solve(const rsfo_problem *p, const rsfo_options *o, double *X,
               row[7] = nrup;
               r->nevent++;
             }
+            /* arrest of an open event: its catalog row when a cell
+               ruptured (rsf_finalize_event, src/rsf_output.c:1090-1178):
+               |slip change| and change of |tau| since the onset snapshot,
+               area 4 l w, M0 = mu sum area |slip change|, Mw from
+               mwfromm0 (src/geometry/llgeo.c:757-760) */
+            if (!slipping) {
+              double sds = 0.0, mds = 0.0, sdd = 0.0, mdd = 0.0, sar = 0.0, smo = 0.0;
+              int cnt = 0;
+              for (i = 0; i < n; i++)
+                if (rupt[i]) {
+                  const double dd = fabs(tau0[i]) - fabs(Xe[(size_t)i * dim + 1]);
+                  const double ds = fabs(Xe[(size_t)i * dim + nc + 2] - slip0[i]);
+                  const double ar = r->cat_area ? r->cat_area[i] : 0.0;
+                  sds += ds; sdd += dd; sar += ar; smo += ar * ds;
+                  if (ds > mds) mds = ds;
+                  if (dd > mdd) mdd = dd;
+                  cnt++;
+                }
+              if (ev_open && cnt > 0 && r->cat && r->ncat < o->max_events) {
+                double *row = r->cat + 11 * r->ncat;
+                const double m0 = r->cat_mu * smo;
+                row[0] = t_onset;
+                row[1] = told + b_th * hstep;
+                row[2] = cnt;
+                row[3] = sar;
+                row[4] = sds / cnt;
+                row[5] = mds;
+                row[6] = sdd / cnt;
+                row[7] = mdd;
+                row[8] = fmax(peakv, st.vmax);
+                row[9] = m0;
+                row[10] = (m0 > 0.0) ? (2.0 / 3.0) * (log10(m0) - 9.05) : -99.0;
+                r->ncat++;
+              }
+            } else {
+              t_onset = told + b_th * hstep;
+            }
+            ev_open = slipping;
             thr = o->vel_event * (slipping ? o->vel_event_hyst : 1.0);
             fprev = evfun(st.vmax, thr);
             th_lo = b_th;

@@ -26,7 +26,8 @@ Outputs
                         dips 0 / 90 / 89.9999995 (cos snap) / 45
   bp5_2km/              the reference's BP5 2 km input files (data)
   ref_results/          the reference's published result files used as
-                        fixtures (events, slider physics, |K 1| answers)
+                        fixtures (events, slider physics, |K 1| answers, the
+                        head of a dense monitor file)
 """
 import argparse
 import gzip
@@ -124,6 +125,20 @@ def triangles(tmp):
             Tf, irf = kmat(gfile, 1)
             out[name + "_Tfull"], out[name + "_iretfull"] = Tf, irf
     np.savez_compressed(os.path.join(HERE, "ktensor_tri.npz"), **out)
+
+
+def monitor_excerpt(ref, rr, nrows=40):
+    """the header and first nrows rows of a dense rsf_monitor.dat written by
+    the reference (test_hmatrix/out_2f_2.0km_sep4.0km_260yr_np4): fixture
+    of the monitor-file writer's column layout"""
+    src = os.path.join(ref, "test_hmatrix", "out_2f_2.0km_sep4.0km_260yr_np4", "dense",
+                       "rsf_monitor.dat")
+    with open(src) as f:
+        lines = f.readlines()
+    head = [l for l in lines if l.startswith("#")][:1]
+    data = [l for l in lines if not l.startswith("#")][:nrows]
+    with open(os.path.join(rr, "monitor_2f_2km_dense_head.dat"), "w") as f:
+        f.writelines(head + data)
 
 
 def main():
@@ -226,6 +241,7 @@ def main():
                     os.path.join(rr, "geom_slider10.in"))
     shutil.copyfile(os.path.join(ref, "scaling_tests", "full_vs_half.csv"),
                     os.path.join(rr, "full_vs_half.csv"))
+    monitor_excerpt(ref, rr)
     # 6. triangle meshes
     triangles(tmp)
     shutil.rmtree(tmp)

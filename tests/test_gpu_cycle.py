@@ -40,8 +40,9 @@ def _planar(ds, store):
 def test_c1_cycle_trajectories_match_the_oracle():
     """300 yr of c1 on the device (FP64, FP32, bf16nf) against the oracle's
     FP64 cycle: the same events, every onset within 0.05 yr (FP64: 1e-4 yr),
-    peak slip rates within 1% (FP64: 1e-4), and the log10 max|v| trace of
-    the FP64 device run within 0.01 RMS of the oracle's."""
+    peak slip rates within 1% (FP64: 1e-4), the log10 max|v| trace of
+    the FP64 device run within 0.01 RMS of the oracle's, and the FP64
+    run's catalog rows (ruptured cells and area exactly, the rest 1e-4)."""
     from interact_amd import accuracy, lib
     store = lib.PREC_FP32 | lib.PREC_BF16NF
     g, f, fs, X0 = _planar(2.0, store)
@@ -55,7 +56,8 @@ def test_c1_cycle_trajectories_match_the_oracle():
     P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ xpl, ncomp=2, eta=f["eta"],
                           v0=f["v0"], f0=f["f0"], nthreads=16)
     t_end = 300 * SEC_PER_YEAR
-    ro = P.solve(X0, t_end, rtol=1e-8)
+    ro = P.solve(X0, t_end, rtol=1e-8, cat_area=4.0 * g["l"] * g["w"],
+                 cat_mu=f["shear_modulus"])
     assert ro["status"] == 0
     ref_ev, ref_mon = ro["events"], ro["monitor"]
     assert len(accuracy.onsets(ref_ev)) == 2
@@ -67,6 +69,14 @@ def test_c1_cycle_trajectories_match_the_oracle():
         assert st["status"] == 0 and st["events_dropped"] == 0
         s = accuracy.score(ref_mon, ref_ev, fs.monitor(), fs.events())
         scores[name] = s
+        if name == "fp64":
+            # catalog rows (rsf_finalize_event): the same ruptured cells
+            cd, co = fs.catalog(), ro["catalog"]
+            assert len(cd) == len(co) == 2
+            assert np.array_equal(cd[:, 2:4], co[:, 2:4])
+            assert np.abs(cd[:, :2] - co[:, :2]).max() / SEC_PER_YEAR < 1e-4
+            for k in (4, 5, 6, 7, 8, 9):
+                assert np.abs(cd[:, k] / co[:, k] - 1.0).max() < 1e-4, k
         assert s["nev_run"] == s["nev_ref"], (name, s)
         tol_on, tol_pk = (1e-4, 1e-4) if name == "fp64" else (0.05, 0.01)
         assert max(abs(d) for d in s["donset"]) < tol_on, (name, s)

@@ -159,6 +159,36 @@ def test_slider_cycles_under_each_law_match_oracle(law):
     fs.close()
 
 
+def test_catalog_rows_match_the_oracle(tmp_path):
+    """rsfg_catalog on the slider (aging law, 450 yr): the device's
+    catalog rows (rsf_finalize_event, src/rsf_output.c:1078-1180) against
+    the oracle's, and the three text files written from the device logs
+    read back with the rows they came from."""
+    from interact_amd import output
+    fs, P, X0 = _slider(1)
+    area = [1e8]   # 4 l w of the 10 km square slider
+    kw = dict(rtol=1e-9, rdx_monitor=1e-5, rupture_vth=0.1)
+    fs.set_state(X0)
+    st = fs.solve(450 * SEC_PER_YEAR, steps_per_launch=256, **kw)
+    ro = P.solve(X0, 450 * SEC_PER_YEAR, cat_area=area, cat_mu=G_BP5, **kw)
+    cd, co = fs.catalog(), ro["catalog"]
+    assert st["ncatalog"] == len(cd) == len(co) == 7 and st["catalog_dropped"] == 0
+    assert np.abs(cd[:, :2] - co[:, :2]).max() / SEC_PER_YEAR < 1e-3
+    assert np.array_equal(cd[:, 2:4], co[:, 2:4])
+    for k in (4, 5, 6, 7, 8, 9):
+        assert np.abs(cd[:, k] / co[:, k] - 1.0).max() < 1e-3, k
+    assert np.abs(cd[:, 10] - co[:, 10]).max() < 1e-3
+    output.write_outputs(fs, str(tmp_path), vel_event=1e-3, rupture_vth=0.1)
+    mon = np.loadtxt(str(tmp_path / output.MONITOR_FILE))
+    evf = np.loadtxt(str(tmp_path / output.EVENTS_FILE))
+    catf = np.loadtxt(str(tmp_path / output.CATALOG_FILE))
+    m, e = fs.monitor(), fs.events()
+    assert mon.shape == (len(m), 10) and np.allclose(mon[:, 1], m[:, 1], rtol=1e-8)
+    assert evf.shape == (len(e), 6) and np.array_equal(evf[:, 2], e[:, 1])
+    assert catf.shape == (7, 13) and np.allclose(catf[:, 11], cd[:, 9], rtol=1e-6)
+    fs.close()
+
+
 # ------------------------------------------------------------------ normal stress
 def _dipping_grid(ds_km, nx=40, nz=12, dip=30.0, top_km=1.0):
     """planar fault of nx x nz square patches, strike 0 (+y), dipping dip
