@@ -265,6 +265,70 @@ def test_sigma_evolution_matches_oracle(limit):
     fs.close()
 
 
+def test_per_patch_plate_rate_and_additive_rates():
+    """-rsf_vpl_file and -rsf_stress_rate_file (src/rsf_solve.c:268-363):
+    a per-patch plate rate in the backslip products and per-patch additive
+    shear and normal rates, on the dipping fault with the normal-stress
+    evolution on.  The device RHS against the oracle's with the same
+    loading (FP64 exact to 1e-10; the FP32 operator, which folds the plate
+    rate into K (v - vpl), to 1e-5), then rk5dp through the seeded
+    nucleation with the oracle's steps."""
+    from interact_amd import faults
+    from interact_amd.model import FaultSystem
+    L = _L()
+    g = _dipping_grid(2.0)
+    f = faults.bp5_fields(g)
+    n = len(g["l"])
+    k = np.arange(n)
+    vpl = 1e-9 * (1.0 + 0.5 * np.sin(0.37 * k))
+    tdot = 2e-3 * np.cos(0.11 * k)
+    sdot = 1e-3 * np.sin(0.23 * k)
+    sc = G_BP5 / 1e4
+    T = oracle.traction_tensor(g, 0, 16)
+    K, Kn = sc * T[:, :, 0, 0], -sc * T[:, :, 0, 2]
+    fs = FaultSystem(g)
+    fs.assemble(L.SLIP_STRIKE, shear_modulus=G_BP5, with_normal=True,
+                store=L.PREC_FP32)
+    kw = dict(calc_sigma_dot=True)
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=1e-6, f0=0.6, eta=f["eta"], **kw)
+    fs.set_loading(1e-9, vpl_patch=vpl, taudot_add=tdot, sigdot_add=sdot)
+    P = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ (-vpl) + tdot, Kn=Kn,
+                          sinc_n=Kn @ (-vpl) + sdot, eta=f["eta"], nthreads=16, **kw)
+    X0 = initial_state(f["tau"], 25e6, f["vel"], f["a"], 1e-6)
+    rng = np.random.default_rng(12)
+    X = X0.reshape(-1, 4).copy()
+    X[:, 0] += rng.normal(0, 0.01, len(X))
+    X = X.ravel()
+    Fo = P.rhs(X).reshape(-1, 4)
+    for prec, tol in ((L.PREC_FP64, 1e-10), (L.PREC_FP32, 1e-5)):
+        Fg, ok = fs.rhs(X, prec)
+        assert ok
+        Fg = Fg.reshape(-1, 4)
+        for c in range(4):
+            assert np.abs(Fg[:, c] - Fo[:, c]).max() <= tol * np.abs(Fo[:, c]).max(), (prec, c)
+    # the per-patch terms move the rates: without them the RHS differs
+    P0 = oracle.RSFProblem(K, f["a"], f["b"], f["dc"], K @ np.full(n, -1e-9), Kn=Kn,
+                           sinc_n=Kn @ np.full(n, -1e-9), eta=f["eta"], nthreads=16, **kw)
+    F0 = P0.rhs(X).reshape(-1, 4)
+    assert np.abs(F0[:, 2] - Fo[:, 2]).max() > 1e-4
+    t_end = 0.02 * SEC_PER_YEAR
+    fs.set_state(X0)
+    st = fs.solve(t_end, rtol=1e-6)
+    ro = P.solve(X0, t_end, rtol=1e-6)
+    assert (st["steps"], st["rejects"]) == (ro["steps"], ro["rejects"])
+    Xg, t = fs.get_state()
+    assert np.abs(Xg - ro["X"]).max() / np.abs(ro["X"]).max() < 1e-8
+    # the FP32 solve folds each patch's plate rate into K (v - vpl_p); a
+    # uniform rate in its place would move tau by ~3e-4 here
+    fs.set_state(X0)
+    st = fs.solve(t_end, rtol=1e-6, prec=L.PREC_FP32)
+    X32, _ = fs.get_state()
+    d = np.abs(X32 - ro["X"]).reshape(-1, 4).max(0) / np.abs(ro["X"]).reshape(-1, 4).max(0)
+    print("FP32 fold vs oracle, per component:", d)
+    assert st["status"] == 0 and d[1] < 2e-5 and d[2] < 2e-5, d
+    fs.close()
+
+
 # ------------------------------------------------------------------ monitor
 def test_monitor_rows_match_oracle():
     """rsf_monitor rows (step, t, t [yr], dt, log10 max|v|, mean slip, mean
