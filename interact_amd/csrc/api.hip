@@ -706,6 +706,7 @@ int rsfg_create(int device, const rsfg_patch* pt, int n, rsfg_handle** out) {
       h->has_tri = true;
       continue;
     }
+    h->has_rect = true;
     // angle range fix-up of check_fault_angles (src/geometry/geometry.c:985-1015)
     double dip = (double)q.dip, strike = (double)q.strike;
     if (dip > 90.0) { dip = 180.0 - dip; strike += 180.0; }
@@ -797,6 +798,7 @@ int rsfg_create_member(rsfg_handle* geom, rsfg_handle** out) {
   h->d_area = geom->d_area;
   h->assembled = true;
   h->has_tri = geom->has_tri;
+  h->has_rect = geom->has_rect;
   h->slip_mode = geom->slip_mode;
   h->ncomp = geom->ncomp;
   h->dim = geom->dim;

@@ -101,38 +101,51 @@ __device__ __forceinline__ bool tri_pair_traction(const double* __restrict__ geo
 #pragma unroll
   for (int k = 0; k < 9; ++k) v[k] = geo[(kGeoV0 + k) * n + j];
   const double P[3] = {geo[kGeoX * n + i], geo[kGeoY * n + i], geo[kGeoZ * n + i]};
-  double S[2][3][3];
-  if (!tri_unit_stress(P, v, 2.0 * mu, lambda, S)) {
+  double nr[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) nr[k] = geo[(kGeoN0 + k) * n + i];
+  double t[2][3];
+  if (!tri_unit_traction(P, v, nr, 2.0 * mu, lambda, t)) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) T[k][0] = T[k][1] = T[k][2] = 0.0;
     return true;
   }
-  double nr[3], sr[3], dr[3];
+  // the receiver's strike and dip vectors only now (not live during the
+  // stress evaluation)
+  double sr[3], dr[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    nr[k] = geo[(kGeoN0 + k) * n + i];
     sr[k] = geo[(kGeoS0 + k) * n + i];
     dr[k] = geo[(kGeoD0 + k) * n + i];
   }
 #pragma unroll
   for (int k = 0; k < 2; ++k) {
-    double t[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) t[a] = nr[0] * S[k][a][0] + nr[1] * S[k][a][1] + nr[2] * S[k][a][2];
-    T[k][0] = t[0] * sr[0] + t[1] * sr[1] + t[2] * sr[2];
-    T[k][1] = t[0] * dr[0] + t[1] * dr[1] + t[2] * dr[2];
-    T[k][2] = t[0] * nr[0] + t[1] * nr[1] + t[2] * nr[2];
+    T[k][0] = t[k][0] * sr[0] + t[k][1] * sr[1] + t[k][2] * sr[2];
+    T[k][1] = t[k][0] * dr[0] + t[k][1] * dr[1] + t[k][2] * dr[2];
+    T[k][2] = t[k][0] * nr[0] + t[k][1] * nr[1] + t[k][2] * nr[2];
   }
   return false;
 }
 
-// source-type dispatch: meshes without triangles compile the Okada path only
-template <bool TRI>
+// source-type dispatch by the mesh's kinds of patch (kSrcRect: rectangles
+// only, kSrcTri: triangles only, kSrcMixed: both): a kernel compiles only
+// the paths its mesh can reach, so its register budget is theirs
+enum { kSrcRect = 0, kSrcMixed = 1, kSrcTri = 2 };
+template <int SRC>
 __device__ __forceinline__ bool any_pair_traction(const double* __restrict__ geo, int n, int i,
                                                   int j, bool full, const OkMedium& med,
                                                   double lambda, double mu, double T[2][3]) {
-  if (TRI && geo[kGeoTri * n + j] != 0.0) return tri_pair_traction(geo, n, i, j, lambda, mu, T);
-  return pair_traction(geo, n, i, j, full, med, lambda, mu, T);
+  if constexpr (SRC == kSrcTri) {
+    return tri_pair_traction(geo, n, i, j, lambda, mu, T);
+  } else {
+    if (SRC == kSrcMixed && geo[kGeoTri * n + j] != 0.0)
+      return tri_pair_traction(geo, n, i, j, lambda, mu, T);
+    return pair_traction(geo, n, i, j, full, med, lambda, mu, T);
+  }
+}
+
+static int mesh_sources(const rsfg_ctx* h) {
+  return h->has_tri ? (h->has_rect ? kSrcMixed : kSrcTri) : kSrcRect;
 }
 
 __device__ __forceinline__ void store_entry(const AsmArgs& a, size_t idx,
@@ -153,13 +166,13 @@ __device__ __forceinline__ void store_entry(const AsmArgs& a, size_t idx,
 #ifndef RSFG_ASM_MINW
 #define RSFG_ASM_MINW 1
 #endif
-template <bool TRI>
+template <int SRC>
 __global__ __launch_bounds__(256, RSFG_ASM_MINW) void k_assemble(AsmArgs a) {
   const int j = blockIdx.x * 256 + threadIdx.x;
   const int i = blockIdx.y;
   if (j >= a.n) return;
   double T[2][3];
-  any_pair_traction<TRI>(a.geo, a.n, i, j, a.full != 0, a.med, a.lambda, a.mu, T);
+  any_pair_traction<SRC>(a.geo, a.n, i, j, a.full != 0, a.med, a.lambda, a.mu, T);
   const int n = a.n, G = a.G, nc = a.ncomp;
   const double sc = a.scale;
   switch (a.mode) {
@@ -197,7 +210,7 @@ __global__ __launch_bounds__(256, RSFG_ASM_MINW) void k_assemble(AsmArgs a) {
 }
 
 // bulk traction blocks for arbitrary pairs (unscaled, internal units)
-template <bool TRI>
+template <int SRC>
 __global__ __launch_bounds__(256) void k_pairs(const double* geo, int n,
                                                long np, const int* recv,
                                                const int* src, int full,
@@ -206,7 +219,7 @@ __global__ __launch_bounds__(256) void k_pairs(const double* geo, int n,
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   if (p >= np) return;
   double T[2][3];
-  any_pair_traction<TRI>(geo, n, recv[p], src[p], full != 0, med, lambda, mu, T);
+  any_pair_traction<SRC>(geo, n, recv[p], src[p], full != 0, med, lambda, mu, T);
 #pragma unroll
   for (int k = 0; k < 2; ++k)
 #pragma unroll
@@ -245,10 +258,16 @@ int launch_assemble(rsfg_ctx* h, double* gpu_ms) {
   a.Kb16 = h->Kb16;
   dim3 grid((h->n + 255) / 256, h->n);
   RSFG_HIP(hipEventRecord(h->ev0, h->stream));
-  if (h->has_tri)
-    hipLaunchKernelGGL(k_assemble<true>, grid, dim3(256), 0, h->stream, a);
-  else
-    hipLaunchKernelGGL(k_assemble<false>, grid, dim3(256), 0, h->stream, a);
+  switch (mesh_sources(h)) {
+    case kSrcTri:
+      hipLaunchKernelGGL(k_assemble<kSrcTri>, grid, dim3(256), 0, h->stream, a);
+      break;
+    case kSrcMixed:
+      hipLaunchKernelGGL(k_assemble<kSrcMixed>, grid, dim3(256), 0, h->stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL(k_assemble<kSrcRect>, grid, dim3(256), 0, h->stream, a);
+  }
   RSFG_HIP(hipGetLastError());
   RSFG_HIP(hipEventRecord(h->ev1, h->stream));
   RSFG_HIP(hipEventSynchronize(h->ev1));
@@ -264,12 +283,15 @@ int launch_pairs(rsfg_ctx* h, long np, const int* d_recv, const int* d_src,
   double lambda, mu;
   internal_medium(m, lambda, mu);
   const long nb = (np + 255) / 256;
-  if (h->has_tri)
-    hipLaunchKernelGGL(k_pairs<true>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->d_geo,
-                       h->n, np, d_recv, d_src, full, m, lambda, mu, d_out);
-  else
-    hipLaunchKernelGGL(k_pairs<false>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->d_geo,
-                       h->n, np, d_recv, d_src, full, m, lambda, mu, d_out);
+#define RSFG_PAIRS(SRC)                                                                      \
+  hipLaunchKernelGGL(k_pairs<SRC>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->d_geo, h->n, \
+                     np, d_recv, d_src, full, m, lambda, mu, d_out)
+  switch (mesh_sources(h)) {
+    case kSrcTri: RSFG_PAIRS(kSrcTri); break;
+    case kSrcMixed: RSFG_PAIRS(kSrcMixed); break;
+    default: RSFG_PAIRS(kSrcRect);
+  }
+#undef RSFG_PAIRS
   RSFG_HIP(hipGetLastError());
   return RSFG_OK;
 }

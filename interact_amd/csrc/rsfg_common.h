@@ -111,6 +111,7 @@ struct rsfg_ctx {
   // operator
   bool assembled = false;
   bool has_tri = false;        // any triangular patch (selects the assembly kernel)
+  bool has_rect = false;       // any rectangular patch
   int slip_mode = 0, ncomp = 1, with_normal = 0, full_space = 0, store = 0;
   int G = 1;                  // rows per patch (ncomp shear + normal)
   int rows_all = 0;           // G * n
