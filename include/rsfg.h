@@ -162,6 +162,18 @@ int rsfg_entries(rsfg_handle *h, int m, const int *rows, int k,
 int rsfg_traction_pairs(rsfg_handle *h, long npairs, const int *recv,
                         const int *src, int full_space, double *out);
 
+/* Vertical-edge tolerance [rad] of the triangles' free-surface correction
+   for later rsfg_assemble and rsfg_traction_pairs calls.  An edge within
+   the tolerance of vertical carries no correction term
+   (AngSetupFSC_S, src/green/tdstresshs.f90:398-401, where the tolerance is
+   EPS_FOR_FORTRAN, 5e-15; vertical_tol <= 0 restores it).  No reference
+   counterpart as a setting: for edges a little further from vertical the
+   correction's two end terms grow like cot(beta)^2 and cancel, and in FP64
+   the reference's rule loses every digit of far entries there (on a rough
+   32000-triangle mesh, errors up to 0.23 of a row's largest entry).
+   3e-3 keeps such entries within 5e-7 of the row scale.  Must be < 0.1. */
+int rsfg_set_triangle_tolerance(rsfg_handle *h, double vertical_tol);
+
 /* ---- stress product ---------------------------------------------------
    MatMult(Is, x, y) (src/rsf_engine.c:168-170): y = K x with the stored
    operator of precision prec (one RSFG_PREC_* bit).  The reduced formats

@@ -978,6 +978,14 @@ int rsfg_entries(rsfg_handle* h, int m, const int* rows, int k, const int* cols,
   return RSFG_OK;
 }
 
+int rsfg_set_triangle_tolerance(rsfg_handle* h, double vertical_tol) {
+  RSFG_CHECK(h, RSFG_EINVAL, "rsfg_set_triangle_tolerance: null handle");
+  RSFG_CHECK(vertical_tol < 0.1, RSFG_EINVAL,
+             "rsfg_set_triangle_tolerance: tolerance must be below 0.1 rad");
+  h->tri_vtol = vertical_tol > 0.0 ? vertical_tol : 5.0e-15;
+  return RSFG_OK;
+}
+
 int rsfg_traction_pairs(rsfg_handle* h, long np, const int* recv, const int* src, int full,
                         double* out) {
   RSFG_CHECK(h && np >= 0 && recv && src && out, RSFG_EINVAL, "rsfg_traction_pairs: bad args");

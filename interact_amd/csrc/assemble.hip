@@ -16,7 +16,7 @@ namespace rsfg {
 struct AsmArgs {
   const double* geo;
   int n, full, mode, G, ncomp, with_normal;
-  double scale, lambda, mu;
+  double scale, lambda, mu, vtol;
   OkMedium med;
   long ld;
   double* K64;
@@ -93,9 +93,10 @@ __device__ __forceinline__ bool pair_traction(const double* __restrict__ geo,
 // evaluates triangles, src/eval_green.c:156-168): T[k] = (strike, dip,
 // normal) components of S n on the receiver plane (resolve_force /
 // resolve_stress_on_fault, src/geometry/geometry.c:73-111).  Returns true
-// when the stress is not finite (T = 0, the reference's entry 0).
+// when the stress is not finite (T = 0, the reference's entry 0).  vtol:
+// the vertical-edge tolerance of the free-surface correction.
 __device__ __forceinline__ bool tri_pair_traction(const double* __restrict__ geo, int n, int i,
-                                                  int j, double lambda, double mu,
+                                                  int j, double lambda, double mu, double vtol,
                                                   double T[2][3]) {
   double v[9];
 #pragma unroll
@@ -105,7 +106,7 @@ __device__ __forceinline__ bool tri_pair_traction(const double* __restrict__ geo
 #pragma unroll
   for (int k = 0; k < 3; ++k) nr[k] = geo[(kGeoN0 + k) * n + i];
   double t[2][3];
-  if (!tri_unit_traction(P, v, nr, 2.0 * mu, lambda, t)) {
+  if (!tri_unit_traction(P, v, nr, 2.0 * mu, lambda, vtol, t)) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) T[k][0] = T[k][1] = T[k][2] = 0.0;
     return true;
@@ -134,12 +135,13 @@ enum { kSrcRect = 0, kSrcMixed = 1, kSrcTri = 2 };
 template <int SRC>
 __device__ __forceinline__ bool any_pair_traction(const double* __restrict__ geo, int n, int i,
                                                   int j, bool full, const OkMedium& med,
-                                                  double lambda, double mu, double T[2][3]) {
+                                                  double lambda, double mu, double vtol,
+                                                  double T[2][3]) {
   if constexpr (SRC == kSrcTri) {
-    return tri_pair_traction(geo, n, i, j, lambda, mu, T);
+    return tri_pair_traction(geo, n, i, j, lambda, mu, vtol, T);
   } else {
     if (SRC == kSrcMixed && geo[kGeoTri * n + j] != 0.0)
-      return tri_pair_traction(geo, n, i, j, lambda, mu, T);
+      return tri_pair_traction(geo, n, i, j, lambda, mu, vtol, T);
     return pair_traction(geo, n, i, j, full, med, lambda, mu, T);
   }
 }
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(256, RSFG_ASM_MINW) void k_assemble(AsmArgs a) {
   const int i = blockIdx.y;
   if (j >= a.n) return;
   double T[2][3];
-  any_pair_traction<SRC>(a.geo, a.n, i, j, a.full != 0, a.med, a.lambda, a.mu, T);
+  any_pair_traction<SRC>(a.geo, a.n, i, j, a.full != 0, a.med, a.lambda, a.mu, a.vtol, T);
   const int n = a.n, G = a.G, nc = a.ncomp;
   const double sc = a.scale;
   switch (a.mode) {
@@ -215,11 +217,11 @@ __global__ __launch_bounds__(256) void k_pairs(const double* geo, int n,
                                                long np, const int* recv,
                                                const int* src, int full,
                                                OkMedium med, double lambda,
-                                               double mu, double* out) {
+                                               double mu, double vtol, double* out) {
   const long p = (long)blockIdx.x * 256 + threadIdx.x;
   if (p >= np) return;
   double T[2][3];
-  any_pair_traction<SRC>(geo, n, recv[p], src[p], full != 0, med, lambda, mu, T);
+  any_pair_traction<SRC>(geo, n, recv[p], src[p], full != 0, med, lambda, mu, vtol, T);
 #pragma unroll
   for (int k = 0; k < 2; ++k)
 #pragma unroll
@@ -251,6 +253,7 @@ int launch_assemble(rsfg_ctx* h, double* gpu_ms) {
   a.with_normal = h->with_normal;
   a.scale = h->scale;
   internal_medium(a.med, a.lambda, a.mu);
+  a.vtol = h->tri_vtol;
   a.ld = h->ld;
   a.K64 = h->K64;
   a.K32 = h->K32;
@@ -285,7 +288,7 @@ int launch_pairs(rsfg_ctx* h, long np, const int* d_recv, const int* d_src,
   const long nb = (np + 255) / 256;
 #define RSFG_PAIRS(SRC)                                                                      \
   hipLaunchKernelGGL(k_pairs<SRC>, dim3((unsigned)nb), dim3(256), 0, h->stream, h->d_geo, h->n, \
-                     np, d_recv, d_src, full, m, lambda, mu, d_out)
+                     np, d_recv, d_src, full, m, lambda, mu, h->tri_vtol, d_out)
   switch (mesh_sources(h)) {
     case kSrcTri: RSFG_PAIRS(kSrcTri); break;
     case kSrcMixed: RSFG_PAIRS(kSrcMixed); break;

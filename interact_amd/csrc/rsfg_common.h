@@ -112,6 +112,7 @@ struct rsfg_ctx {
   bool assembled = false;
   bool has_tri = false;        // any triangular patch (selects the assembly kernel)
   bool has_rect = false;       // any rectangular patch
+  double tri_vtol = 5.0e-15;   // triangles' vertical-edge tolerance [rad] (the reference's)
   int slip_mode = 0, ncomp = 1, with_normal = 0, full_space = 0, store = 0;
   int G = 1;                  // rows per patch (ncomp shear + normal)
   int rows_all = 0;           // G * n

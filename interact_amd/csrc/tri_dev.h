@@ -23,8 +23,12 @@
 
 namespace rsfg {
 
-// the reference's pi in the angle offsets and its edge tolerance
-// (src/green/tdstresshs.f90 pi parameter, EPS_FOR_FORTRAN)
+// the reference's pi in the angle offsets and its vertical-edge tolerance
+// (src/green/tdstresshs.f90 pi parameter, EPS_FOR_FORTRAN): an edge whose
+// angle beta from the downward vertical is within the tolerance of 0 or pi
+// carries no free-surface correction.  Edges a little further from vertical
+// make the correction's two end terms grow like cot(beta)^2 and cancel, so
+// a caller may widen the tolerance (rsfg_set_triangle_tolerance).
 constexpr double kTriPi = 3.14159265358979;
 constexpr double kTriEps = 5.0e-15;
 constexpr double kTriNu = 0.25;
@@ -194,14 +198,16 @@ __device__ __forceinline__ bool td_fs_traction(const double* P, const double* p1
 // bs (strike slip) and bd (dip slip), its receiver-plane tractions added to
 // t (AngSetupFSC_S, src/green/tdstresshs.f90:371-476).  The edge frame A is
 // symmetric (its own transpose); the two ends' gradients are consumed one
-// after the other.
+// after the other.  vtol: the vertical-edge tolerance (kTriEps is the
+// reference's).
 __device__ __forceinline__ void fsc_edge_traction(const double* P, const double* bs,
                                                   const double* bd, const double* pa,
                                                   const double* pb, const double* nr,
-                                                  double mu2, double lambda, double t[2][3]) {
+                                                  double mu2, double lambda, double vtol,
+                                                  double t[2][3]) {
   const double sv[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
   const double beta = acos(-sv[2] / sqrt(tdot(sv, sv)));
-  if (fabs(beta) < kTriEps || fabs(kTriPi - beta) < kTriEps) return;
+  if (fabs(beta) < vtol || fabs(kTriPi - beta) < vtol) return;
   double e1[3] = {sv[0], sv[1], 0.0};
   tunit(e1);
   // frame (e1, (-e_z) x e1, -e_z)
@@ -259,18 +265,18 @@ __device__ __forceinline__ void fsc_edge_traction(const double* P, const double*
 // constants (mu2 = 2 mu, lambda): the main dislocation, the free-surface
 // correction of its three edges, and the image dislocation.  false when the
 // result is not finite (the reference's iret = 1, entry 0).  A triangle with
-// all vertices at z = 0 gives zero.
+// all vertices at z = 0 gives zero.  vtol: the vertical-edge tolerance.
 __device__ __forceinline__ bool tri_unit_traction(const double* P, const double* v,
                                                   const double* nr, double mu2, double lambda,
-                                                  double t[2][3]) {
+                                                  double vtol, double t[2][3]) {
 #pragma unroll
   for (int s = 0; s < 2; ++s) t[s][0] = t[s][1] = t[s][2] = 0.0;
   if (v[2] == 0.0 && v[5] == 0.0 && v[8] == 0.0) return true;
   double vs[3], vd[3];
   if (!td_fs_traction(P, v, v + 3, v + 6, nr, mu2, lambda, t, vs, vd)) return false;
-  fsc_edge_traction(P, vs, vd, v, v + 3, nr, mu2, lambda, t);
-  fsc_edge_traction(P, vs, vd, v + 3, v + 6, nr, mu2, lambda, t);
-  fsc_edge_traction(P, vs, vd, v + 6, v, nr, mu2, lambda, t);
+  fsc_edge_traction(P, vs, vd, v, v + 3, nr, mu2, lambda, vtol, t);
+  fsc_edge_traction(P, vs, vd, v + 3, v + 6, nr, mu2, lambda, vtol, t);
+  fsc_edge_traction(P, vs, vd, v + 6, v, nr, mu2, lambda, vtol, t);
   const double w[9] = {v[0], v[1], -v[2], v[3], v[4], -v[5], v[6], v[7], -v[8]};
   double vsi[3], vdi[3];
   if (!td_fs_traction(P, w, w + 3, w + 6, nr, mu2, lambda, t, vsi, vdi)) return false;

@@ -31,6 +31,7 @@ EXPORTS = [
     "rsfg_last_error", "rsfg_version", "rsfg_device_count", "rsfg_create",
     "rsfg_destroy", "rsfg_read_patch_file", "rsfg_assemble",
     "rsfg_operator_dims", "rsfg_operator_bytes", "rsfg_entries", "rsfg_traction_pairs",
+    "rsfg_set_triangle_tolerance",
     "rsfg_matvec", "rsfg_matvec_dev", "rsfg_matvec_dev_noalloc", "rsfg_matvec_t",
     "rsfg_matvec_bench",
     "rsfg_set_friction", "rsfg_set_loading", "rsfg_rhs",
@@ -151,6 +152,7 @@ def load(path=None):
                                      C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.rsfg_entries.argtypes = [h, C.c_int, _ip, C.c_int, _ip, _dp]
     L.rsfg_traction_pairs.argtypes = [h, C.c_long, _ip, _ip, C.c_int, _dp]
+    L.rsfg_set_triangle_tolerance.argtypes = [h, C.c_double]
     L.rsfg_matvec.argtypes = [h, C.c_int, _dp, _dp]
     L.rsfg_matvec_dev.argtypes = [h, C.c_int, C.c_void_p, C.c_void_p]
     L.rsfg_matvec_dev_noalloc.argtypes = [h, C.c_int, C.c_void_p, C.c_void_p]

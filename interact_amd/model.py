@@ -121,6 +121,14 @@ class FaultSystem:
         r, c, _ = self.dims()
         return self.entries(r + np.arange(self.n), np.arange(c))
 
+    def set_triangle_tolerance(self, vertical_tol=0.0):
+        """vertical-edge tolerance [rad] of the triangles' free-surface
+        correction for later assemblies and traction_pairs; 0 restores the
+        reference's 5e-15 (3e-3 keeps near-vertical edges' far entries
+        accurate; see include/rsfg.h)"""
+        L.check(self._lib.rsfg_set_triangle_tolerance(self.h, float(vertical_tol)),
+                "rsfg_set_triangle_tolerance")
+
     def traction_pairs(self, recv, src, full_space=False):
         recv = np.ascontiguousarray(recv, np.int32)
         src = np.ascontiguousarray(src, np.int32)

@@ -92,6 +92,11 @@ def _setup(L):
     L.rsfo_tri_frame.argtypes = [_dp] * 7
     L.rsfo_tri_stress.restype = C.c_int
     L.rsfo_tri_stress.argtypes = [_dp] * 4 + [C.c_double] * 6 + [_dp]
+    L.rsfo_set_tri_vertical_tol.restype = None
+    L.rsfo_set_tri_vertical_tol.argtypes = [C.c_double]
+    L.rsfo_mesh_traction_pairs.restype = None
+    L.rsfo_mesh_traction_pairs.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                                           C.c_int, C.c_longlong, _ip, _ip, _dp]
     L.rsfo_mesh_traction_tensor.restype = None
     L.rsfo_mesh_traction_tensor.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp,
                                             _dp, C.c_int, C.c_int, _dp,
@@ -227,6 +232,29 @@ def mesh_receivers(g):
         else:
             fr[i] = patch_frame(g["strike"][i], g["dip"][i])
     return is_tri, xyz, fr, vert
+
+
+def set_tri_vertical_tol(tol=0.0):
+    """vertical-edge tolerance [rad] of the triangles' free-surface
+    correction for later evaluations; 0 restores the reference's 5e-15"""
+    lib().rsfo_set_tri_vertical_tol(float(tol))
+
+
+def mesh_traction_pairs(g, recv, src, full_space=0, nthreads=8, rx=None):
+    """T[p, slip, trac] for the (recv[p], src[p]) pairs of a mesh of
+    rectangles and triangles; rx: mesh_receivers(g), when already formed"""
+    n = len(g["l"])
+    is_tri, xyz, fr, vert = mesh_receivers(g) if rx is None else rx
+    recv = np.ascontiguousarray(recv, np.int32)
+    src = np.ascontiguousarray(src, np.int32)
+    assert recv.shape == src.shape and recv.min() >= 0 and src.min() >= 0
+    assert recv.max() < n and src.max() < n
+    T = np.zeros(len(recv) * 6)
+    lib().rsfo_mesh_traction_pairs(
+        n, is_tri, xyz.ravel(), fr.ravel(), np.ascontiguousarray(g["l"], np.float64),
+        np.ascontiguousarray(g["w"], np.float64), vert.ravel(), int(full_space),
+        int(nthreads), len(recv), recv, src, T)
+    return T.reshape(-1, 2, 3)
 
 
 def mesh_traction_tensor(g, full_space=0, nthreads=8, want_iret=False):

@@ -63,6 +63,13 @@ int rsfo_tri_stress(const double *P, const double *p1, const double *p2,
                     const double *p3, double ss, double ds, double ts,
                     double nu, double mu2, double lambda, double S[3][3]);
 
+/* vertical-edge tolerance [rad] of the free-surface correction: an edge
+   whose angle from the downward vertical is within it of 0 or pi carries
+   no correction (AngSetupFSC_S, src/green/tdstresshs.f90:398-401, with
+   EPS_FOR_FORTRAN = 5e-15, restored by tol <= 0).  Process-wide; not
+   for use while an evaluation runs. */
+void rsfo_set_tri_vertical_tol(double tol);
+
 /* tractions (strike, dip, normal) at every receiver for unit strike and
    unit dip slip on every source of a mesh of rectangles and triangles,
    internal units (shear modulus 1e4, nu 0.25), as GenKEntries_petsc sees
@@ -76,6 +83,14 @@ void rsfo_mesh_traction_tensor(int n, const int *is_tri, const double *xyz,
                                const double *w, const double *verts,
                                int full_space, int nthreads, double *T,
                                int *iret);
+
+/* the same tractions for np (recv[p], source src[p]) pairs of the mesh
+   into T[6 p + 3 s + t] */
+void rsfo_mesh_traction_pairs(int n, const int *is_tri, const double *xyz,
+                              const double *fr, const double *l, const double *w,
+                              const double *verts, int full_space, int nthreads,
+                              long long np, const int *recv, const int *src,
+                              double *T);
 
 /* patch-file reader with the reference's scan format
    "%lf %lf %lf %f %f %lf %lf %i" (src/includes/precision_double.h:31,

@@ -21,6 +21,11 @@ Outputs
                         surface-touching, a horizontal and a vertical-edge
                         one, and a mixed rectangle + triangle mesh (half and
                         full space)
+  tri_vertical_edge.npz reference tractions among six triangles of the
+                        32000-triangle rough mesh (0.5 km cells, Hurst 0.8
+                        offset): three receivers near the surface or the
+                        ends, each with a far source one of whose edges lies
+                        within 1e-5 rad of vertical
   dc3d_cases.npz        direct DC3D calls: random points, points on edges,
                         on negative edge extensions, at the surface, and
                         dips 0 / 90 / 89.9999995 (cos snap) / 45
@@ -127,6 +132,25 @@ def triangles(tmp):
     np.savez_compressed(os.path.join(HERE, "ktensor_tri.npz"), **out)
 
 
+def vertical_edges(tmp):
+    """the reference's tractions among patches 0, 154, 31952, 31709, 3994
+    and 3761 of the rough 0.5 km triangle mesh; sources 154, 31709 and 3761
+    have an edge 6e-6, 1e-5 and 1e-5 rad from vertical"""
+    sys.path.insert(0, ROOT)
+    from interact_amd import faults
+    g = faults.planar_tri_grid(0.5, offset=faults.rough_offset(200, 80, 0.5))
+    ids = np.array([0, 154, 31952, 31709, 3994, 3761])
+    vert = np.asarray(g["vert"]).reshape(-1, 9)[ids]
+    m = len(ids)
+    sub = dict(xyz=(vert[:, 0:3] + vert[:, 3:6] + vert[:, 6:9]) / 3.0, strike=np.zeros(m),
+               dip=np.full(m, 90.0), l=np.full(m, -1.0), w=np.full(m, -1.0), vert=vert)
+    gfile = os.path.join(tmp, "vertical.in")
+    write_mesh(gfile, sub)
+    T, ir = kmat(gfile, 0)
+    np.savez_compressed(os.path.join(HERE, "tri_vertical_edge.npz"), ids=ids, vert=vert, T=T,
+                        iret=ir, geom=open(gfile).read())
+
+
 def monitor_excerpt(ref, rr, nrows=40):
     """the header and first nrows rows of a dense rsf_monitor.dat written by
     the reference (test_hmatrix/out_2f_2.0km_sep4.0km_260yr_np4): fixture
@@ -144,7 +168,7 @@ def monitor_excerpt(ref, rr, nrows=40):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
-    ap.add_argument("--only", choices=["tri"], default=None,
+    ap.add_argument("--only", choices=["tri", "vertical"], default=None,
                     help="write only one group of fixtures")
     args = ap.parse_args()
     ref = args.ref
@@ -155,6 +179,11 @@ def main():
         triangles(tmp)
         shutil.rmtree(tmp)
         print("triangle fixtures written to", HERE)
+        return
+    if args.only == "vertical":
+        vertical_edges(tmp)
+        shutil.rmtree(tmp)
+        print("vertical-edge fixture written to", HERE)
         return
 
     # 1. the full_vs_half mesh (scaling_tests/full_vs_half_test.sh:85-86)
@@ -244,6 +273,7 @@ def main():
     monitor_excerpt(ref, rr)
     # 6. triangle meshes
     triangles(tmp)
+    vertical_edges(tmp)
     shutil.rmtree(tmp)
     print("golden vectors written to", HERE)
 

@@ -112,6 +112,59 @@ def test_traction_pairs_triangles(name, tol):
     fs.close()
 
 
+def test_rough_mesh_full_size_vertical_edge_tolerance():
+    """The 32000-triangle rough mesh of the 100 x 40 km outline at 0.5 km
+    (the benchmark's assembly workload): device tractions of 6 receivers
+    from every source vs the oracle's, with the reference's vertical-edge
+    tolerance and with 3e-3 rad.  With the reference's, sources with an
+    edge within 1e-2 rad of vertical are ill-conditioned (the free-surface
+    correction's cot(beta)^2 terms, see the oracle's vertical-edge test):
+    the two FP64 codes differ there by up to a fifth of a row's self term.
+    Every triangle of this mesh has an edge within 3e-2 rad of vertical
+    (median 5e-3): sources whose edges stay 1e-2 rad or more from vertical
+    agree to 5e-8 of the self term, those 3e-3 to 1e-2 rad from it to 1e-5.
+    With 3e-3 rad every source agrees to 1e-5, and the sources with no edge
+    within 3e-3 rad keep their values."""
+    from interact_amd import faults
+    g = faults.planar_tri_grid(0.5, offset=faults.rough_offset(200, 80, 0.5))
+    n = len(g["l"])
+    assert n == 32000
+    V = np.asarray(g["vert"]).reshape(n, 3, 3)
+    sv = np.stack([V[:, 1] - V[:, 0], V[:, 2] - V[:, 1], V[:, 0] - V[:, 2]], 1)
+    beta = np.arccos(np.clip(-sv[..., 2] / np.linalg.norm(sv, axis=-1), -1.0, 1.0))
+    off_vertical = np.minimum(beta, np.pi - beta).min(1)
+    well = off_vertical >= 1e-2
+    fair = (off_vertical >= 3e-3) & ~well
+    recv_ids = np.array([0, 3994, 11982, 15976, 27958, 31952])
+    recv = np.repeat(recv_ids, n)
+    src = np.tile(np.arange(n), len(recv_ids))
+    rx = oracle.mesh_receivers(g)
+    fs = _fs(g)
+    try:
+        out = {}
+        for tol in (0.0, 3e-3):
+            fs.set_triangle_tolerance(tol)
+            oracle.set_tri_vertical_tol(tol)
+            T = fs.traction_pairs(recv, src).reshape(len(recv_ids), n, 2, 3)
+            To = oracle.mesh_traction_pairs(g, recv, src, rx=rx).reshape(len(recv_ids), n, 2, 3)
+            self_term = np.abs(To[np.arange(len(recv_ids)), recv_ids]).max(axis=(1, 2))
+            err = np.abs(T - To).max(axis=(2, 3)) / self_term[:, None]
+            out[tol] = T
+            print("tolerance %g: device vs oracle / self term: edges >= 1e-2 rad from vertical "
+                  "%.2e, 3e-3..1e-2 %.2e, all %.2e" % (tol, err[:, well].max(), err[:, fair].max(),
+                                                       err.max()), flush=True)
+            assert err[:, well].max() <= 5e-8
+            assert err[:, fair].max() <= 1e-5
+            if tol == 0.0:
+                assert err.max() > 1e-3      # the conditioning, seen
+            else:
+                assert err.max() <= 1e-5
+        assert np.array_equal(out[0.0][:, off_vertical >= 3e-3], out[3e-3][:, off_vertical >= 3e-3])
+    finally:
+        oracle.set_tri_vertical_tol(0.0)
+        fs.close()
+
+
 def test_assembly_triangles_vector_mode():
     """Assembled two-component shear operator of the planar triangle grid
     (with the normal rows present in the layout) = scaled golden tractions;

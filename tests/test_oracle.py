@@ -100,6 +100,59 @@ def test_triangle_free_surface_is_traction_free():
             assert np.abs(S[:, 2]).max() <= 1e-12 * np.abs(Sb).max()
 
 
+def test_triangle_vertical_edge_tolerance():
+    """Edges a little off vertical.  The free-surface correction of an edge
+    is the difference of two angular-dislocation terms that grow like
+    cot(beta)^2 as the edge nears vertical (beta -> 0 or pi), so in FP64
+    it loses every digit of a far entry there.  Sources 1, 3 and 5 of the
+    fixture (triangles of the rough 0.5 km mesh) have an edge 6e-6, 6e-6
+    and 1.4e-4 rad from vertical.  At their receivers, 17-39 km away, the
+    tractions of an equal-area rectangle on the mean plane (the same moment
+    and orientation, Okada 1992) stand in for the far field.  With the
+    reference's tolerance (5e-15) the reference build
+    (tests/golden/tri_vertical_edge.npz) and the oracle are off that far
+    field by 1e2-1e7 times its size, and differ from each other.  With a
+    3e-3 rad tolerance the oracle lands within 5e-3 of it.  Sources
+    without an edge within 3e-3 of vertical keep the reference's values."""
+    d = np.load(os.path.join(GOLDEN, "tri_vertical_edge.npz"))
+    vert, Tref = d["vert"], d["T"]
+    m = len(vert)
+    g = dict(xyz=(vert[:, 0:3] + vert[:, 3:6] + vert[:, 6:9]) / 3, strike=np.zeros(m),
+             dip=np.full(m, 90.0), l=-np.ones(m), w=-np.ones(m), vert=vert)
+    V = vert.reshape(m, 3, 3)
+    sv = np.stack([V[:, 1] - V[:, 0], V[:, 2] - V[:, 1], V[:, 0] - V[:, 2]], 1)
+    beta = np.arccos(-sv[..., 2] / np.linalg.norm(sv, axis=-1))
+    off_vertical = np.minimum(beta, np.pi - beta).min(1)
+    assert (off_vertical[[1, 3, 5]] < 2e-4).all()
+    To = oracle.mesh_traction_tensor(g, 0, 1)
+    try:
+        oracle.set_tri_vertical_tol(3e-3)
+        Tt = oracle.mesh_traction_tensor(g, 0, 1)
+    finally:
+        oracle.set_tri_vertical_tol(0.0)
+    assert np.array_equal(oracle.mesh_traction_tensor(g, 0, 1), To)
+    for i, j in ((0, 1), (2, 3), (4, 5)):
+        _, _, _, area = oracle.tri_frame(vert[j])
+        h = np.sqrt(area) / 2
+        rect = dict(xyz=np.array([g["xyz"][i], V[j].mean(0)]), strike=np.zeros(2),
+                    dip=np.full(2, 90.0), l=np.array([-1.0, h]), w=np.array([-1.0, h]),
+                    vert=np.array([vert[i], np.zeros(9)]))
+        far = oracle.mesh_traction_tensor(rect, 0, 1)[0, 1]
+        s = np.abs(far).max()
+        assert np.linalg.norm(g["xyz"][i] - V[j].mean(0)) > 15e3
+        assert np.abs(Tref[i, j] - far).max() > 100 * s
+        assert np.abs(To[i, j] - far).max() > 100 * s
+        assert np.abs(Tt[i, j] - far).max() < 5e-3 * s
+    keep = off_vertical > 3e-3
+    assert keep.sum() == 2
+    assert np.array_equal(Tt[:, keep], To[:, keep])
+    # their edges are 8e-3 and 1e-2 from vertical, where cot(beta)^2 ~ 1e4
+    # still costs digits: the two FP64 codes agree to 1e-8 of a row's self
+    # term (3e-9 at the far receiver of the 8e-3 edge)
+    self_term = np.abs(Tref[np.arange(m), np.arange(m)]).max(axis=(1, 2))
+    assert (np.abs(To[:, keep] - Tref[:, keep]).max(axis=(1, 2, 3)) < 1e-8 * self_term).all()
+
+
 def test_triangle_operation_count_runs_the_oracle_arithmetic():
     """oracle/opcount_tri (the oracle's triangle pair compiled with an
     operation-counting scalar) gives F_pair of triangle assembly, the
