@@ -1060,21 +1060,33 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
     tb = a0.tb;
   }
   const int per = o->steps_per_launch > 0 ? o->steps_per_launch : 32;
+  // a batch: per attempts of every running member, then the halted/attempt
+  // readout; captured once in a graph and replayed (RSFG_NO_GRAPH=1 launches
+  // the same kernels directly, e.g. for tracing tools)
+  const char* ng = getenv("RSFG_NO_GRAPH");
+  const bool direct = ng && ng[0] == '1';
+  auto enqueue_batch = [&]() -> hipError_t {
+    for (int k = 0; k < per; ++k) ens_attempt(e, tb.s);
+    hipLaunchKernelGGL(k_halted_b, dim3((e->M + 255) / 256), dim3(256), 0, e->stream,
+                       ens_args(e, 0), e->M, e->d_halted);
+    return hipGetLastError();
+  };
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  {
-    // the capture is ended whatever the launches returned, so the stream
-    // never stays in capture mode
+  // the capture is ended whatever the launches returned, so the stream
+  // never stays in capture mode
+  auto capture = [&]() -> hipError_t {
     hipError_t he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
     if (he == hipSuccess) {
-      for (int k = 0; k < per; ++k) ens_attempt(e, tb.s);
-      hipLaunchKernelGGL(k_halted_b, dim3((e->M + 255) / 256), dim3(256), 0, e->stream,
-                         ens_args(e, 0), e->M, e->d_halted);
-      const hipError_t el = hipGetLastError();
+      const hipError_t el = enqueue_batch();
       he = hipStreamEndCapture(e->stream, &graph);
       if (he == hipSuccess) he = el;
     }
     if (he == hipSuccess) he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    return he;
+  };
+  if (!direct) {
+    const hipError_t he = capture();
     if (he != hipSuccess) {
       if (graph) hipGraphDestroy(graph);
       return hip_fail(he, "rsfg_ensemble_solve: capture", __FILE__, __LINE__);
@@ -1086,7 +1098,7 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
   bool replanned = false;
   long launches = 0;
   for (;;) {
-    hipError_t he = hipGraphLaunch(exec, e->stream);
+    hipError_t he = direct ? enqueue_batch() : hipGraphLaunch(exec, e->stream);
     if (he == hipSuccess)
       he = hipMemcpyAsync(hs.data(), e->d_halted, 2 * sizeof(long long) * e->M,
                           hipMemcpyDeviceToHost, e->stream);
@@ -1118,26 +1130,19 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
         if (!hs[2 * m]) now[m / e->nm] = 1;
       if (now != running) {
         running = now;
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
         rc = build_plan(e, e->cfg, &running);
         if (rc) break;
         replanned = true;
-        he = hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal);
-        if (he == hipSuccess) {
-          for (int k = 0; k < per; ++k) ens_attempt(e, tb.s);
-          hipLaunchKernelGGL(k_halted_b, dim3((e->M + 255) / 256), dim3(256), 0, e->stream,
-                             ens_args(e, 0), e->M, e->d_halted);
-          const hipError_t el = hipGetLastError();
-          he = hipStreamEndCapture(e->stream, &graph);
-          if (he == hipSuccess) he = el;
-        }
-        if (he == hipSuccess) he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (he != hipSuccess) {
-          rc = hip_fail(he, "rsfg_ensemble_solve: capture", __FILE__, __LINE__);
-          break;
+        if (!direct) {
+          he = capture();
+          if (he != hipSuccess) {
+            rc = hip_fail(he, "rsfg_ensemble_solve: capture", __FILE__, __LINE__);
+            break;
+          }
         }
       }
     }

@@ -258,6 +258,40 @@ def test_ensemble_of_64_members():
     fs.close()
 
 
+@pytest.mark.parametrize("shape", [0, 18])
+def test_direct_launches_match_graph_launches(monkeypatch, shape):
+    """RSFG_NO_GRAPH=1 launches each batch's kernels directly instead of
+    replaying the captured graph (for tracing tools): the same kernels in the
+    same order, so bitwise the same states, events and step counts, with a
+    unit shape (0) and a stream-K shape (18), whose work split is rebuilt
+    when a geometry's members have all halted."""
+    from interact_amd import faults, lib
+    from interact_amd.model import Ensemble
+    geoms = _geoms(2.0, lib.SLIP_VECTOR)
+    nm = 16
+    ens = Ensemble([q[0] for q in geoms], nm)
+    X0s = []
+    for gi, (fs, g, L, W) in enumerate(geoms):
+        for k in range(nm):
+            X0s.append(_configure(ens.members[gi * nm + k], faults.ensemble_fields(g, L, W, k, gi), 2))
+    ens.select(shape)
+    out = {}
+    for mode in ("graph", "direct"):
+        if mode == "direct":
+            monkeypatch.setenv("RSFG_NO_GRAPH", "1")
+        for m, X0 in zip(ens.members, X0s):
+            m.set_state(X0)
+        st = ens.solve(8.0 * SEC_PER_YEAR, rtol=1e-6)
+        out[mode] = [(m.get_state()[0], m.events(), q["steps"]) for m, q in zip(ens.members, st)]
+    monkeypatch.delenv("RSFG_NO_GRAPH")
+    assert sum(len(_onsets(ev)) for _, ev, _ in out["graph"]) >= 4
+    for (X, ev, n), (Xd, evd, nd) in zip(out["graph"], out["direct"]):
+        assert n == nd and np.array_equal(X, Xd) and np.array_equal(ev, evd)
+    ens.close()
+    for q in geoms:
+        q[0].close()
+
+
 def test_ensemble_rk3bs_matches_single_runs():
     """The rk3bs pair in the batched stages (four-stage tableau, its own
     controller exponent): 16 members on one geometry over 20 yr reproduce
