@@ -261,7 +261,7 @@ def kernels_only(fs, store, rough_ds=None):
     log(f"kernels-only: copy of {n} bytes done")
     if rough_ds is not None:
         fs.close()
-        rough_triangle_assembly(rough_ds)
+        rough_triangle_assembly(rough_ds, products=False)
 
 
 def main():
@@ -288,6 +288,12 @@ def main():
     ap.add_argument("--singles", type=int, default=None,
                     help="c4: members also run one after another through the single-fault "
                          "solver (default all 128)")
+    ap.add_argument("--rough-cycle-years", type=float, default=0.0,
+                    help="c3: also run cycles of this length on the rough triangle mesh "
+                         "(64000 DOF) in FP64, FP32 and bf16nf (default 0: products only)")
+    ap.add_argument("--rough-only", action="store_true",
+                    help="c3: only the rough triangle mesh (assembly, products, and the cycles "
+                         "of --rough-cycle-years); prints its JSON object")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precisions", default=None,
@@ -322,6 +328,11 @@ def main():
         print(json.dumps(adjoint_bench(args)))
         return
     args.ds = {"c2": 1.0, "c3": 0.5}[args.config]
+    if args.rough_only:
+        if args.config != "c3":
+            raise SystemExit("--rough-only belongs to c3")
+        print(json.dumps(rough_triangle_assembly(args.ds, args.rough_cycle_years, args.rtol)))
+        return
     if args.years is None:
         args.years = {"c2": 400.0, "c3": 200.0}[args.config]
     if args.precisions is None:
@@ -474,7 +485,8 @@ def main():
         del K64
     fs.close()
     if args.config == "c3":
-        out["assembly_rough_triangles"] = rough_triangle_assembly(args.ds)
+        out["assembly_rough_triangles"] = rough_triangle_assembly(
+            args.ds, args.rough_cycle_years, args.rtol)
     if not args.no_oracle_c1:
         out["errors_vs_oracle_c1"] = oracle_c1(args.rtol)
     print(json.dumps(out))
@@ -923,28 +935,84 @@ def adjoint_bench(args):
     }
 
 
-def rough_triangle_assembly(ds_km):
+ROUGH_TRI_TOL = 3e-3   # vertical-edge tolerance [rad] of the rough mesh's products and cycles
+
+
+def rough_triangle_assembly(ds_km, cycle_years=0.0, rtol=1e-8, precisions="fp64,fp32,bf16nf",
+                            products=True):
     """FP64 assembly throughput on the rough variant of the same outline:
     two Nikkhoo-Walter triangles per cell (32000 at 0.5 km) on a seeded
     Hurst 0.8 self-affine surface with amplitude-to-wavelength ratio 1e-3,
-    two slip components (64000 DOF, K 32.8 GB in FP64)."""
+    two slip components (64000 DOF, K 32.8 GB in FP64).  Timed with the
+    reference's vertical-edge rule; then assembled again with the 3e-3 rad
+    tolerance (the reference's rule leaves rounding noise near the mesh's
+    near-vertical cell edges) in FP64, FP32 and bf16nf for the K v products,
+    and (cycle_years > 0) cycles of the benchmark's friction fields."""
     from interact_amd import faults, lib
-    from interact_amd.model import FaultSystem
+    from interact_amd.model import FaultSystem, initial_state
     nx, nz = int(round(100.0 / ds_km)), int(round(40.0 / ds_km))
     g = faults.planar_tri_grid(ds_km, offset=faults.rough_offset(nx, nz, ds_km))
     n = len(g["l"])
+    D = 2 * n
     fs = FaultSystem(g)
     fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9)
     ms = fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9)
-    fs.close()
     pairs = n * n
     log(f"assembly, rough triangles: {n} triangles, {pairs} pairs in {ms:.1f} ms = "
         f"{pairs / ms * 1e3:.3e} pairs/s")
     rate = pairs / ms * 1e3
-    return dict(triangles=n, pairs=pairs, ms=ms, pairs_per_s=rate,
-                entries_per_s=4 * rate, dof=2 * n, f_pair=F_PAIR_TRI,
-                tflops=rate * F_PAIR_TRI / 1e12,
-                frac_fp64_peak=rate * F_PAIR_TRI / 1e12 / FP64_PEAK_TFLOPS)
+    out = dict(triangles=n, pairs=pairs, ms=ms, pairs_per_s=rate,
+               entries_per_s=4 * rate, dof=D, f_pair=F_PAIR_TRI,
+               tflops=rate * F_PAIR_TRI / 1e12,
+               frac_fp64_peak=rate * F_PAIR_TRI / 1e12 / FP64_PEAK_TFLOPS)
+    if not products:
+        fs.close()
+        return out
+    f = faults.cycle_fields(g)
+    fs.set_triangle_tolerance(ROUGH_TRI_TOL)
+    fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"],
+                store=lib.PREC_FP32 | lib.PREC_BF16NF)
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=f["v0"], f0=f["f0"], eta=f["eta"])
+    fs.set_loading(f["vpl"])
+    X0 = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
+    fs.set_state(X0)
+    i = np.arange(D)
+    xa = np.sin(i + 1.0) + 0.3 * np.cos(3.0 * i)
+    y64 = fs.matvec(xa, lib.PREC_FP64)
+    kv = {}
+    for name in ("fp64", "fp32", "bf16nf"):
+        p = lib.PREC_BY_NAME[name]
+        y = fs.matvec(xa, p)
+        mv_ms = float(np.median([fs.matvec_bench(p, 20) for _ in range(3)]))
+        st_ms = float(np.median([fs.stage_bench(p, 20) for _ in range(3)]))
+        by = fs.operator_bytes(p) + 16.0 * D
+        kv[name] = dict(relerr=float(np.linalg.norm(y - y64) / np.linalg.norm(y64)),
+                        matvec_ms=mv_ms, stage_ms=st_ms, matvec_gbs=by / mv_ms / 1e6,
+                        stage_gbs=by / st_ms / 1e6, bytes=by,
+                        matvec_frac=by / mv_ms / 1e6 / HBM_PEAK_GBS)
+        log(f"rough K.v {name}: relerr {kv[name]['relerr']:.2e}  matvec {mv_ms*1e3:.1f} us "
+            f"{kv[name]['matvec_gbs']:.0f} GB/s  stage {st_ms*1e3:.1f} us "
+            f"{kv[name]['stage_gbs']:.0f} GB/s")
+    out["vertical_edge_tol"] = ROUGH_TRI_TOL
+    out["kv"] = kv
+    if cycle_years > 0:
+        cyc = {}
+        for name in precisions.split(","):
+            fs.set_state(X0)
+            t0 = time.perf_counter()
+            st = fs.solve(cycle_years * SEC_PER_YEAR, rtol=rtol, prec=lib.PREC_BY_NAME[name],
+                          steps_per_launch=64)
+            wall = time.perf_counter() - t0
+            ev = fs.events()
+            on = [float(t) / SEC_PER_YEAR for t in (ev[ev[:, 1] > 0, 0] if len(ev) else [])]
+            cyc[name] = dict(wall_s=wall, steps=st["steps"], rejects=st["rejects"],
+                             onsets_yr=on, sim_years_per_hour=cycle_years / wall * 3600.0)
+            log(f"rough cycle {name}: {cycle_years:g} yr in {wall:.2f} s "
+                f"({cycle_years / wall * 3600:.3e} sim-yr/h), steps {st['steps']} rejects "
+                f"{st['rejects']}, onsets {[round(t, 4) for t in on]} yr")
+        out["cycles"] = dict(years=cycle_years, rtol=rtol, runs=cyc)
+    fs.close()
+    return out
 
 
 if __name__ == "__main__":
