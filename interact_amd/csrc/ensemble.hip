@@ -397,12 +397,12 @@ __device__ __forceinline__ void member_patch_stage(const StageArgs& a, const Mem
       for (int f = 0; f < D; ++f) ws[f] += w * Kj[f * n + p];
     }
     const double w = a.tb.A[s + 1][s];
-    double* Yn = (s + 1 == S - 1) ? Xn : a.Yb[(s + 1) & 1];
+    const bool last = (s + 1 == S - 1);   // only the new solution is stored
     double yn[D];
 #pragma unroll
     for (int f = 0; f < D; ++f) {
       yn[f] = X[f * n + p] + ms.h * (ws[f] + w * kf[f]);
-      Yn[f * n + p] = yn[f];
+      if (last) Xn[f * n + p] = yn[f];
     }
     double Pn[PD];
     rsf_patch_pre<NC>(a.fp, cf, yn, Pn);

@@ -178,13 +178,15 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs& a, DevCtl* c, in
   if (mid) {
     // next stage state Y_{s+1} = X + h (sum_{j<s} A[s+1][j] K_j + A[s+1][s] K_s)
     // and its state-only record
+    // (only the last stage state is stored: it is the new solution; the
+    // others live on in their state-only records)
     const double w = a.tb.A[s + 1][s];
-    double* Yn = (s + 1 == S - 1) ? Xn : a.Yb[(s + 1) & 1];
+    const bool last = (s + 1 == S - 1);
     double yn[D];
 #pragma unroll
     for (int f = 0; f < D; ++f) {
       yn[f] = o.x0[f] + hstep * (o.ws[f] + w * kf[f]);
-      Yn[f * n + p] = yn[f];
+      if (last) Xn[f * n + p] = yn[f];
     }
     double Pn[PD];
     rsf_patch_pre<NC>(a.fp, cf, yn, Pn);

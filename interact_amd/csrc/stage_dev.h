@@ -523,13 +523,10 @@ __device__ __forceinline__ void prep_patch(const StageArgs& a, int p) {
   }
   if (c->rupt_update && vel_mag<NC>(a.fp, cf.a, x) >= a.so.rupture_vth) a.rupt[p] = 1;
   const double w = a.tb.A[1][0];
-  // stage 1 is never the last stage of the supported pairs (s >= 4)
-  double* Y = a.Yb[1];
+  // stage 1 is never the last stage of the supported pairs (s >= 4), so
+  // its state is not stored: its state-only record carries it
 #pragma unroll
-  for (int f = 0; f < D; ++f) {
-    y[f] = x[f] + h * (w * K0[f * n + p]);
-    Y[f * n + p] = y[f];
-  }
+  for (int f = 0; f < D; ++f) y[f] = x[f] + h * (w * K0[f * n + p]);
   double P[PreDim<NC>::v];
   rsf_patch_pre<NC>(a.fp, cf, y, P);
   store_pre<NC>(P, a.Pb[1], n, p);
