@@ -106,13 +106,12 @@ METRIC = "sim-years/hour + K·v HBM GB/s, mixed-prec vs FP64 GPU at matched even
 # values, counted in whole chunks of 512 (two-byte formats) or 256 columns;
 # the stage kernel only from 3 patches per wave up (more than 4096 patches
 # on the 256 CUs of an MI355X), below which the per-patch kernel is faster
-LDS_X_FLOATS = 36864
 PERSIST_MIN_PATCHES = 2 * 256 * 8 + 1
 
 
 def persistent(name, D):
-    chunk = 512 if name in ("bf16", "bf16nf") else 256
-    return name != "fp64" and -(-D // chunk) * chunk <= LDS_X_FLOATS
+    # every FP32-input format (x longer than the LDS copy runs in column blocks)
+    return name != "fp64"
 
 
 def stage_kernel(name, D, n):

@@ -401,7 +401,7 @@ static int matvec_t_launch(rsfg_ctx* h, const TK* K, const double* d_x, double* 
 // units against that copy (a unit is R = 2 rows of the plain formats, or
 // one patch's G rows and its near-field lo chunks for bf16nf).  Every row
 // sum is formed in the order of k_matvec / k_matvec_nf (same bits).
-template <typename TK, int R>
+template <typename TK, int R, bool CB>
 __global__ __launch_bounds__(64 * kPW, 1) void k_matvec_p(const void* __restrict__ Kv,
                                                          const bf16_t* __restrict__ lo,
                                                          const int* __restrict__ off,
@@ -411,39 +411,59 @@ __global__ __launch_bounds__(64 * kPW, 1) void k_matvec_p(const void* __restrict
                                                          double* __restrict__ y, int upw) {
   using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
   __shared__ __attribute__((aligned(16))) float xs[kXL];
-  x_to_lds<TH>(x, cols, xs);
-  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nunit = (rows + R - 1) / R;
   const int ub = min((blockIdx.x * kPW + wave) * upw, nunit), ue = min(ub + upw, nunit);
-  for (int u = ub; u < ue; ++u) {
-    double acc[R];
+  const int nch = lds_x_floats<TH>(cols) / chunk_elems<TH>();
+  // the wave's rows over the chunks [c0, c1) of x, copied into LDS; a row's
+  // block sums are added in block order by the lane that writes it
+  auto block = [&](int c0, int c1) {
+    x_to_lds<TH>(x, c0, c1, xs);
+    __syncthreads();
+    for (int u = ub; u < ue; ++u) {
+      double acc[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) acc[r] = 0.0;
-    gemv_lds_acc<TH, R, 2>(static_cast<const TH*>(Kv), ld, u * R, rows, cols, xs, acc);
-    if constexpr (std::is_same<TK, bf16nf_t>::value)
-      gemv_nf_lo_lds<R>(lo + (size_t)off[u] * R * kNfChunk, list + off[u], off[u + 1] - off[u],
-                        xs, acc);
+      for (int r = 0; r < R; ++r) acc[r] = 0.0;
+      gemv_lds_acc<TH, R, 2>(static_cast<const TH*>(Kv), ld, u * R, rows, c0, c1, xs, acc);
+      if constexpr (std::is_same<TK, bf16nf_t>::value)
+        gemv_nf_lo_lds<R>(lo + (size_t)off[u] * R * kNfChunk, list + off[u],
+                          off[u + 1] - off[u], c0, CB ? c1 : INT_MAX, xs, acc);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double tot = wave_sum(acc[r]);
-      if (lane == 0 && u * R + r < rows) y[u * R + r] = tot;
+      for (int r = 0; r < R; ++r) {
+        const double tot = wave_sum(acc[r]);
+        if (lane == 0 && u * R + r < rows) y[u * R + r] = (c0 == 0) ? tot : y[u * R + r] + tot;
+      }
     }
+  };
+  if constexpr (CB) {
+    // x longer than the LDS copy: column blocks of lds_block_chunks chunks
+    constexpr int BC = lds_block_chunks<TH>();
+    for (int c0 = 0; c0 < nch; c0 += BC) {
+      if (c0 > 0) __syncthreads();   // every wave is done with the previous block
+      block(c0, min(c0 + BC, nch));
+    }
+  } else {
+    block(0, nch);
   }
 }
 
-// Launches k_matvec_p when the format takes FP32 x, x fits the LDS copy and
-// the units spread over the CUs; returns false (nothing launched) otherwise.
+// Launches k_matvec_p when the format takes FP32 x and the units spread
+// over the CUs; returns false (nothing launched) otherwise.
 template <typename TK, int R>
 static bool matvec_persistent(rsfg_ctx* h, const void* K, const float* x, double* y) {
-  using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
-  if (lds_x_floats<TH>(h->cols) > kXL) return false;
   const char* e = getenv("RSFG_STAGE_PERSIST");
   if (e && e[0] == '0') return false;
   int nwg = 0, upw = 0;
   if (!persist_split(h->ncu, (h->rows_all + R - 1) / R, 64, &nwg, &upw)) return false;
-  hipLaunchKernelGGL((k_matvec_p<TK, R>), dim3(nwg), dim3(64 * kPW), 0, h->stream, K, h->Knf_lo,
-                     h->nf_off, h->nf_list, (long)h->ld, h->rows_all, h->cols, x, y, upw);
+  using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
+  if (lds_blocks<TH>(h->cols) > 1)
+    hipLaunchKernelGGL((k_matvec_p<TK, R, true>), dim3(nwg), dim3(64 * kPW), 0, h->stream, K,
+                       h->Knf_lo, h->nf_off, h->nf_list, (long)h->ld, h->rows_all, h->cols, x, y,
+                       upw);
+  else
+    hipLaunchKernelGGL((k_matvec_p<TK, R, false>), dim3(nwg), dim3(64 * kPW), 0, h->stream, K,
+                       h->Knf_lo, h->nf_off, h->nf_list, (long)h->ld, h->rows_all, h->cols, x, y,
+                       upw);
   return true;
 }
 

@@ -354,20 +354,37 @@ constexpr int lds_x_floats(int cols) {
   return (cols + chunk_elems<TK>() - 1) / chunk_elems<TK>() * chunk_elems<TK>();
 }
 
+// Column blocks: when x is longer than the LDS copy holds, the persistent
+// kernels take the columns in blocks of lds_block_chunks chunks, each copied
+// into LDS in turn, and add the blocks' row sums in block order (one block,
+// and the sums of the unblocked kernels, when x fits).
 template <typename TK>
-__device__ __forceinline__ void x_to_lds(const float* __restrict__ x, int cols, float* xs) {
+constexpr int lds_block_chunks() {
+  return kXL / chunk_elems<TK>();
+}
+
+template <typename TK>
+constexpr int lds_blocks(int cols) {
+  return (lds_x_floats<TK>(cols) / chunk_elems<TK>() + lds_block_chunks<TK>() - 1) /
+         lds_block_chunks<TK>();
+}
+
+// chunks [c0, c1) of x into LDS, chunk c at xs + (c - c0) CH
+template <typename TK>
+__device__ __forceinline__ void x_to_lds(const float* __restrict__ x, int c0, int c1, float* xs) {
   static_assert(std::is_same<typename KTraits<TK>::VT, float>::value, "FP32 x only");
   constexpr int CH = chunk_elems<TK>();
-  const int nslot = lds_x_floats<TK>(cols) / KTraits<TK>::VEC;   // (chunk, lane) pairs
+  const float* xb = x + (size_t)c0 * CH;
+  const int nslot = (c1 - c0) * CH / KTraits<TK>::VEC;   // (chunk, lane) pairs
   for (int i = threadIdx.x; i < nslot; i += blockDim.x) {
     if constexpr (KTraits<TK>::VEC == 8) {
       const int c = i >> 6, l = i & 63;
-      const v4f lo4 = *reinterpret_cast<const v4f*>(x + c * CH + 8 * l);
-      const v4f hi4 = *reinterpret_cast<const v4f*>(x + c * CH + 8 * l + 4);
+      const v4f lo4 = *reinterpret_cast<const v4f*>(xb + c * CH + 8 * l);
+      const v4f hi4 = *reinterpret_cast<const v4f*>(xb + c * CH + 8 * l + 4);
       *reinterpret_cast<v4f*>(xs + c * CH + 4 * l) = lo4;
       *reinterpret_cast<v4f*>(xs + c * CH + 256 + 4 * l) = hi4;
     } else {
-      *reinterpret_cast<v4f*>(xs + 4 * i) = *reinterpret_cast<const v4f*>(x + 4 * i);
+      *reinterpret_cast<v4f*>(xs + 4 * i) = *reinterpret_cast<const v4f*>(xb + 4 * i);
     }
   }
 }
@@ -385,16 +402,16 @@ __device__ __forceinline__ void lds_x_slice(const float* xs, int c, Slice<TK, R>
   }
 }
 
-// Lane partial sums of rows [row0, row0 + R) against the LDS copy of x,
-// U chunks per loop trip, added into acc (rows past nrows - 1 are clamped
-// to it; the caller discards them).
+// Lane partial sums of rows [row0, row0 + R) over the chunks [c0, c1)
+// against their LDS copy (chunk c at xs + (c - c0) CH), U chunks per loop
+// trip, added into acc (rows past nrows - 1 are clamped to it; the caller
+// discards them).
 template <typename TK, int R, int U>
 __device__ __forceinline__ void gemv_lds_acc(const TK* __restrict__ K, long ld, int row0,
-                                             int nrows, int cols, const float* xs,
+                                             int nrows, int c0, int c1, const float* xs,
                                              double* acc) {
   constexpr int CH = chunk_elems<TK>();
   const int lo = (threadIdx.x & 63) * KTraits<TK>::VEC;
-  const int nch = (cols + CH - 1) / CH;
   const TK* rp[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) rp[r] = K + (size_t)min(row0 + r, nrows - 1) * ld;
@@ -403,32 +420,39 @@ __device__ __forceinline__ void gemv_lds_acc(const TK* __restrict__ K, long ld, 
 #pragma unroll
     for (int r = 0; r < R; ++r)
       sl.k[r] = __builtin_nontemporal_load(
-          reinterpret_cast<const typename Slice<TK, R>::KV*>(rp[r] + c * CH + lo));
-    lds_x_slice<TK, R>(xs, c, sl);
+          reinterpret_cast<const typename Slice<TK, R>::KV*>(rp[r] + (size_t)c * CH + lo));
+    lds_x_slice<TK, R>(xs, c - c0, sl);
     sl.fma(acc);
   };
-  int c = 0;
-  for (; c + U <= nch; c += U) {
+  int c = c0;
+  for (; c + U <= c1; c += U) {
 #pragma unroll
     for (int u = 0; u < U; ++u) chunk(c + u);
   }
-  for (; c < nch; ++c) chunk(c);
+  for (; c < c1; ++c) chunk(c);
 }
 
 // The near-field lo chunks of one patch (G rows, nlist chunks listed in
-// list, lo block laid out as for gemv_nf_wave) against the LDS copy of x.
+// list, lo block laid out as for gemv_nf_wave) that fall in the column
+// block [c0, c1), against its LDS copy.
 template <int G>
 __device__ __forceinline__ void gemv_nf_lo_lds(const bf16_t* __restrict__ lo,
                                                const uint16_t* __restrict__ list, int nlist,
-                                               const float* xs, double* acc) {
+                                               int c0, int c1, const float* xs, double* acc) {
   const int l8 = (threadIdx.x & 63) * 8;
-  for (int k = 0; k < nlist; ++k) {
+  // the list is sorted: its entries in [c0, c1) are the run [kb, ke)
+  int kb = 0;
+  while (kb < nlist && (int)list[kb] < c0) ++kb;
+  int ke = kb;
+  while (ke < nlist && (int)list[ke] < c1) ++ke;
+#pragma unroll 2
+  for (int k = kb; k < ke; ++k) {
     Slice<bf16_t, G> sl;
 #pragma unroll
     for (int r = 0; r < G; ++r)
       sl.k[r] = __builtin_nontemporal_load(
           reinterpret_cast<const v4u*>(lo + ((size_t)k * G + r) * kNfChunk + l8));
-    lds_x_slice<bf16_t, G>(xs, (int)list[k], sl);
+    lds_x_slice<bf16_t, G>(xs, (int)list[k] - c0, sl);
     sl.fma(acc);
   }
 }

@@ -309,7 +309,7 @@ __global__ __launch_bounds__(64 * kStagePPB, RSFG_STAGE_MINW) void k_stage(Stage
 constexpr int kEW = 4;          // epilogue waves per workgroup (one per SIMD)
 constexpr int kMaxPPW = 32;     // patches per wave: kPW kMaxPPW / kEW <= 64 lanes
 
-template <typename TK, int NC, int G>
+template <typename TK, int NC, int G, bool CB>
 __global__ __launch_bounds__(64 * kPW, 1) void k_stage_p(StageArgs a, int ppw) {
   using VT = typename KTraits<TK>::VT;
   using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
@@ -338,24 +338,39 @@ __global__ __launch_bounds__(64 * kPW, 1) void k_stage_p(StageArgs a, int ppw) {
   const bool reduce = (s == 0) || (s == S - 1);
   StageOperands<NC> op;
   stage_operands<NC, true>(a, ks, sst, X, Xn, pp, own, op);
-  x_to_lds<TH>(static_cast<const float*>(a.vin), a.cols, xs);
-  __syncthreads();
-  for (int q = pb; q < pe; ++q) {
-    double acc[G];
+  // the rows of this wave's patches over the chunks [c0, c1) of x, copied
+  // into LDS; a row's block sums are added in block order
+  const int nch = lds_x_floats<TH>(a.cols) / chunk_elems<TH>();
+  auto block = [&](int c0, int c1) {
+    x_to_lds<TH>(static_cast<const float*>(a.vin), c0, c1, xs);
+    __syncthreads();
+    for (int q = pb; q < pe; ++q) {
+      double acc[G];
 #pragma unroll
-    for (int k = 0; k < G; ++k) acc[k] = 0.0;
-    gemv_lds_acc<TH, G, 2>(static_cast<const TH*>(a.K), a.ld, q * G, a.rows_all, a.cols, xs,
-                           acc);
-    if constexpr (std::is_same<TK, bf16nf_t>::value) {
-      const int o = a.nf_off[q];
-      gemv_nf_lo_lds<G>(a.Knf_lo + (size_t)o * G * kNfChunk, a.nf_list + o,
-                        a.nf_off[q + 1] - o, xs, acc);
-    }
+      for (int k = 0; k < G; ++k) acc[k] = 0.0;
+      gemv_lds_acc<TH, G, 2>(static_cast<const TH*>(a.K), a.ld, q * G, a.rows_all, c0, c1, xs,
+                             acc);
+      if constexpr (std::is_same<TK, bf16nf_t>::value) {
+        const int o = a.nf_off[q];
+        gemv_nf_lo_lds<G>(a.Knf_lo + (size_t)o * G * kNfChunk, a.nf_list + o,
+                          a.nf_off[q + 1] - o, c0, CB ? c1 : INT_MAX, xs, acc);
+      }
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
-      const double tot = wave_sum(acc[k]);
-      if (lane == 0) red[q - p0][k] = tot;
+      for (int k = 0; k < G; ++k) {
+        const double tot = wave_sum(acc[k]);
+        if (lane == 0) red[q - p0][k] = (c0 == 0) ? tot : red[q - p0][k] + tot;
+      }
     }
+  };
+  if constexpr (CB) {
+    // x longer than the LDS copy: column blocks of lds_block_chunks chunks
+    constexpr int BC = lds_block_chunks<TH>();
+    for (int c0 = 0; c0 < nch; c0 += BC) {
+      if (c0 > 0) __syncthreads();   // every wave is done with the previous block
+      block(c0, min(c0 + BC, nch));
+    }
+  } else {
+    block(0, nch);
   }
   __syncthreads();
   if (wave >= kEW) return;
@@ -619,11 +634,10 @@ template <typename TK>
 static void persist_geometry(int ncu, int n, int cols, bool allow, int* nwg, int* ppw) {
   *nwg = 0;
   *ppw = 0;
-  using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
   if constexpr (std::is_same<typename KTraits<TK>::VT, float>::value) {
     const char* e = getenv("RSFG_STAGE_PERSIST");
     if (!allow || (e && e[0] == '0')) return;
-    if (lds_x_floats<TH>(cols) > kXL) return;
+    (void)cols;   // x longer than the LDS copy runs in column blocks
     const bool force = e && e[0] == '1';
     if (!persist_split(ncu, n, kMaxPPW, nwg, ppw) || (!force && *ppw < kMinPPW)) *nwg = *ppw = 0;
   }
@@ -632,9 +646,12 @@ static void persist_geometry(int ncu, int n, int cols, bool allow, int* nwg, int
 template <typename TK, int NC, int G>
 struct LauncherT : Launcher {
   using VT = typename KTraits<TK>::VT;
+  using TH = typename std::conditional<std::is_same<TK, bf16nf_t>::value, bf16_t, TK>::type;
   int nwg_p = 0, ppw_p = 0, n_ = 0;
+  bool blocks_ = false;   // x longer than the LDS copy: column blocks
   LauncherT(int ncu, int n, int cols, bool allow_persist) : n_(n) {
     persist_geometry<TK>(ncu, n, cols, allow_persist, &nwg_p, &ppw_p);
+    blocks_ = lds_blocks<TH>(cols) > 1;
   }
   void init_vel(const StageArgs& a, hipStream_t st) override {
     hipLaunchKernelGGL((k_init_vel<VT, NC>), dim3((a.n + 255) / 256), dim3(256), 0, st, a);
@@ -645,7 +662,12 @@ struct LauncherT : Launcher {
   void stage(const StageArgs& a, hipStream_t st) override {
     if constexpr (std::is_same<VT, float>::value) {
       if (nwg_p > 0) {
-        hipLaunchKernelGGL((k_stage_p<TK, NC, G>), dim3(nwg_p), dim3(64 * kPW), 0, st, a, ppw_p);
+        if (blocks_)
+          hipLaunchKernelGGL((k_stage_p<TK, NC, G, true>), dim3(nwg_p), dim3(64 * kPW), 0, st,
+                             a, ppw_p);
+        else
+          hipLaunchKernelGGL((k_stage_p<TK, NC, G, false>), dim3(nwg_p), dim3(64 * kPW), 0, st,
+                             a, ppw_p);
         return;
       }
     }

@@ -157,3 +157,45 @@ def test_scalar_layout_one_row_per_patch():
     for prec in (lib.PREC_FP32, lib.PREC_BF16):
         _compare(fs, X0, 2.0 * SEC_PER_YEAR, prec)
     fs.close()
+
+
+def test_column_blocks_beyond_the_lds_copy():
+    """x longer than the LDS copy (36864 floats): a 100 x 48 km fault at
+    0.5 km, two slip components, 38400 columns in two column blocks.  The
+    persistent kernels add each row's two block sums; the per-patch kernels
+    sum the whole row in one go, so the products agree to FP64 rounding
+    (not bit for bit), and 0.5 yr of the cycle takes the same steps with
+    states equal to 1e-10."""
+    from interact_amd import faults, lib
+    from interact_amd.model import FaultSystem
+    g = faults.planar_grid(0.5, 100.0, 48.0)
+    n = len(g["l"])
+    assert 2 * n > 36864
+    f = faults.cycle_fields(g)
+    store = lib.PREC_FP32 | lib.PREC_BF16X2 | lib.PREC_BF16 | lib.PREC_BF16NF
+    fs = FaultSystem(g)
+    fs.assemble(lib.SLIP_VECTOR, shear_modulus=f["shear_modulus"], store=store)
+    i = np.arange(2 * n)
+    x = np.sin(i + 1.0) + 0.3 * np.cos(3.0 * i)
+    y64 = fs.matvec(x, lib.PREC_FP64)
+    for name in ("fp32", "bf16x2", "bf16", "bf16nf"):
+        p = lib.PREC_BY_NAME[name]
+        y1 = _with_persist("1", lambda: fs.matvec(x, p))
+        y0 = _with_persist("0", lambda: fs.matvec(x, p))
+        assert np.abs(y1 - y0).max() <= 1e-13 * np.abs(y0).max(), name
+        err = np.linalg.norm(y1 - y64) / np.linalg.norm(y64)
+        assert err < {"fp32": 1e-7, "bf16x2": 5e-6, "bf16": 5e-3, "bf16nf": 5e-5}[name], name
+    fs.set_friction(f["a"], f["b"], f["dc"], v0=f["v0"], f0=f["f0"], eta=f["eta"])
+    fs.set_loading(f["vpl"])
+    X0 = initial_state(f["tau"], f["sigma"], f["vel"], f["a"], f["v0"], ncomp=2)
+    out = {}
+    for flag in ("1", "0"):
+        fs.set_state(X0)
+        st = _with_persist(flag, lambda: fs.solve(0.5 * SEC_PER_YEAR, rtol=1e-8,
+                                                  prec=lib.PREC_BF16NF))
+        out[flag] = (st["steps"], st["rejects"], fs.get_state()[0])
+    assert out["1"][:2] == out["0"][:2]
+    X1, X0s = out["1"][2].reshape(n, -1), out["0"][2].reshape(n, -1)
+    for k in range(X1.shape[1]):    # each field against its own scale
+        assert np.abs(X1[:, k] - X0s[:, k]).max() <= 1e-10 * np.abs(X0s[:, k]).max() + 1e-300
+    fs.close()
