@@ -160,6 +160,9 @@ def test_rough_mesh_full_size_vertical_edge_tolerance():
             else:
                 assert err.max() <= 1e-5
         assert np.array_equal(out[0.0][:, off_vertical >= 3e-3], out[3e-3][:, off_vertical >= 3e-3])
+        from interact_amd import lib
+        with pytest.raises(lib.RsfgError, match="below 0.1"):
+            fs.set_triangle_tolerance(0.2)
     finally:
         oracle.set_tri_vertical_tol(0.0)
         fs.close()
