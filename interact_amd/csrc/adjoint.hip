@@ -443,7 +443,7 @@ __global__ __launch_bounds__(256) void k_adj_kt(const TK* __restrict__ K, long l
     vec_t kv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      kv[u] = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(K + (size_t)(i + u) * ld + c0));
+      kv[u] = RSFG_KLOAD(reinterpret_cast<const vec_t*>(K + (size_t)(i + u) * ld + c0));
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const double wi = w[i + u];
@@ -452,7 +452,7 @@ __global__ __launch_bounds__(256) void k_adj_kt(const TK* __restrict__ K, long l
     }
   }
   for (; i < i1; ++i) {
-    const vec_t kv = __builtin_nontemporal_load(reinterpret_cast<const vec_t*>(K + (size_t)i * ld + c0));
+    const vec_t kv = RSFG_KLOAD(reinterpret_cast<const vec_t*>(K + (size_t)i * ld + c0));
     const double wi = w[i];
 #pragma unroll
     for (int e = 0; e < V; ++e) acc[e] = fma((double)kv[e], wi, acc[e]);

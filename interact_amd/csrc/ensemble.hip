@@ -195,7 +195,7 @@ __device__ __forceinline__ void wave_product(const float* __restrict__ Kt, long 
     for (int u = 0; u < U; ++u) {
 #pragma unroll
       for (int rf = 0; rf < RF; ++rf)
-        a[u][rf] = __builtin_nontemporal_load(
+        a[u][rf] = RSFG_KLOAD(
             reinterpret_cast<gv4*>(ka[rf] + (size_t)(kc + u) * 256));
 #pragma unroll
       for (int mf = 0; mf < MF; ++mf)
@@ -296,7 +296,7 @@ __device__ __forceinline__ void unit_product_slab(const float* __restrict__ Kt, 
     for (int u = 0; u < U; ++u)
 #pragma unroll
       for (int rf = 0; rf < RF; ++rf)
-        a[u][rf] = __builtin_nontemporal_load(
+        a[u][rf] = RSFG_KLOAD(
             reinterpret_cast<gv4*>(ka[rf] + (size_t)(kc + u) * 256));
   };
   vload(kc0);

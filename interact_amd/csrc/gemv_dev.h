@@ -71,7 +71,7 @@ struct ChunkFma<double, R> {
     const v2d xv = *reinterpret_cast<const v2d*>(x + col);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const v2d kv = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(rp[r] + col));
+      const v2d kv = RSFG_KLOAD(reinterpret_cast<const v2d*>(rp[r] + col));
       acc[r] = fma(kv.x, xv.x, acc[r]);
       acc[r] = fma(kv.y, xv.y, acc[r]);
     }
@@ -85,7 +85,7 @@ struct ChunkFma<float, R> {
     const v4f xv = *reinterpret_cast<const v4f*>(x + col);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const v4f kv = __builtin_nontemporal_load(reinterpret_cast<const v4f*>(rp[r] + col));
+      const v4f kv = RSFG_KLOAD(reinterpret_cast<const v4f*>(rp[r] + col));
       float t = kv.x * xv.x;
       t = fmaf(kv.y, xv.y, t);
       t = fmaf(kv.z, xv.z, t);
@@ -102,7 +102,7 @@ struct ChunkFma<bf16x2_t, R> {
     const v4f xv = *reinterpret_cast<const v4f*>(x + col);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const v4u kv = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(rp[r] + col));
+      const v4u kv = RSFG_KLOAD(reinterpret_cast<const v4u*>(rp[r] + col));
       float t = bf_hi(kv.x) * xv.x;
       t = fmaf(bf_lo(kv.x), xv.x, t);
       t = fmaf(bf_hi(kv.y), xv.y, t);
@@ -124,7 +124,7 @@ struct ChunkFma<bf16_t, R> {
     const v4f x1 = *reinterpret_cast<const v4f*>(x + col + 4);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const v4u kv = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(rp[r] + col));
+      const v4u kv = RSFG_KLOAD(reinterpret_cast<const v4u*>(rp[r] + col));
       float t = bf_lo(kv.x) * x0.x;
       t = fmaf(bf_hi(kv.x), x0.y, t);
       t = fmaf(bf_lo(kv.y), x0.z, t);
@@ -308,7 +308,7 @@ __device__ __forceinline__ void gemv_nf_wave(const bf16_t* __restrict__ Khi,
     const v4f x1 = *reinterpret_cast<const v4f*>(x + col + 4);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const v4u kv = __builtin_nontemporal_load(
+      const v4u kv = RSFG_KLOAD(
           reinterpret_cast<const v4u*>(lo + ((size_t)k * R + r) * kNfChunk + l8));
       float t = bf_lo(kv.x) * x0.x;
       t = fmaf(bf_hi(kv.x), x0.y, t);
@@ -419,7 +419,7 @@ __device__ __forceinline__ void gemv_lds_acc(const TK* __restrict__ K, long ld, 
     Slice<TK, R> sl;
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      sl.k[r] = __builtin_nontemporal_load(
+      sl.k[r] = RSFG_KLOAD(
           reinterpret_cast<const typename Slice<TK, R>::KV*>(rp[r] + (size_t)c * CH + lo));
     lds_x_slice<TK, R>(xs, c - c0, sl);
     sl.fma(acc);
@@ -450,7 +450,7 @@ __device__ __forceinline__ void gemv_nf_lo_lds(const bf16_t* __restrict__ lo,
     Slice<bf16_t, G> sl;
 #pragma unroll
     for (int r = 0; r < G; ++r)
-      sl.k[r] = __builtin_nontemporal_load(
+      sl.k[r] = RSFG_KLOAD(
           reinterpret_cast<const v4u*>(lo + ((size_t)k * G + r) * kNfChunk + l8));
     lds_x_slice<bf16_t, G>(xs, (int)list[k] - c0, sl);
     sl.fma(acc);

@@ -29,6 +29,17 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
     }                               \
   } while (0)
 
+// Loads of operator entries: streamed once per product, so marked
+// nontemporal; RSFG_TEMPORAL_K builds them as ordinary loads (a build for
+// operators small enough to stay in the last-level cache between launches)
+#ifndef RSFG_KLOAD
+#ifdef RSFG_TEMPORAL_K
+#define RSFG_KLOAD(...) (*(__VA_ARGS__))
+#else
+#define RSFG_KLOAD(...) __builtin_nontemporal_load(__VA_ARGS__)
+#endif
+#endif
+
 // storage element of the bf16 hi+lo pair: hi in the upper 16 bits
 typedef uint32_t bf16x2_t;
 typedef uint16_t bf16_t;
