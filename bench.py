@@ -101,9 +101,8 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 METRIC = "sim-years/hour + K·v HBM GB/s, mixed-prec vs FP64 GPU at matched event times"
 
 
-# the persistent kernels (x copied into LDS, one workgroup per CU) run for
-# the FP32-input formats while x fits in their 144 KiB LDS copy: 36864 FP32
-# values, counted in whole chunks of 512 (two-byte formats) or 256 columns;
+# the persistent kernels (x copied into LDS, one workgroup per CU; x longer
+# than the 144 KiB LDS copy in column blocks) run for the FP32-input formats;
 # the stage kernel only from 3 patches per wave up (more than 4096 patches
 # on the 256 CUs of an MI355X), below which the per-patch kernel is faster
 PERSIST_MIN_PATCHES = 2 * 256 * 8 + 1

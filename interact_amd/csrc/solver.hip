@@ -621,13 +621,13 @@ struct Launcher {
 
 // Persistent stage kernel geometry for n patches and cols columns: ppw
 // patches per wave and the workgroup count, or nwg = 0 where the per-patch
-// kernel runs: FP64 entries, x larger than the LDS budget, more than
-// kMaxPPW patches per wave, fewer than kMinPPW (the persistent kernel's
-// fixed cost, about 11 us per launch of x copy, operand requests and the
-// epilogue after the last stream, then outweighs its faster stream:
-// measured 33 against 31 us at 2 patches per wave, 318 against 361 us at
-// 8).  RSFG_STAGE_PERSIST=0 in the environment selects the per-patch
-// kernel everywhere, =1 the persistent kernel wherever it can run.
+// kernel runs: FP64 entries, more than kMaxPPW patches per wave, fewer than
+// kMinPPW (the persistent kernel's fixed cost, about 11 us per launch of x
+// copy, operand requests and the epilogue after the last stream, then
+// outweighs its faster stream: measured 33 against 31 us at 2 patches per
+// wave, 318 against 361 us at 8).  x longer than the LDS copy runs in
+// column blocks.  RSFG_STAGE_PERSIST=0 in the environment selects the
+// per-patch kernel everywhere, =1 the persistent kernel wherever it can run.
 constexpr int kMinPPW = 3;
 
 template <typename TK>
