@@ -378,6 +378,16 @@ def main():
                         bytes=bytes_alg)
         log(f"K.v {name}: relerr {kv[name]['relerr']:.2e}  matvec {mv_ms*1e3:.1f} us "
             f"{kv[name]['matvec_gbs']:.0f} GB/s  stage {st_ms*1e3:.1f} us {kv[name]['stage_gbs']:.0f} GB/s")
+    # what a plain read of 4 GiB reaches on this device with the products'
+    # load pattern and no arithmetic (rsfg_read_bench), in the same process
+    rd_bytes = 4 << 30
+    rd_ms = float(np.median([fs.read_bench(rd_bytes, 20) for _ in range(3)]))
+    read_ceil = dict(gbs=rd_bytes / rd_ms / 1e6, bytes=rd_bytes, launch_us=rd_ms * 1e3,
+                     frac=rd_bytes / rd_ms / 1e6 / HBM_PEAK_GBS,
+                     kernel="k_read_stream (16-byte nontemporal loads, 8 in flight per lane, "
+                            "2 workgroups of 8 waves per CU)")
+    log(f"read stream: {rd_bytes} bytes in {rd_ms*1e3:.1f} us = {read_ceil['gbs']:.0f} GB/s "
+        f"({read_ceil['frac']:.3f} of {HBM_PEAK_GBS:.0f})")
 
     t_end = args.years * SEC_PER_YEAR
     seg = t_end / args.steps
@@ -436,7 +446,8 @@ def main():
     roof = dict(kernel=stage_kernel(hname, D, n), bound="hbm",
                 achieved=stg["stage_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=stg["stage_gbs"] / HBM_PEAK_GBS, traffic=traffic,
-                bytes_per_launch=stg["bytes"], launch_us=stg["stage_ms"] * 1e3)
+                bytes_per_launch=stg["bytes"], launch_us=stg["stage_ms"] * 1e3,
+                frac_of_read_stream=stg["stage_gbs"] / read_ceil["gbs"])
     # the FP32 product K v on its own (the storage the north-star K v figure
     # names), HIP events on the handle's stream
     k32 = kv["fp32"]
@@ -444,7 +455,8 @@ def main():
                      achieved=k32["matvec_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
                      frac=k32["matvec_gbs"] / HBM_PEAK_GBS,
                      traffic=MATVEC_TRAFFIC[args.config].get("fp32"),
-                     bytes_per_launch=k32["bytes"], launch_us=k32["matvec_ms"] * 1e3)
+                     bytes_per_launch=k32["bytes"], launch_us=k32["matvec_ms"] * 1e3,
+                     frac_of_read_stream=k32["matvec_gbs"] / read_ceil["gbs"])
     out = {
         "metric": METRIC,
         "value": head["sim_years_per_hour"],
@@ -475,6 +487,7 @@ def main():
                          frac_fp64_peak=pairs / asm_ms * 1e3 * F_PAIR / 1e12 / FP64_PEAK_TFLOPS),
         "roofline": roof,
         "roofline_kv_fp32": roof_kv32,
+        "hbm_read_stream": read_ceil,
     }
     if not args.no_cpu and ref is not None:
         K64 = fs.dense()

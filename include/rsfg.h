@@ -203,6 +203,15 @@ int rsfg_matvec_t(rsfg_handle *h, int prec, const double *x, double *y);
    mean kernel time [ms] measured with HIP events on the handle's stream */
 int rsfg_matvec_bench(rsfg_handle *h, int prec, int nrep, double *ms);
 
+/* Average time [ms] of nrep launches of a plain read stream over a fresh
+   device buffer of `bytes` (1 MiB to 64 GiB) on the handle's device and
+   stream: 16-byte nontemporal loads, eight in flight per lane, two
+   workgroups of 8 waves per CU, nothing written.  No reference
+   counterpart: the read rate the device reaches with the products' load
+   pattern and no arithmetic, beside which their HBM rates can be read.
+   Allocates and frees the buffer. */
+int rsfg_read_bench(rsfg_handle *h, double bytes, int nrep, double *ms);
+
 /* ---- rate-and-state problem -------------------------------------------
    Parameters of rsf_get_settings (src/rsf_init.c:273-503) that the RHS
    uses; per-patch a, b (read_rsf, src/interact/input.c:83) and D_c

@@ -1171,6 +1171,62 @@ int rsfg_matvec_bench(rsfg_handle* h, int prec, int nrep, double* ms) {
   return RSFG_OK;
 }
 
+// Plain read stream: each wave reads a contiguous span with 16-byte
+// nontemporal loads, eight in flight, and sums them (stored only if the sum
+// hits an impossible value, so the loads stay).
+__global__ __launch_bounds__(512) void k_read_stream(const v4f* __restrict__ a, size_t n4,
+                                                    size_t per_wave, float* out) {
+  constexpr int U = 8;
+  const size_t wave = (size_t)blockIdx.x * 8 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const size_t b = min(wave * per_wave, n4), e = min(b + per_wave, n4);
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  size_t i = b + lane;
+  for (; i + 64 * (U - 1) < e; i += 64 * U) {
+    v4f t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) t[u] = __builtin_nontemporal_load(a + i + 64 * u);
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += t[u];
+  }
+  for (; i < e; i += 64) acc += __builtin_nontemporal_load(a + i);
+  if (acc.x + acc.y + acc.z + acc.w == -1.0f) out[lane] = acc.x;
+}
+
+int rsfg_read_bench(rsfg_handle* h, double bytes, int nrep, double* ms) {
+  RSFG_CHECK(h && nrep > 0 && ms && bytes >= 1 << 20 && bytes <= 64.0 * (1ull << 30),
+             RSFG_EINVAL, "rsfg_read_bench: bad args (1 MiB <= bytes <= 64 GiB)");
+  RSFG_HIP(hipSetDevice(h->device));
+  const size_t n4 = (size_t)(bytes / 16.0);
+  void* buf = nullptr;
+  RSFG_HIP(hipMalloc(&buf, n4 * 16 + 256));
+  float* out = reinterpret_cast<float*>(static_cast<char*>(buf) + n4 * 16);
+  const int ncu = h->ncu > 0 ? h->ncu : 256;
+  const int nwg = 2 * ncu;
+  const size_t waves = (size_t)nwg * 8, per = (n4 + waves - 1) / waves;
+  hipError_t e = hipMemsetAsync(buf, 0, n4 * 16 + 256, h->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_read_stream, dim3(nwg), dim3(512), 0, h->stream,
+                       static_cast<const v4f*>(buf), n4, per, out);   // warm
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(h->ev0, h->stream);
+  for (int k = 0; k < nrep && e == hipSuccess; ++k) {
+    hipLaunchKernelGGL(k_read_stream, dim3(nwg), dim3(512), 0, h->stream,
+                       static_cast<const v4f*>(buf), n4, per, out);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipEventRecord(h->ev1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(h->ev1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, h->ev0, h->ev1);
+  const hipError_t ef = hipFree(buf);
+  if (e != hipSuccess) return hip_fail(e, "rsfg_read_bench", __FILE__, __LINE__);
+  if (ef != hipSuccess) return hip_fail(ef, "rsfg_read_bench: free", __FILE__, __LINE__);
+  *ms = t / nrep;
+  return RSFG_OK;
+}
+
 int rsfg_set_friction(rsfg_handle* h, const rsfg_friction* f, const double* a, const double* b,
                       const double* dc) {
   RSFG_CHECK(h && f && a && b && dc, RSFG_EINVAL, "rsfg_set_friction: null argument");

@@ -33,7 +33,7 @@ EXPORTS = [
     "rsfg_operator_dims", "rsfg_operator_bytes", "rsfg_entries", "rsfg_traction_pairs",
     "rsfg_set_triangle_tolerance",
     "rsfg_matvec", "rsfg_matvec_dev", "rsfg_matvec_dev_noalloc", "rsfg_matvec_t",
-    "rsfg_matvec_bench",
+    "rsfg_matvec_bench", "rsfg_read_bench",
     "rsfg_set_friction", "rsfg_set_loading", "rsfg_rhs",
     "rsfg_solver_defaults", "rsfg_set_state", "rsfg_get_state", "rsfg_solve",
     "rsfg_solve_continue",
@@ -153,6 +153,7 @@ def load(path=None):
     L.rsfg_entries.argtypes = [h, C.c_int, _ip, C.c_int, _ip, _dp]
     L.rsfg_traction_pairs.argtypes = [h, C.c_long, _ip, _ip, C.c_int, _dp]
     L.rsfg_set_triangle_tolerance.argtypes = [h, C.c_double]
+    L.rsfg_read_bench.argtypes = [h, C.c_double, C.c_int, C.POINTER(C.c_double)]
     L.rsfg_matvec.argtypes = [h, C.c_int, _dp, _dp]
     L.rsfg_matvec_dev.argtypes = [h, C.c_int, C.c_void_p, C.c_void_p]
     L.rsfg_matvec_dev_noalloc.argtypes = [h, C.c_int, C.c_void_p, C.c_void_p]

@@ -158,6 +158,14 @@ class FaultSystem:
                 "rsfg_matvec_bench")
         return ms.value
 
+    def read_bench(self, nbytes=4 << 30, nrep=20):
+        """ms per launch of a plain read stream over nbytes on this device
+        (the products' load pattern without arithmetic)"""
+        ms = C.c_double()
+        L.check(self._lib.rsfg_read_bench(self.h, float(nbytes), nrep, C.byref(ms)),
+                "rsfg_read_bench")
+        return ms.value
+
     # ------------------------------------------------------------ physics
     def set_friction(self, a, b, dc, *, law=1, v0=1e-6, f0=0.6, eta=None,
                      shear_modulus=3.204e10, s_wave_speed=3464.0, rd_fac=1.0,

@@ -545,6 +545,19 @@ def test_bf16_nearfield_format_bytes_and_limits():
     fs.close()
 
 
+def test_read_stream_rate():
+    """rsfg_read_bench (the plain read stream the bench reports beside the
+    products' HBM rates) reaches a plausible rate; bad sizes are refused"""
+    from interact_amd import faults, lib
+    fs = _fs(faults.planar_grid(2.0))
+    ms = fs.read_bench(1 << 30, 10)
+    gbs = (1 << 30) / ms / 1e6
+    assert 2000.0 < gbs < 8400.0, gbs
+    with pytest.raises(lib.RsfgError, match="bad args"):
+        fs.read_bench(1000, 1)
+    fs.close()
+
+
 # ------------------------------------------------------------------ full size
 def test_full_size_operator_properties():
     """8000-DOF operator (1 km mesh, two slip components): along-strike
