@@ -199,3 +199,33 @@ def test_column_blocks_beyond_the_lds_copy():
     for k in range(X1.shape[1]):    # each field against its own scale
         assert np.abs(X1[:, k] - X0s[:, k]).max() <= 1e-10 * np.abs(X0s[:, k]).max() + 1e-300
     fs.close()
+
+
+@pytest.mark.parametrize("layout", ["strike_40000", "vector_normal_19200"])
+def test_column_blocks_other_layouts(layout):
+    """Column blocks with one row per patch (strike slip, 40000 patches:
+    40000 columns) and with three (two components and the normal row,
+    19200 patches: 38400 columns): persistent products against the
+    per-patch kernels to FP64 rounding in every FP32-input format."""
+    from interact_amd import faults, lib
+    from interact_amd.model import FaultSystem
+    store = lib.PREC_FP32 | lib.PREC_BF16X2 | lib.PREC_BF16 | lib.PREC_BF16NF
+    if layout == "strike_40000":
+        g = faults.planar_grid(0.5, 100.0, 100.0)
+        mode, normal = lib.SLIP_STRIKE, False
+    else:
+        g = faults.planar_grid(0.5, 100.0, 48.0)
+        mode, normal = lib.SLIP_VECTOR, True
+    fs = FaultSystem(g)
+    fs.assemble(mode, shear_modulus=G_BP5, with_normal=normal, store=store)
+    rows, cols, _ = fs.dims()
+    assert cols > 36864
+    i = np.arange(cols)
+    x = np.sin(i + 1.0) + 0.3 * np.cos(3.0 * i)
+    for name in ("fp32", "bf16x2", "bf16", "bf16nf"):
+        p = lib.PREC_BY_NAME[name]
+        y1 = _with_persist("1", lambda: fs.matvec(x, p))
+        y0 = _with_persist("0", lambda: fs.matvec(x, p))
+        assert y1.shape == y0.shape
+        assert np.abs(y1 - y0).max() <= 1e-13 * np.abs(y0).max(), (layout, name)
+    fs.close()
