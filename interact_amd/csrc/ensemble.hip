@@ -52,6 +52,7 @@ struct EnsGeo {        // one geometry of the ensemble
 
 constexpr int kEpiPatches = 16;   // patches per epilogue tile (16 G operator rows)
 constexpr int kEpiWaves = 4;      // epilogue waves per SIMD the register budget allows (no spills)
+constexpr int kSumGroup = 4;      // partials of a row loaded together by the epilogue
 constexpr int kRowPad = 256;      // rows_pad granularity (largest product row group)
 constexpr int kColPad = 256;      // ld granularity (the product's FP64 flush segment)
 
@@ -547,20 +548,44 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
   r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
   r9[RED_DX2] = 0.0;
   if (p < gg.n && !snap[ml].halted) {
+    // the rows' partials over the pieces (stream-K: the tile lies in one
+    // row group, whose pieces are tile.z .. + tile.w) or the column blocks,
+    // added in order; the loads of a group of kSumGroup partials of every
+    // row are issued together (the sum is a chain of dependent loads
+    // otherwise)
+    const double* q[G];
+    size_t stride;
+    int cnt;
+    if (e.sk_tr > 0) {
+      stride = (size_t)e.sk_tr * NM;
+      cnt = tile.w;
+#pragma unroll
+      for (int k = 0; k < G; ++k)
+        q[k] = e.Psk + (size_t)tile.z * stride + (size_t)((p * G + k) % e.sk_tr) * NM + m;
+    } else {
+      stride = cbs;
+      cnt = ncb;
+#pragma unroll
+      for (int k = 0; k < G; ++k) q[k] = gg.P + (size_t)(p * G + k) * NM + m;
+    }
     double td[G];
 #pragma unroll
-    for (int k = 0; k < G; ++k) {
-      double v = 0.0;
-      if (e.sk_tr > 0) {
-        // the tile lies in one row group, whose pieces are tile.z .. + tile.w
-        const size_t ps = (size_t)e.sk_tr * NM;
-        const double* q = e.Psk + (size_t)tile.z * ps + (size_t)((p * G + k) % e.sk_tr) * NM + m;
-        for (int c = 0; c < tile.w; ++c) v += q[c * ps];
-      } else {
-        const double* q = gg.P + (size_t)(p * G + k) * NM + m;
-        for (int cb = 0; cb < ncb; ++cb) v += q[cb * cbs];
-      }
-      td[k] = v;
+    for (int k = 0; k < G; ++k) td[k] = 0.0;
+    int c = 0;
+    for (; c + kSumGroup <= cnt; c += kSumGroup) {
+      double t[G][kSumGroup];
+#pragma unroll
+      for (int k = 0; k < G; ++k)
+#pragma unroll
+        for (int u = 0; u < kSumGroup; ++u) t[k][u] = q[k][(size_t)(c + u) * stride];
+#pragma unroll
+      for (int k = 0; k < G; ++k)
+#pragma unroll
+        for (int u = 0; u < kSumGroup; ++u) td[k] += t[k][u];
+    }
+    for (; c < cnt; ++c) {
+#pragma unroll
+      for (int k = 0; k < G; ++k) td[k] += q[k][(size_t)c * stride];
     }
     member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[ml], s, p, td,
                               gg.V[(s + 1) & 1] + (size_t)m * gg.ld, r9);
