@@ -349,12 +349,62 @@ __device__ __forceinline__ void unit_product_slab(const float* __restrict__ Kt, 
   }
 }
 
+// Operands of the stage s update of patch p of one member, requested
+// before its partials are summed (so their loads overlap those): friction
+// coefficients, loading rates, the state-only record (s > 0), the state X
+// and the tableau-weighted sum of the earlier stage derivatives (middle and
+// final stages: sum_{j<s} A[s+1][j] K_j, or sum_{j<S-1} bh[j] K_j).  The
+// reducing stages' state and monitor snapshot are read later (registers).
+template <int NC>
+struct MemberOps {
+  static constexpr int D = Dim<NC>::v, PD = PreDim<NC>::v;
+  PatchCoef cf;
+  double P[PD], sn[NC], snn, x0[D], ws[D];
+};
+
+template <int NC>
+__device__ __forceinline__ void member_operands(const StageArgs& a, const MemberSnap& ms, int s,
+                                                int p, MemberOps<NC>& o) {
+  constexpr int D = Dim<NC>::v;
+  constexpr int PD = PreDim<NC>::v;
+  const int n = a.n, S = a.tb.s;
+  const double* X = ms.xc ? a.Xb[1] : a.Xb[0];
+  const size_t sst = a.Stg[1] - a.Stg[0];
+  const bool mid = (s > 0) && (s < S - 1);
+  o.cf = load_coef(a.coef, n, p);
+#pragma unroll
+  for (int k = 0; k < NC; ++k) o.sn[k] = a.sinc[NC * p + k];
+  o.snn = a.sinc[NC * n + p];
+#pragma unroll
+  for (int f = 0; f < D; ++f) o.x0[f] = o.ws[f] = 0.0;
+#pragma unroll
+  for (int k = 0; k < PD; ++k) o.P[k] = 0.0;
+  if (s > 0) {
+    const double* Ps = a.Pb[s & 1];
+#pragma unroll
+    for (int k = 0; k < PD; ++k) o.P[k] = Ps[k * n + p];
+#pragma unroll
+    for (int f = 0; f < D; ++f) o.x0[f] = X[f * n + p];
+    const int nj = mid ? s : S - 1;
+#pragma unroll
+    for (int j = 0; j < kMaxStages - 1; ++j) {
+      const double w = (j < nj) ? (mid ? a.tb.A[s + 1][j] : a.tb.bh[j]) : 0.0;
+      if (w != 0.0) {
+        const double* Kj = a.Stg[0] + (size_t)ms.ks[j] * sst;
+#pragma unroll
+        for (int f = 0; f < D; ++f) o.ws[f] += w * Kj[f * n + p];
+      }
+    }
+  }
+}
+
 // Stage s update of patch p of one member from its elastic stressing rates
-// td (G rows): the epilogue of k_stage, written for one (patch, member)
-// pair with its operands fetched here.
+// td (G rows) and its operands o: the epilogue of k_stage, written for one
+// (patch, member) pair.
 template <int NC, int G>
 __device__ __forceinline__ void member_patch_stage(const StageArgs& a, const MemberSnap& ms,
-                                                   int s, int p, const double* td, float* vnext,
+                                                   int s, int p, const double* td,
+                                                   const MemberOps<NC>& o, float* vnext,
                                                    double* r9) {
   constexpr int D = Dim<NC>::v;
   constexpr int PD = PreDim<NC>::v;
@@ -364,65 +414,45 @@ __device__ __forceinline__ void member_patch_stage(const StageArgs& a, const Mem
   const size_t sst = a.Stg[1] - a.Stg[0];
   const bool reduce = (s == 0) || (s == S - 1);
   const bool mid = (s > 0) && (s < S - 1);
-  const PatchCoef cf = load_coef(a.coef, n, p);
-  double P[PD], y[D], kf[D], sn[NC];
+  double P[PD], kf[D], y[D];
   if (reduce) {
     const double* Yc = (s == 0) ? X : Xn;
 #pragma unroll
     for (int f = 0; f < D; ++f) y[f] = Yc[f * n + p];
   }
   if (s == 0) {
-    rsf_patch_pre<NC>(a.fp, cf, y, P);
+    rsf_patch_pre<NC>(a.fp, o.cf, y, P);
   } else {
-    const double* Ps = a.Pb[s & 1];
 #pragma unroll
-    for (int k = 0; k < PD; ++k) P[k] = Ps[k * n + p];
+    for (int k = 0; k < PD; ++k) P[k] = o.P[k];
   }
-#pragma unroll
-  for (int k = 0; k < NC; ++k) sn[k] = a.sinc[NC * p + k];
-  const double snn = a.sinc[NC * n + p];
   const double sdn = (G > NC) ? td[NC] : 0.0;
-  rsf_patch_finish<NC>(a.fp, P, td, sdn, sn, snn, kf);
+  rsf_patch_finish<NC>(a.fp, P, td, sdn, o.sn, o.snn, kf);
   double* Ks = a.Stg[0] + (size_t)ms.ks[s] * sst;
 #pragma unroll
   for (int f = 0; f < D; ++f) Ks[f * n + p] = kf[f];
   if (mid) {
-    double ws[D];
-#pragma unroll
-    for (int f = 0; f < D; ++f) ws[f] = 0.0;
-    for (int j = 0; j < s; ++j) {
-      const double w = a.tb.A[s + 1][j];
-      if (w == 0.0) continue;
-      const double* Kj = a.Stg[0] + (size_t)ms.ks[j] * sst;
-#pragma unroll
-      for (int f = 0; f < D; ++f) ws[f] += w * Kj[f * n + p];
-    }
     const double w = a.tb.A[s + 1][s];
     const bool last = (s + 1 == S - 1);   // only the new solution is stored
     double yn[D];
 #pragma unroll
     for (int f = 0; f < D; ++f) {
-      yn[f] = X[f * n + p] + ms.h * (ws[f] + w * kf[f]);
+      yn[f] = o.x0[f] + ms.h * (o.ws[f] + w * kf[f]);
       if (last) Xn[f * n + p] = yn[f];
     }
     double Pn[PD];
-    rsf_patch_pre<NC>(a.fp, cf, yn, Pn);
+    rsf_patch_pre<NC>(a.fp, o.cf, yn, Pn);
     store_pre<NC>(Pn, a.Pb[(s + 1) & 1], n, p);
     write_vel_pre<float, NC>(Pn, vnext, p, a.vpl);
-    if (!rsf_patch_ok<NC>(a.fp, cf, yn)) atomicOr(&a.ctl->stage_fail, 1);
+    if (!rsf_patch_ok<NC>(a.fp, o.cf, yn)) atomicOr(&a.ctl->stage_fail, 1);
     return;
   }
   if (s == S - 1) {
     double en = 0.0;
 #pragma unroll
     for (int f = 0; f < D; ++f) {
-      double e = 0.0;
-      for (int j = 0; j < S - 1; ++j) {
-        const double w = a.tb.bh[j];
-        if (w != 0.0) e += w * a.Stg[0][(size_t)ms.ks[j] * sst + f * n + p];
-      }
-      e += a.tb.bh[S - 1] * kf[f];
-      const double ye = X[f * n + p] + ms.h * e;
+      const double e = o.ws[f] + a.tb.bh[S - 1] * kf[f];
+      const double ye = o.x0[f] + ms.h * e;
       const double tol = a.so.atol[f] + a.so.rtol * fmax(fabs(y[f]), fabs(ye));
       if (tol > 0.0) en = fmax(en, fabs(y[f] - ye) / tol);
       const double dm = a.Xmon[f * n + p] - y[f];
@@ -548,6 +578,8 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
   r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
   r9[RED_DX2] = 0.0;
   if (p < gg.n && !snap[ml].halted) {
+    MemberOps<NC> ops;
+    member_operands<NC>(e.args[gg.m0 + m], snap[ml], s, p, ops);
     // the rows' partials over the pieces (stream-K: the tile lies in one
     // row group, whose pieces are tile.z .. + tile.w) or the column blocks,
     // added in order; the loads of a group of kSumGroup partials of every
@@ -587,7 +619,7 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
 #pragma unroll
       for (int k = 0; k < G; ++k) td[k] += q[k][(size_t)c * stride];
     }
-    member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[ml], s, p, td,
+    member_patch_stage<NC, G>(e.args[gg.m0 + m], snap[ml], s, p, td, ops,
                               gg.V[(s + 1) & 1] + (size_t)m * gg.ld, r9);
   }
   if (reduce) {
