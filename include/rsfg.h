@@ -347,8 +347,24 @@ typedef struct rsfg_ensemble rsfg_ensemble;
 int rsfg_ensemble_create(rsfg_handle *const *geoms, int ngeom, int members_per_geom,
                          rsfg_ensemble **e);
 int rsfg_ensemble_member(rsfg_ensemble *e, int member, rsfg_handle **h);
-/* o->prec must be RSFG_PREC_FP32, method rk5dp or rk3bs; st: one record
-   per member */
+/* operator of the batched product: RSFG_PREC_FP32 (the default) or
+   RSFG_PREC_BF16NF, a two-byte copy built from each geometry's FP32
+   operator: bf16 hi of every entry in tiles for v_mfma_f32_16x16x32_bf16,
+   plus the bf16 lo (the FP32 entry minus hi) of every 16-row block in each
+   256-column segment that one of its rows needs: per row, the segments in
+   decreasing energy until those left out hold at most nf_tol^2 of the row's
+   energy (nf_tol <= 0: 2^-6, the single-fault RSFG_PREC_BF16NF default).
+   The member slip rates enter as three bf16 terms whose sum is their FP32
+   value, so each tile's product is its bf16 entries times the FP32 slip
+   rates, accumulated in FP32 and flushed to FP64 every 256 columns.
+   RSFG_PREC_BF16NF takes 16 or 32 members per geometry and runs on the
+   stream-K shapes; switching keeps the shape when it fits, else takes the
+   first that does.  bytes (may be NULL): operator bytes one product
+   streams (hi, lo and padding).  No counterpart in the reference. */
+int rsfg_ensemble_set_operator(rsfg_ensemble *e, int prec, double nf_tol, double *bytes);
+/* o->prec must name the ensemble's operator (RSFG_PREC_FP32 unless
+   rsfg_ensemble_set_operator chose RSFG_PREC_BF16NF), method rk5dp or
+   rk3bs; st: one record per member */
 int rsfg_ensemble_solve(rsfg_ensemble *e, const rsfg_solver_opts *o, double t_end,
                         rsfg_stats *st);
 /* autotuner: times every candidate tile shape of the batched stage on this

@@ -48,6 +48,13 @@ struct EnsGeo {        // one geometry of the ensemble
   float* V[2];         // product inputs by stage parity, nm x ld (member-major)
   double* P;           // product partials [column block][rows_pad][nm]
   int rows_pad;        // rows rounded up to a whole number of product row groups
+  // two-byte operator (rsfg_ensemble_set_operator): hi tiles of 16 x 32
+  // (k_tile_h), lo tile groups of the near-field (16-row block, segment)
+  // pairs, lo_idx[b nseg + s] = group of block b in segment s or -1
+  const bf16_t* Kh;
+  const bf16_t* Kl;
+  const int* lo_idx;
+  int nseg;            // 256-column segments (ld / 256)
 };
 
 constexpr int kEpiPatches = 16;   // patches per epilogue tile (16 G operator rows)
@@ -72,6 +79,7 @@ struct EnsArgs {
   const int* wpc;
   double* Psk;
   int sk_tr;
+  const bf16_t* zero;   // kSegChunksH zero tiles (lo pieces of blocks without lo terms)
 };
 
 // members of a workgroup's geometry: controller fields read once
@@ -349,6 +357,142 @@ __device__ __forceinline__ void unit_product_slab(const float* __restrict__ Kt, 
   }
 }
 
+// ---- two-byte operator (bf16 hi everywhere, bf16 lo in near-field segments)
+//
+// Tiles of 16 rows x 32 columns (one v_mfma_f32_16x16x32_bf16 each, 1 KB):
+// lane l = 16 j + i holds row i, columns 8 j .. 8 j + 7 of the tile, so a
+// wave loads one contiguous kilobyte per tile.  The member slip rates stay
+// FP32 in HBM; a workgroup stages each 256-column segment of them in LDS as
+// three bf16 planes h, m, l with h + m + l equal to the FP32 value (8
+// significant bits each), and every tile is multiplied with all three, so a
+// tile's product is the bf16 entries times the FP32 slip rates, accumulated
+// in FP32 (flushed to FP64 every 256 columns as in the FP32 product).
+typedef __bf16 v8bf_t __attribute__((ext_vector_type(8)));
+constexpr int kSlabRowH = 256 + 8;   // bf16 per member row of a staged plane (padded)
+constexpr int kTileH = 512;          // bf16 per 16 x 32 tile
+constexpr int kSegChunksH = 8;       // 32-column chunks per 256-column segment
+
+// h, m, l with h + m + l = f exactly: each residual is exact in FP32 and
+// the last one has at most 8 significant bits
+__device__ __forceinline__ void split3(float f, uint32_t& h, uint32_t& m, uint32_t& l) {
+  h = bf16_rn(f);
+  const float r1 = f - __uint_as_float(h << 16);
+  m = bf16_rn(r1);
+  const float r2 = r1 - __uint_as_float(m << 16);
+  l = bf16_rn(r2);
+}
+
+// The product of one piece with the two-byte tiles: the rows of wave rf
+// block start at kab[rf] (the tile of chunk kc0; consecutive chunks are
+// consecutive tiles), the 32-column chunks [kc0, kc1) (whole segments) of
+// the member slip rates V.  Same ring and LDS double buffering as
+// unit_product_slab.
+template <int RF, int MF, int W, int U, int D>
+__device__ __forceinline__ void unit_product_slab_h(const bf16_t* const (&kab)[RF],
+                                                    const float* __restrict__ V, long ld,
+                                                    int kc0, int kc1, bf16_t* sV,
+                                                    double (&acc)[RF][MF][4]) {
+  constexpr int NM = 16 * MF, NG = kSegChunksH / U;
+  constexpr int NQ = NM * 64;         // 16-byte pieces of one segment's FP32 slip rates
+  constexpr int NV = NQ / (64 * W);   // pieces per thread
+  static_assert(NQ % (64 * W) == 0, "whole pieces per thread");
+  static_assert(kSegChunksH % U == 0 && D >= 2 && NG % D == 0, "ring slots repeat every segment");
+  const int tid = threadIdx.x, lane = tid & 63, i = lane & 15, j = lane >> 4;
+  typedef const __attribute__((address_space(1))) v4f_t gv4;
+  typedef const __attribute__((address_space(1))) v4u gu4;
+  const __attribute__((address_space(1))) bf16_t* ka[RF];
+#pragma unroll
+  for (int rf = 0; rf < RF; ++rf) ka[rf] = global_ptr(kab[rf] + 8 * lane);
+  const __attribute__((address_space(1))) float* vg = global_ptr(V);
+#pragma unroll
+  for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+    for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[rf][mf][r] = 0.0;
+  if (kc0 >= kc1) return;
+  // piece x = tid + 64 W t: member x / 64, columns 4 (x % 64) .. + 3 of the segment
+  v4f_t rv[NV];
+  auto vload = [&](int kc) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) {
+      const int x = tid + 64 * W * t;
+      rv[t] = *reinterpret_cast<gv4*>(vg + (size_t)(x / 64) * ld + (size_t)kc * 32 + 4 * (x % 64));
+    }
+  };
+  auto vstore = [&](int buf) {
+#pragma unroll
+    for (int t = 0; t < NV; ++t) {
+      const int x = tid + 64 * W * t;
+      uint32_t h[4], m[4], l[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) split3(rv[t][q], h[q], m[q], l[q]);
+      bf16_t* row = sV + (size_t)(buf * 3 * NM + x / 64) * kSlabRowH + 4 * (x % 64);
+      *reinterpret_cast<uint2*>(row) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+      *reinterpret_cast<uint2*>(row + NM * kSlabRowH) =
+          make_uint2(m[0] | (m[1] << 16), m[2] | (m[3] << 16));
+      *reinterpret_cast<uint2*>(row + 2 * NM * kSlabRowH) =
+          make_uint2(l[0] | (l[1] << 16), l[2] | (l[3] << 16));
+    }
+  };
+  v4u A[D][U][RF];
+  auto kload = [&](int kc, v4u (&a)[U][RF]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int rf = 0; rf < RF; ++rf)
+        a[u][rf] = RSFG_KLOAD(reinterpret_cast<gu4*>(ka[rf] + (size_t)(kc - kc0 + u) * kTileH));
+  };
+  vload(kc0);
+  vstore(0);
+  __syncthreads();
+#pragma unroll
+  for (int d = 0; d < D - 1; ++d) kload(kc0 + d * U, A[d]);
+  int buf = 0;
+  for (int seg = kc0; seg < kc1; seg += kSegChunksH) {
+    vload(seg + kSegChunksH < kc1 ? seg + kSegChunksH : seg);
+    v4f_t c[RF][MF];
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf) c[rf][mf] = v4f_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+      kload(min(seg + (g + D - 1) * U, kc1 - U), A[(g + D - 1) % D]);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        v8bf_t B[3][MF];
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+          for (int mf = 0; mf < MF; ++mf)
+            B[p][mf] = __builtin_bit_cast(
+                v8bf_t, *reinterpret_cast<const v4u*>(
+                            sV + (size_t)((buf * 3 + p) * NM + 16 * mf + i) * kSlabRowH +
+                            32 * (g * U + u) + 8 * j));
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+#pragma unroll
+          for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+            for (int mf = 0; mf < MF; ++mf)
+              c[rf][mf] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                  __builtin_bit_cast(v8bf_t, A[g % D][u][rf]), B[p][mf], c[rf][mf], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[rf][mf][r] += (double)c[rf][mf][r];
+    vstore(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+}
+
 // Operands of the stage s update of patch p of one member, requested
 // before its partials are summed (so their loads overlap those): friction
 // coefficients, loading rates, the state-only record (s > 0), the state X
@@ -543,6 +687,120 @@ __global__ __launch_bounds__(64 * W) void k_prod_sk(EnsArgs e) {
   }
 }
 
+// Stream-K product over the two-byte operator.  A piece whose geometry
+// field carries kLoPiece is the near-field lo part of one segment of its row
+// group: each 16-row block takes its lo tiles of that segment, or the zero
+// tiles where it has none.  Its partials are one more piece of the row
+// group, added by the epilogue after the hi pieces.
+constexpr int kLoPiece = 1 << 16;
+
+template <int RF, int MF, int W, int U, int D>
+__global__ __launch_bounds__(64 * W) void k_prod_sk_h(EnsArgs e) {
+  __shared__ bf16_t sV[2 * 3 * 16 * MF * kSlabRowH];
+  constexpr int NM = 16 * MF, TR = 16 * RF * W;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int k1 = e.wpc[blockIdx.x + 1];
+  for (int k = e.wpc[blockIdx.x]; k < k1; ++k) {
+    const int4 pc = e.pieces[k];
+    const EnsGeo& gg = e.geo[pc.x & (kLoPiece - 1)];
+    if (!geometry_running(e, gg)) continue;
+    const int rw = pc.y + wave * 16 * RF;
+    const int kc0 = pc.z / 2, kc1 = pc.w / 2;   // 32-column chunks
+    const bf16_t* kab[RF];
+    if (pc.x & kLoPiece) {
+      const int seg = pc.z / 16;
+#pragma unroll
+      for (int rf = 0; rf < RF; ++rf) {
+        const int q = gg.lo_idx[(size_t)(rw / 16 + rf) * gg.nseg + seg];
+        kab[rf] = q >= 0 ? gg.Kl + (size_t)q * kSegChunksH * kTileH : e.zero;
+      }
+    } else {
+      const long nk = gg.ld / 32;
+#pragma unroll
+      for (int rf = 0; rf < RF; ++rf) kab[rf] = gg.Kh + ((size_t)(rw / 16 + rf) * nk + kc0) * kTileH;
+    }
+    double acc[RF][MF][4];
+    unit_product_slab_h<RF, MF, W, U, D>(kab, gg.V[e.s & 1], gg.ld, kc0, kc1, sV, acc);
+    double* P = e.Psk + (size_t)k * TR * NM;
+#pragma unroll
+    for (int rf = 0; rf < RF; ++rf)
+#pragma unroll
+      for (int mf = 0; mf < MF; ++mf)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wave * 16 * RF + 16 * rf + 4 * (lane >> 4) + r;
+          P[(size_t)row * NM + 16 * mf + (lane & 15)] = acc[rf][mf][r];
+        }
+  }
+}
+
+// the FP32 operator K (rows x ldk, ldk a multiple of 16) as two-byte hi
+// tiles: tile t = b ld / 32 + c of 16 x 32 at Kh + t kTileH, lane l = 16 j + i
+// holding bf16(K[16 b + i][32 c + 8 j .. + 7]), zero past the operator
+__global__ void k_tile_h(const float* __restrict__ K, long ldk, int rows, int nrb, long ld,
+                         bf16_t* __restrict__ Kh) {
+  const long nk = ld / 32;
+  const long t = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (t >= (long)nrb * nk) return;
+  const int l = threadIdx.x & 63, i = l & 15, j = l >> 4;
+  const long b = t / nk, c = t % nk;
+  const long row = 16 * b + i, col = 32 * c + 8 * j;
+  uint32_t h[8];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    v4f_t v = v4f_t{0.f, 0.f, 0.f, 0.f};
+    if (row < rows && col + 4 * q < ldk)
+      v = *reinterpret_cast<const v4f_t*>(K + row * ldk + col + 4 * q);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) h[4 * q + r] = bf16_rn(v[r]);
+  }
+  *reinterpret_cast<v4u*>(Kh + t * kTileH + 8 * l) =
+      v4u{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
+}
+
+// the lo tiles bf16(K - hi) of the near-field (16-row block, segment)
+// groups grp[0 .. ngrp): group q's 8 tiles at Kl + (8 q + c) kTileH, lane
+// layout of k_tile_h
+__global__ void k_tile_lo(const float* __restrict__ K, long ldk, int rows,
+                          const int2* __restrict__ grp, int ngrp, bf16_t* __restrict__ Kl) {
+  const long t = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (t >= (long)ngrp * kSegChunksH) return;
+  const int l = threadIdx.x & 63, i = l & 15, j = l >> 4;
+  const int2 bs = grp[t / kSegChunksH];
+  const long row = 16L * bs.x + i, col = 256L * bs.y + 32 * (t % kSegChunksH) + 8 * j;
+  uint32_t lo[8];
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    v4f_t v = v4f_t{0.f, 0.f, 0.f, 0.f};
+    if (row < rows && col + 4 * q < ldk)
+      v = *reinterpret_cast<const v4f_t*>(K + row * ldk + col + 4 * q);
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      lo[4 * q + r] = bf16_rn(v[r] - __uint_as_float(bf16_rn(v[r]) << 16));
+  }
+  *reinterpret_cast<v4u*>(Kl + t * kTileH + 8 * l) = v4u{
+      lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16), lo[4] | (lo[5] << 16), lo[6] | (lo[7] << 16)};
+}
+
+// E[row nseg + s] = sum of K[row][c]^2 over the 256 columns of segment s
+// (one wave per row and segment)
+__global__ void k_seg_energy(const float* __restrict__ K, long ldk, int rows, int nseg,
+                             double* __restrict__ E) {
+  const long w = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (w >= (long)rows * nseg) return;
+  const int l = threadIdx.x & 63;
+  const long row = w / nseg, col = 256 * (w % nseg) + 4 * l;
+  double s = 0.0;
+  if (col < ldk) {
+    const v4f_t v = *reinterpret_cast<const v4f_t*>(K + row * ldk + col);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) s += (double)v[r] * (double)v[r];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (l == 0) E[w] = s;
+}
+
 // Stage epilogue: one workgroup per 16 patches (16 G operator rows) of one
 // geometry and 16 of its members, one thread per (patch, member): each
 // member's fields are read as 128-byte runs of 16 consecutive patches.  The
@@ -670,6 +928,11 @@ static void launch_prod_sk(const EnsArgs& e, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL((k_prod_sk<RF, MF, W, U, D>), dim3(nblocks), dim3(64 * W), 0, st, e);
 }
 
+template <int RF, int MF, int W, int U, int D>
+static void launch_prod_sk_h(const EnsArgs& e, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_prod_sk_h<RF, MF, W, U, D>), dim3(nblocks), dim3(64 * W), 0, st, e);
+}
+
 template <int NC, int G, int MF>
 static void launch_epi_b(const EnsArgs& e, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL((k_epi_b<NC, G, MF>), dim3(nblocks, MF), dim3(kEpiPatches * 16), 0, st, e);
@@ -703,7 +966,29 @@ static EnsProdFn prod_fn_mf(int cfg) {
   }
 }
 
-static EnsProdFn prod_fn(int nm, int cfg) {
+// the two-byte operator: the stream-K shapes, up to 32 members (the three
+// staged bf16 planes of 32 members take 99 KiB of LDS)
+template <int MF>
+static EnsProdFn prod_fn_h_mf(int cfg) {
+  switch (cfg) {
+    case 14: return launch_prod_sk_h<4, MF, 4, 1, 4>;
+    case 15: return launch_prod_sk_h<4, MF, 4, 1, 2>;
+    case 16: return launch_prod_sk_h<4, MF, 4, 1, 4>;
+    case 17: return launch_prod_sk_h<2, MF, 8, 1, 4>;
+    case 18: return launch_prod_sk_h<2, MF, 4, 1, 8>;
+    case 19: return launch_prod_sk_h<2, MF, 4, 1, 4>;
+    default: return nullptr;
+  }
+}
+
+// the product of shape cfg for nm members per geometry over the FP32
+// operator or (h) the two-byte one; nullptr where the shape does not fit
+static EnsProdFn prod_fn(int nm, int cfg, bool h = false) {
+  if (h) {
+    if (nm == 16) return prod_fn_h_mf<1>(cfg);
+    if (nm == 32) return prod_fn_h_mf<2>(cfg);
+    return nullptr;
+  }
   if (nm == 16) return prod_fn_mf<1>(cfg);
   if (nm == 32) return prod_fn_mf<2>(cfg);
   if (nm == 64) return prod_fn_mf<4>(cfg);
@@ -751,6 +1036,15 @@ struct rsfg_ensemble {
   float* d_v = nullptr;
   double* d_P = nullptr;
   float* d_Kt = nullptr;
+  // two-byte operator (rsfg_ensemble_set_operator)
+  int prec = RSFG_PREC_FP32;   // operator of the product: RSFG_PREC_FP32 or RSFG_PREC_BF16NF
+  double nf_tol = 0.0;         // tolerance the lo groups were chosen with (0: not built)
+  bf16_t* d_Kh = nullptr;
+  bf16_t* d_Kl = nullptr;
+  int* d_loidx = nullptr;
+  bf16_t* d_zero = nullptr;
+  std::vector<std::vector<int>> lo_host;   // lo_idx per geometry (host copy for plans)
+  double h_bytes = 0.0;        // operator bytes one two-byte product streams
   long long* d_halted = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -761,18 +1055,32 @@ struct rsfg_ensemble {
 // [w T / nwg, (w + 1) T / nwg) to workgroup w, cut at row-group boundaries.
 static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR, int nwg,
                          std::vector<int4>& tiles) {
+  // with the two-byte operator a row group's items are its segments, then
+  // one lo piece per segment in which one of its 16-row blocks has lo tiles
   struct RowGroup {
     int g, r, nseg;
+    std::vector<int> lo;
   };
+  const bool h = e->prec == RSFG_PREC_BF16NF;
   std::vector<RowGroup> rgs;
   std::vector<int> rg0(e->ngeo, -1);
   for (int g : order) {
     const EnsGeo& q = e->geo[g];
     rg0[g] = (int)rgs.size();
-    for (int r = 0; r < q.rows; r += TR) rgs.push_back({g, r, (int)(q.ld / 256)});
+    for (int r = 0; r < q.rows; r += TR) {
+      RowGroup x{g, r, (int)(q.ld / 256), {}};
+      if (h)
+        for (int s = 0; s < q.nseg; ++s)
+          for (int b = r / 16; b < (r + TR) / 16; ++b)
+            if (e->lo_host[g][(size_t)b * q.nseg + s] >= 0) {
+              x.lo.push_back(s);
+              break;
+            }
+      rgs.push_back(x);
+    }
   }
   long long T = 0;
-  for (const auto& x : rgs) T += x.nseg;
+  for (const auto& x : rgs) T += x.nseg + (long long)x.lo.size();
   std::vector<int4> pieces;
   std::vector<int> wpc(nwg + 1);
   std::vector<int2> rgtab(rgs.size(), make_int2(0, 0));
@@ -784,13 +1092,20 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
     const long long end = (long long)(w + 1) * T / nwg;
     while (item < end) {
       const RowGroup& x = rgs[ri];
-      const int take = (int)std::min<long long>(end - item, x.nseg - seg);
+      const int nit = x.nseg + (int)x.lo.size();
       if (rgtab[ri].y == 0) rgtab[ri].x = (int)pieces.size();
       rgtab[ri].y++;
-      pieces.push_back(make_int4(x.g, x.r, 16 * seg, 16 * (seg + take)));
+      int take = 1;
+      if (seg < x.nseg) {   // a run of hi segments
+        take = (int)std::min<long long>(end - item, x.nseg - seg);
+        pieces.push_back(make_int4(x.g, x.r, 16 * seg, 16 * (seg + take)));
+      } else {              // the lo piece of one segment
+        const int s = x.lo[seg - x.nseg];
+        pieces.push_back(make_int4(x.g | kLoPiece, x.r, 16 * s, 16 * (s + 1)));
+      }
       item += take;
       seg += take;
-      if (seg == x.nseg) {
+      if (seg == nit) {
         ++ri;
         seg = 0;
       }
@@ -913,12 +1228,13 @@ static EnsArgs ens_args(rsfg_ensemble* e, int s) {
   a.wpc = sk ? e->d_wpc : nullptr;
   a.Psk = sk ? e->d_Psk : nullptr;
   a.sk_tr = sk ? cfg_rows(e->cfg) : 0;
+  a.zero = e->d_zero;
   return a;
 }
 
 // the stage product and epilogue of stage a.s for every member
 static void ens_stage(rsfg_ensemble* e, const EnsArgs& a) {
-  prod_fn(e->nm, e->cfg)(a, e->nprod, e->stream);
+  prod_fn(e->nm, e->cfg, e->prec == RSFG_PREC_BF16NF)(a, e->nprod, e->stream);
   epi_fn(e->ncomp, e->G, e->nm)(a, e->ntiles, e->stream);
 }
 
@@ -971,6 +1287,113 @@ static void ens_attempt(rsfg_ensemble* e, int S) {
   }
   if (e->ncomp == 2) hipLaunchKernelGGL(k_ctrl_b<2>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
   else hipLaunchKernelGGL(k_ctrl_b<1>, dim3(e->M), dim3(kCtrlThreads), 0, e->stream, a);
+}
+
+// The two-byte operator of every geometry from its FP32 operator: hi tiles
+// of every entry; per row, the 256-column segments in decreasing energy
+// until the segments left out hold at most tol^2 of the row's energy, and
+// lo tiles for each (16-row block, segment) that one of its rows takes.
+static int ens_build_h(rsfg_ensemble* e, double tol) {
+  e->nf_tol = 0.0;
+  hipFree(e->d_Kh);
+  hipFree(e->d_Kl);
+  hipFree(e->d_loidx);
+  e->d_Kh = nullptr;
+  e->d_Kl = nullptr;
+  e->d_loidx = nullptr;
+  e->lo_host.assign(e->ngeo, std::vector<int>());
+  size_t kh = 0, nidx = 0;
+  for (auto& q : e->geo) {
+    q.nseg = (int)(q.ld / 256);
+    kh += (size_t)q.rows_pad * q.ld;
+    nidx += (size_t)(q.rows_pad / 16) * q.nseg;
+  }
+  if (!e->d_zero) {
+    RSFG_HIP(hipMalloc(&e->d_zero, sizeof(bf16_t) * kSegChunksH * kTileH));
+    RSFG_HIP(hipMemset(e->d_zero, 0, sizeof(bf16_t) * kSegChunksH * kTileH));
+  }
+  RSFG_HIP(hipMalloc(&e->d_Kh, sizeof(bf16_t) * kh));
+  RSFG_HIP(hipMalloc(&e->d_loidx, sizeof(int) * nidx));
+  std::vector<std::vector<int2>> grp(e->ngeo);
+  size_t ngrp = 0, ko = 0, io = 0;
+  for (int g = 0; g < e->ngeo; ++g) {
+    EnsGeo& q = e->geo[g];
+    const int nrb = q.rows_pad / 16;
+    q.Kh = e->d_Kh + ko;
+    ko += (size_t)q.rows_pad * q.ld;
+    const long ntile = (long)nrb * (q.ld / 32);
+    hipLaunchKernelGGL(k_tile_h, dim3((unsigned)((ntile + 3) / 4)), dim3(256), 0, e->stream, q.K,
+                       q.ldk, q.rows, nrb, q.ld, const_cast<bf16_t*>(q.Kh));
+    RSFG_HIP(hipGetLastError());
+    // energies of the rows' segments
+    double* dE = nullptr;
+    const size_t ne = (size_t)q.rows * q.nseg;
+    RSFG_HIP(hipMalloc(&dE, sizeof(double) * ne));
+    hipLaunchKernelGGL(k_seg_energy, dim3((unsigned)((ne + 3) / 4)), dim3(256), 0, e->stream, q.K,
+                       q.ldk, q.rows, q.nseg, dE);
+    std::vector<double> E(ne);
+    hipError_t he = hipGetLastError();
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(E.data(), dE, sizeof(double) * ne, hipMemcpyDeviceToHost, e->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+    hipFree(dE);
+    if (he != hipSuccess) return hip_fail(he, "rsfg_ensemble_set_operator", __FILE__, __LINE__);
+    std::vector<char> flag((size_t)nrb * q.nseg, 0);
+    std::vector<int> order(q.nseg);
+    for (int r = 0; r < q.rows; ++r) {
+      const double* er = E.data() + (size_t)r * q.nseg;
+      double total = 0.0;
+      for (int s = 0; s < q.nseg; ++s) total += er[s];
+      for (int s = 0; s < q.nseg; ++s) order[s] = s;
+      std::sort(order.begin(), order.end(), [&](int a, int b) { return er[a] > er[b]; });
+      double left = total;
+      for (int k = 0; k < q.nseg && left > tol * tol * total; ++k) {
+        flag[(size_t)(r / 16) * q.nseg + order[k]] = 1;
+        left -= er[order[k]];
+      }
+    }
+    std::vector<int>& idx = e->lo_host[g];
+    idx.assign((size_t)nrb * q.nseg, -1);
+    for (int b = 0; b < nrb; ++b)
+      for (int s = 0; s < q.nseg; ++s)
+        if (flag[(size_t)b * q.nseg + s]) {
+          idx[(size_t)b * q.nseg + s] = (int)grp[g].size();
+          grp[g].push_back(make_int2(b, s));
+        }
+    q.lo_idx = e->d_loidx + io;
+    RSFG_HIP(hipMemcpy(const_cast<int*>(q.lo_idx), idx.data(), sizeof(int) * idx.size(),
+                       hipMemcpyHostToDevice));
+    io += idx.size();
+    ngrp += grp[g].size();
+  }
+  RSFG_HIP(hipMalloc(&e->d_Kl, sizeof(bf16_t) * std::max<size_t>(1, ngrp) * kSegChunksH * kTileH));
+  size_t go = 0;
+  double hb = 0.0;
+  for (int g = 0; g < e->ngeo; ++g) {
+    EnsGeo& q = e->geo[g];
+    q.Kl = e->d_Kl + go * kSegChunksH * kTileH;
+    hb += 2.0 * q.rows_pad * q.ld + 2.0 * grp[g].size() * kSegChunksH * kTileH;
+    if (!grp[g].empty()) {
+      int2* dg = nullptr;
+      RSFG_HIP(hipMalloc(&dg, sizeof(int2) * grp[g].size()));
+      hipError_t he = hipMemcpy(dg, grp[g].data(), sizeof(int2) * grp[g].size(),
+                                hipMemcpyHostToDevice);
+      const long nt = (long)grp[g].size() * kSegChunksH;
+      if (he == hipSuccess) {
+        hipLaunchKernelGGL(k_tile_lo, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, e->stream,
+                           q.K, q.ldk, q.rows, dg, (int)grp[g].size(), const_cast<bf16_t*>(q.Kl));
+        he = hipGetLastError();
+      }
+      if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
+      hipFree(dg);
+      if (he != hipSuccess) return hip_fail(he, "rsfg_ensemble_set_operator", __FILE__, __LINE__);
+    }
+    go += grp[g].size();
+  }
+  RSFG_HIP(hipStreamSynchronize(e->stream));
+  e->h_bytes = hb;
+  e->nf_tol = tol;
+  return RSFG_OK;
 }
 
 extern "C" {
@@ -1095,25 +1518,28 @@ int rsfg_ensemble_solve(rsfg_ensemble* e, const rsfg_solver_opts* o, double t_en
   RSFG_CHECK(e && o, RSFG_EINVAL, "rsfg_ensemble_solve: null argument");
   RSFG_CHECK(o->method == RSFG_RK5DP || o->method == RSFG_RK3BS, RSFG_EINVAL,
              "rsfg_ensemble_solve: method must be rk5dp or rk3bs");
-  RSFG_CHECK(o->prec == RSFG_PREC_FP32, RSFG_EINVAL,
-             "rsfg_ensemble_solve: the batched product uses the FP32 operator (prec FP32)");
+  RSFG_CHECK(o->prec == e->prec, RSFG_EINVAL,
+             "rsfg_ensemble_solve: prec must name the ensemble's operator (RSFG_PREC_FP32, or "
+             "RSFG_PREC_BF16NF after rsfg_ensemble_set_operator)");
   RSFG_CHECK(e->plan_ok, RSFG_ESTATE,
              "rsfg_ensemble_solve: no complete work plan (select or tune a shape again)");
   RSFG_HIP(hipSetDevice(e->device));
   const auto wall0 = std::chrono::steady_clock::now();
-  int rc = fill_member_args(e, o);
+  rsfg_solver_opts o32 = *o;   // the members' stage arguments are those of the FP32 solver
+  o32.prec = RSFG_PREC_FP32;
+  int rc = fill_member_args(e, &o32);
   if (rc) return rc;
   for (int m = 0; m < e->M; ++m) {
     rsfg_ctx* h = e->members[m];
     RSFG_CHECK((size_t)e->geo[m / e->nm].ntiles * kNRed <= h->partial_count, RSFG_ESTATE,
                "rsfg_ensemble_solve: partial buffer too small");
   }
-  rc = ens_start(e, o, t_end);
+  rc = ens_start(e, &o32, t_end);
   if (rc) return rc;
   Tableau tb;
   {
     StageArgs a0;
-    fill_stage_args(e->members[0], o, e->geo[0].K, a0);
+    fill_stage_args(e->members[0], &o32, e->geo[0].K, a0);
     tb = a0.tb;
   }
   const int per = o->steps_per_launch > 0 ? o->steps_per_launch : 32;
@@ -1259,7 +1685,7 @@ int rsfg_ensemble_tune(rsfg_ensemble* e, const rsfg_solver_opts* o, int* choice,
   const dim3 gp((max_patches(e) + 255) / 256, e->M);
   for (int cfg = 0; cfg < kEnsNCfg; ++cfg) {
     if (ms) ms[cfg] = -1.0;
-    if (!prod_fn(e->nm, cfg)) continue;
+    if (!prod_fn(e->nm, cfg, e->prec == RSFG_PREC_BF16NF)) continue;
     rc = build_plan(e, cfg);
     if (rc) return rc;
     rc = ens_start(e, &oo, t_end);
@@ -1310,7 +1736,7 @@ int rsfg_ensemble_bench(rsfg_ensemble* e, const rsfg_solver_opts* o, int nrep, d
   else hipLaunchKernelGGL(k_prep_b<1>, gp, dim3(256), 0, e->stream, a);
   a.s = 1;
   ens_stage(e, a);
-  EnsProdFn pf = prod_fn(e->nm, e->cfg);
+  EnsProdFn pf = prod_fn(e->nm, e->cfg, e->prec == RSFG_PREC_BF16NF);
   EnsEpiFn ef = epi_fn(e->ncomp, e->G, e->nm);
   float t = 0.f;
   RSFG_HIP(hipEventRecord(e->ev0, e->stream));
@@ -1350,10 +1776,44 @@ int rsfg_ensemble_config(rsfg_ensemble* e, int* cfg, int* ncfg, int* shape) {
 int rsfg_ensemble_select(rsfg_ensemble* e, int cfg) {
   RSFG_CHECK(e, RSFG_EINVAL, "rsfg_ensemble_select: null argument");
   RSFG_CHECK(cfg >= 0 && cfg < kEnsNCfg, RSFG_EINVAL, "rsfg_ensemble_select: shape out of range");
-  RSFG_CHECK(prod_fn(e->nm, cfg), RSFG_EINVAL,
-             "rsfg_ensemble_select: shape does not fit the members per geometry");
+  RSFG_CHECK(prod_fn(e->nm, cfg, e->prec == RSFG_PREC_BF16NF), RSFG_EINVAL,
+             "rsfg_ensemble_select: shape does not fit the members per geometry or the "
+             "operator format");
   RSFG_HIP(hipSetDevice(e->device));
   return build_plan(e, cfg);
+}
+
+int rsfg_ensemble_set_operator(rsfg_ensemble* e, int prec, double nf_tol, double* bytes) {
+  RSFG_CHECK(e, RSFG_EINVAL, "rsfg_ensemble_set_operator: null argument");
+  RSFG_CHECK(prec == RSFG_PREC_FP32 || prec == RSFG_PREC_BF16NF, RSFG_EINVAL,
+             "rsfg_ensemble_set_operator: prec must be RSFG_PREC_FP32 or RSFG_PREC_BF16NF");
+  RSFG_HIP(hipSetDevice(e->device));
+  const bool h = prec == RSFG_PREC_BF16NF;
+  if (h) {
+    RSFG_CHECK(e->nm == 16 || e->nm == 32, RSFG_EINVAL,
+               "rsfg_ensemble_set_operator: the two-byte operator takes 16 or 32 members per "
+               "geometry");
+    const double tol = nf_tol > 0.0 ? nf_tol : 1.0 / 64.0;
+    RSFG_CHECK(tol < 1.0, RSFG_EINVAL, "rsfg_ensemble_set_operator: nf_tol must be below 1");
+    if (tol != e->nf_tol) {
+      const int rc = ens_build_h(e, tol);
+      if (rc) return rc;
+    }
+  }
+  e->prec = prec;
+  int cfg = e->cfg;
+  if (!prod_fn(e->nm, cfg, h)) {
+    cfg = 0;
+    while (cfg < kEnsNCfg && !prod_fn(e->nm, cfg, h)) ++cfg;
+  }
+  const int rc = build_plan(e, cfg);
+  if (rc) return rc;
+  if (bytes) {
+    double b = 0.0;
+    for (const auto& q : e->geo) b += 4.0 * q.rows_pad * q.ld;
+    *bytes = h ? e->h_bytes : b;
+  }
+  return RSFG_OK;
 }
 
 int rsfg_ensemble_destroy(rsfg_ensemble* e) {
@@ -1371,6 +1831,10 @@ int rsfg_ensemble_destroy(rsfg_ensemble* e) {
   hipFree(e->d_Psk);
   hipFree(e->d_P);
   hipFree(e->d_Kt);
+  hipFree(e->d_Kh);
+  hipFree(e->d_Kl);
+  hipFree(e->d_loidx);
+  hipFree(e->d_zero);
   hipFree(e->d_halted);
   if (e->ev0) hipEventDestroy(e->ev0);
   if (e->ev1) hipEventDestroy(e->ev1);

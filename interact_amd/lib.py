@@ -41,7 +41,7 @@ EXPORTS = [
     "rsfg_read_table", "rsfg_write_checkpoint", "rsfg_read_checkpoint",
     "rsfg_create_member", "rsfg_ensemble_create", "rsfg_ensemble_member",
     "rsfg_ensemble_solve", "rsfg_ensemble_tune", "rsfg_ensemble_config", "rsfg_ensemble_select",
-    "rsfg_ensemble_destroy", "rsfg_ensemble_bench", "rsfg_set_step_log", "rsfg_get_step_log", "rsfg_replay",
+    "rsfg_ensemble_destroy", "rsfg_ensemble_bench", "rsfg_ensemble_set_operator", "rsfg_set_step_log", "rsfg_get_step_log", "rsfg_replay",
     "rsfg_adjoint_gradient", "rsfg_adjoint_bench",
 ]
 
@@ -134,6 +134,7 @@ def load(path=None):
     L.rsfg_ensemble_config.argtypes = [h, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]
     L.rsfg_ensemble_select.argtypes = [h, C.c_int]
+    L.rsfg_ensemble_set_operator.argtypes = [h, C.c_int, C.c_double, C.POINTER(C.c_double)]
     L.rsfg_ensemble_destroy.argtypes = [h]
     L.rsfg_ensemble_bench.argtypes = [h, C.POINTER(SolverOpts), C.c_int, C.POINTER(C.c_double),
                                       C.POINTER(C.c_double)]

@@ -370,6 +370,7 @@ class Ensemble:
                                                C.byref(e)), "rsfg_ensemble_create")
         self.e = e
         self.nm = members_per_geom
+        self.prec = L.PREC_FP32
         self.members = []
         for gi, g in enumerate(self.geoms):
             for k in range(members_per_geom):
@@ -383,8 +384,19 @@ class Ensemble:
                 m.dim = getattr(g, "dim", 2 * g.ncomp + 2)
                 self.members.append(m)
 
+    def set_operator(self, prec, nf_tol=0.0):
+        """Operator of the batched product: PREC_FP32 or PREC_BF16NF (bf16 hi
+        tiles plus bf16 lo in the near-field segments of each row, chosen with
+        energy tolerance nf_tol, <= 0 for 2^-6).  Returns the operator bytes
+        one product streams."""
+        b = C.c_double()
+        L.check(self._lib.rsfg_ensemble_set_operator(self.e, prec, nf_tol, C.byref(b)),
+                "rsfg_ensemble_set_operator")
+        self.prec = prec
+        return b.value
+
     def solve(self, t_end, **kw):
-        kw.setdefault("prec", L.PREC_FP32)
+        kw.setdefault("prec", self.prec)
         o = FaultSystem.solver_options(**kw)
         st = (L.Stats * len(self.members))()
         L.check(self._lib.rsfg_ensemble_solve(self.e, C.byref(o), t_end, st),

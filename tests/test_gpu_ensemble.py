@@ -345,7 +345,7 @@ def test_ensemble_argument_checks():
         ens.solve(SEC_PER_YEAR, rtol=1e-6)
     for k in range(16):
         _configure(ens.members[k], faults.ensemble_fields(g, L, W, k), 2)
-    with pytest.raises(lib.RsfgError, match="prec FP32"):
+    with pytest.raises(lib.RsfgError, match="prec must name"):
         ens.solve(SEC_PER_YEAR, rtol=1e-6, prec=lib.PREC_FP64)
     with pytest.raises(lib.RsfgError, match="rk5dp or rk3bs"):
         ens.solve(SEC_PER_YEAR, rtol=1e-6, method=lib.ARKIMEX_L2)
