@@ -34,7 +34,9 @@ first steps.  --config c4 runs the batched ensemble instead: four planar outline
 seeded friction draws each, 100 yr per member, all 128 members in one
 ensemble (K [v_1 .. v_32] per geometry on the matrix cores, FP32 operators in MFMA tiles),
 against the same 128 members run one after another through the single-fault
-solver.
+solver; then the same ensemble on the two-byte operator (bf16 hi tiles plus
+near-field lo tiles on bf16 MFMAs) against single runs on the bf16nf operator
+(--no-two-byte skips that leg).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--config c3|c2|c4|c5]
 """
@@ -283,6 +285,9 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--fd-directions", type=int, default=32,
                     help="c5: random directions of the central-difference check")
+    ap.add_argument("--no-two-byte", action="store_true",
+                    help="c4: skip the ensemble on the two-byte operator (bf16 hi + near-field "
+                         "lo tiles on bf16 MFMAs) and its single runs on the bf16nf operator")
     ap.add_argument("--singles", type=int, default=None,
                     help="c4: members also run one after another through the single-fault "
                          "solver (default all 128)")
@@ -564,11 +569,13 @@ def ensemble_bench(args):
     years = 100.0 if args.years is None else args.years
     t_end = years * SEC_PER_YEAR
     nm = ENS_MEMBERS
+    two = not args.no_two_byte
     geoms, gdat = [], []
     for (L, W) in ENS_OUTLINES:
         g = faults.planar_grid(1.0, L, W)
         fs = FaultSystem(g)
-        fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9, store=lib.PREC_FP32)
+        fs.assemble(lib.SLIP_VECTOR, shear_modulus=32e9,
+                    store=lib.PREC_FP32 | (lib.PREC_BF16NF if two else 0))
         geoms.append(fs)
         gdat.append((g, L, W))
     npat = [len(g["l"]) for g, _, _ in gdat]
@@ -612,6 +619,49 @@ def ensemble_bench(args):
     steps = [s["steps"] + s["rejects"] for s in st]
     log(f"ensemble: {M} members x {years:g} yr in {wall:.2f} s ({ens_rate:.3e} sim-yr/h); "
         f"attempts per member min {min(steps)} max {max(steps)}")
+    ev32 = [m.events() for m in ens.members]
+
+    def onset_diff(evs_a, evs_b):
+        worst, nother = 0.0, 0
+        for ea, eb in zip(evs_a, evs_b):
+            oa = ea[ea[:, 1] > 0, 0] / SEC_PER_YEAR if len(ea) else np.zeros(0)
+            ob = eb[eb[:, 1] > 0, 0] / SEC_PER_YEAR if len(eb) else np.zeros(0)
+            k = min(len(oa), len(ob))
+            if k:
+                worst = max(worst, float(np.abs(oa[:k] - ob[:k]).max()))
+            nother += int(len(oa) != len(ob))
+        return worst, nother
+
+    if two:
+        # the same members on the two-byte operator
+        bh = ens.set_operator(lib.PREC_BF16NF)
+        t0 = time.perf_counter()
+        choice_h, ms_h, _ = ens.tune(rtol=args.rtol)
+        tune_h = time.perf_counter() - t0
+        log("two-byte tuner: " + ", ".join(f"{shapes[i]} {ms_h[i]*1e3:.1f} us"
+                                           for i in range(len(ms_h)) if ms_h[i] > 0) +
+            f" -> {shapes[choice_h]} ({tune_h:.2f} s)")
+        prod_h, epi_h = ens.bench(20, rtol=args.rtol)
+        log(f"two-byte tuned shape: product {prod_h * 1e3:.1f} us, epilogue {epi_h * 1e3:.1f} us")
+        for _ in range(args.warmup):
+            for m, (gi, f, X0) in zip(ens.members, members):
+                m.set_state(X0)
+            ens.solve(t_end, rtol=args.rtol, steps_per_launch=64)
+        walls_h = []
+        for _ in range(args.steps):
+            for m, (gi, f, X0) in zip(ens.members, members):
+                m.set_state(X0)
+            t0 = time.perf_counter()
+            st_h = ens.solve(t_end, rtol=args.rtol, steps_per_launch=64)
+            walls_h.append(time.perf_counter() - t0)
+        wall_h = float(np.mean(walls_h))
+        evh = [m.events() for m in ens.members]
+        on_h32, n_h32 = onset_diff(evh, ev32)
+        steps_h = [q["steps"] + q["rejects"] for q in st_h]
+        log(f"two-byte ensemble: {M} members x {years:g} yr in {wall_h:.2f} s "
+            f"({M * years / (wall_h / 3600.0):.3e} sim-yr/h); attempts per member min "
+            f"{min(steps_h)} max {max(steps_h)}; onsets vs the FP32 ensemble: worst "
+            f"{on_h32:.2e} yr, {n_h32} members with another count")
     # the same members one after another through the single-fault solver
     nsing = M if args.singles is None else min(args.singles, M)
     single_s, worst_on, n_on = 0.0, 0.0, 0
@@ -622,7 +672,7 @@ def ensemble_bench(args):
         t0 = time.perf_counter()
         s.solve(t_end, rtol=args.rtol, prec=lib.PREC_FP32, steps_per_launch=64)
         single_s += time.perf_counter() - t0
-        e1, e2 = s.events(), ens.members[i].events()
+        e1, e2 = s.events(), ev32[i]
         o1 = e1[e1[:, 1] > 0, 0] / SEC_PER_YEAR if len(e1) else np.zeros(0)
         o2 = e2[e2[:, 1] > 0, 0] / SEC_PER_YEAR if len(e2) else np.zeros(0)
         k = min(len(o1), len(o2))
@@ -635,6 +685,24 @@ def ensemble_bench(args):
     seq_rate = nsing * years / (single_s / 3600.0)
     log(f"singles: {nsing} runs one after another in {single_s:.2f} s ({seq_rate:.3e} sim-yr/h); "
         f"onsets vs ensemble: worst {worst_on:.2e} yr, {n_on} members with another count")
+    if two:
+        # the same members one after another on the single-fault bf16nf operator
+        single_h, evs = 0.0, []
+        for i in range(nsing):
+            gi, f, X0 = members[i]
+            s = geoms[gi].member()
+            configure(s, f, X0)
+            t0 = time.perf_counter()
+            s.solve(t_end, rtol=args.rtol, prec=lib.PREC_BF16NF, steps_per_launch=64)
+            single_h += time.perf_counter() - t0
+            evs.append(s.events())
+            s.close()
+            if (i + 1) % nm == 0:
+                log(f"bf16nf singles: {i + 1} of {nsing} done, {single_h:.1f} s")
+        on_hs, n_hs = onset_diff(evh[:nsing], evs)
+        seq_h = nsing * years / (single_h / 3600.0)
+        log(f"bf16nf singles: {nsing} runs in {single_h:.2f} s ({seq_h:.3e} sim-yr/h); onsets vs "
+            f"the two-byte ensemble: worst {on_hs:.2e} yr, {n_hs} members with another count")
     # roofline of the batched product kernel at the tuned shape (HIP events
     # around repeated launches of a middle stage, on the ensemble's stream)
     launch_s = prod_ms * 1e-3
@@ -679,6 +747,36 @@ def ensemble_bench(args):
         "attempts_per_member": dict(min=min(steps), max=max(steps)),
         "roofline": roof,
     }
+    if two:
+        rate_h = M * years / (wall_h / 3600.0)
+        pad = sum(2.0 * (-(-2 * n // 256) * 256) ** 2 for n in npat)   # hi tiles as streamed
+        lo_bytes = bh - pad
+        hb = sum((2 * n) * (2 * n) * 2.0 for n in npat) + lo_bytes
+        ls = prod_h * 1e-3
+        out["two_byte"] = dict(
+            operator="bf16 hi of every entry (16 x 32 tiles, v_mfma_f32_16x16x32_bf16) + bf16 lo "
+                     "tiles of each 16-row block in the 256-column segments its rows need "
+                     "(energy tolerance 2^-6); slip rates as three bf16 terms summing to FP32; "
+                     "FP32 accumulation, FP64 every 256 columns",
+            value=rate_h, unit="sim-years/hour", wall_s=wall_h,
+            speedup_vs_fp32_ensemble=wall / wall_h,
+            sequential_single_runs_bf16nf=dict(runs=nsing, wall_s=single_h,
+                                               sim_years_per_hour=seq_h),
+            speedup_vs_sequential_bf16nf=rate_h / seq_h,
+            speedup_vs_sequential_fp32=rate_h / seq_rate,
+            parity_vs_fp32_ensemble=dict(worst_onset_diff_yr=on_h32,
+                                         members_with_other_count=n_h32),
+            parity_vs_single_bf16nf=dict(worst_onset_diff_yr=on_hs,
+                                         members_with_other_count=n_hs),
+            attempts_per_member=dict(min=min(steps_h), max=max(steps_h)),
+            tuner=dict(choice=list(shapes[choice_h]), wall_s=tune_h,
+                       ms={str(tuple(shapes[i])): ms_h[i] for i in range(len(ms_h)) if ms_h[i] > 0}),
+            roofline=dict(kernel=f"k_prod_sk_h (two-byte operator), shape {shapes[choice_h]}",
+                          bound="hbm", achieved=hb / ls / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                          frac=hb / ls / 1e9 / HBM_PEAK_GBS, traffic=None,
+                          bytes_per_launch=hb, lo_bytes=lo_bytes, streamed_bytes=bh,
+                          launch_us=prod_h * 1e3, epilogue_us=epi_h * 1e3,
+                          stage_us=ms_h[choice_h] * 1e3))
     if not args.no_cpu:
         # FP64 oracle right-hand sides on each geometry, scaled by every
         # member's RHS count from the ensemble run
@@ -743,6 +841,15 @@ def ensemble_kernels_only(args):
     pm, pe = ens.bench(20, rtol=args.rtol)
     log(f"kernels-only c4: tuned shape {shapes[choice]}: product {pm*1e3:.1f} us, "
         f"epilogue {pe*1e3:.1f} us by HIP events (20 launches each)")
+    if not args.no_two_byte:
+        bh = ens.set_operator(lib.PREC_BF16NF)
+        choice, ms, shapes = ens.tune(rtol=args.rtol)
+        log("kernels-only c4 two-byte: " + ", ".join(
+            f"{shapes[i]} {ms[i]*1e3:.1f} us" for i in range(len(ms)) if ms[i] > 0))
+        pm, pe = ens.bench(20, rtol=args.rtol)
+        log(f"kernels-only c4 two-byte: {bh / 1e6:.1f} MB streamed per product; tuned shape "
+            f"{shapes[choice]}: product {pm*1e3:.1f} us, epilogue {pe*1e3:.1f} us by HIP "
+            f"events (20 launches each)")
     ens.close()
     for g in geoms:
         g.close()

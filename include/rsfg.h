@@ -334,7 +334,8 @@ int rsfg_solve_continue(rsfg_handle *h, const rsfg_solver_opts *o, double t_end,
 /* ---- ensembles ---------------------------------------------------------
    Batched runs over shared operators (no counterpart in the reference,
    which runs one parameter set per process): members_per_geom members per
-   geometry handle (16, 32 or 64), each a member handle (rsfg_create_member)
+   geometry handle (16, 32 or 64; up to 64 geometry handles), each a member
+   handle (rsfg_create_member)
    configured through the ordinary calls (rsfg_set_friction,
    rsfg_set_loading, rsfg_set_state; rsfg_events, rsfg_monitor,
    rsfg_get_state afterwards).  Every geometry needs an FP32 operator.  The
@@ -350,10 +351,11 @@ int rsfg_ensemble_member(rsfg_ensemble *e, int member, rsfg_handle **h);
 /* operator of the batched product: RSFG_PREC_FP32 (the default) or
    RSFG_PREC_BF16NF, a two-byte copy built from each geometry's FP32
    operator: bf16 hi of every entry in tiles for v_mfma_f32_16x16x32_bf16,
-   plus the bf16 lo (the FP32 entry minus hi) of every 16-row block in each
-   256-column segment that one of its rows needs: per row, the segments in
-   decreasing energy until those left out hold at most nf_tol^2 of the row's
-   energy (nf_tol <= 0: 2^-6, the single-fault RSFG_PREC_BF16NF default).
+   plus the bf16 lo (the FP32 entry minus hi) of every 256-row group over
+   the hull of the 256-column segments its rows need: per row, the segments
+   in decreasing energy until those left out hold at most nf_tol^2 of the
+   row's energy (nf_tol <= 0: 2^-6, the single-fault RSFG_PREC_BF16NF
+   default).
    The member slip rates enter as three bf16 terms whose sum is their FP32
    value, so each tile's product is its bf16 entries times the FP32 slip
    rates, accumulated in FP32 and flushed to FP64 every 256 columns.

@@ -49,11 +49,14 @@ struct EnsGeo {        // one geometry of the ensemble
   double* P;           // product partials [column block][rows_pad][nm]
   int rows_pad;        // rows rounded up to a whole number of product row groups
   // two-byte operator (rsfg_ensemble_set_operator): hi tiles of 16 x 32
-  // (k_tile_h), lo tile groups of the near-field (16-row block, segment)
-  // pairs, lo_idx[b nseg + s] = group of block b in segment s or -1
+  // (k_tile_h); per 256-row group q, lo_rg[q] = (s0, s1, t0, 0): the lo
+  // tiles of its 16 blocks over the segments [s0, s1) (the hull of the
+  // near-field segments its rows take; s0 = s1 without lo), block-major
+  // from tile t0 of Kl (block l, 32-column chunk c of the hull at tile
+  // t0 + l 8 (s1 - s0) + c)
   const bf16_t* Kh;
   const bf16_t* Kl;
-  const int* lo_idx;
+  const int4* lo_rg;
   int nseg;            // 256-column segments (ld / 256)
 };
 
@@ -79,7 +82,6 @@ struct EnsArgs {
   const int* wpc;
   double* Psk;
   int sk_tr;
-  const bf16_t* zero;   // kSegChunksH zero tiles (lo pieces of blocks without lo terms)
 };
 
 // members of a workgroup's geometry: controller fields read once
@@ -308,11 +310,14 @@ __device__ __forceinline__ void unit_product_slab(const float* __restrict__ Kt, 
         a[u][rf] = RSFG_KLOAD(
             reinterpret_cast<gv4*>(ka[rf] + (size_t)(kc + u) * 256));
   };
+  // the first operator groups are requested before the first slip rates
+  // are written to LDS (that waits for the slip-rate loads only)
   vload(kc0);
-  vstore(0);
-  __syncthreads();
 #pragma unroll
   for (int d = 0; d < D - 1; ++d) kload(kc0 + d * U, A[d]);
+  __builtin_amdgcn_sched_barrier(0);
+  vstore(0);
+  __syncthreads();
   int buf = 0;
   for (int seg = kc0; seg < kc1; seg += 16) {
     // the last segment reloads itself (loads stay unconditional); its copy
@@ -443,11 +448,14 @@ __device__ __forceinline__ void unit_product_slab_h(const bf16_t* const (&kab)[R
       for (int rf = 0; rf < RF; ++rf)
         a[u][rf] = RSFG_KLOAD(reinterpret_cast<gu4*>(ka[rf] + (size_t)(kc - kc0 + u) * kTileH));
   };
+  // the first operator groups are requested before the first slip rates
+  // are written to LDS (that waits for the slip-rate loads only)
   vload(kc0);
-  vstore(0);
-  __syncthreads();
 #pragma unroll
   for (int d = 0; d < D - 1; ++d) kload(kc0 + d * U, A[d]);
+  __builtin_amdgcn_sched_barrier(0);
+  vstore(0);
+  __syncthreads();
   int buf = 0;
   for (int seg = kc0; seg < kc1; seg += kSegChunksH) {
     vload(seg + kSegChunksH < kc1 ? seg + kSegChunksH : seg);
@@ -617,6 +625,20 @@ __device__ __forceinline__ void member_patch_stage(const StageArgs& a, const Mem
 
 // true if any member of geometry gg is still running (every thread of
 // the workgroup gets the answer)
+// running flag of every geometry (some member not halted), once per
+// workgroup of a stream-K product (flags change only between launches)
+constexpr int kMaxGeo = 64;
+__device__ __forceinline__ void running_flags(const EnsArgs& e, int* srun) {
+  for (int g = 0; g < e.ngeo; ++g) {
+    const EnsGeo& gg = e.geo[g];
+    int run = 0;
+    for (int m = threadIdx.x; m < gg.nm; m += blockDim.x) run |= !e.args[gg.m0 + m].ctl->halted;
+    run = __syncthreads_or(run);
+    if (threadIdx.x == 0) srun[g] = run;
+  }
+  __syncthreads();
+}
+
 __device__ __forceinline__ bool geometry_running(const EnsArgs& e, const EnsGeo& gg) {
   int run = 0;
   for (int m = threadIdx.x; m < gg.nm; m += blockDim.x) run |= !e.args[gg.m0 + m].ctl->halted;
@@ -664,13 +686,15 @@ __global__ __launch_bounds__(64 * W) void k_prod_b(EnsArgs e) {
 template <int RF, int MF, int W, int U, int D>
 __global__ __launch_bounds__(64 * W) void k_prod_sk(EnsArgs e) {
   __shared__ float sV[2 * 16 * MF * kSlabRow];
+  __shared__ int srun[kMaxGeo];
   constexpr int NM = 16 * MF, TR = 16 * RF * W;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int k1 = e.wpc[blockIdx.x + 1];
+  running_flags(e, srun);
   for (int k = e.wpc[blockIdx.x]; k < k1; ++k) {
     const int4 pc = e.pieces[k];
     const EnsGeo& gg = e.geo[pc.x];
-    if (!geometry_running(e, gg)) continue;
+    if (!srun[pc.x]) continue;
     const int rw = pc.y + wave * 16 * RF;
     double acc[RF][MF][4];
     unit_product_slab<RF, MF, W, U, D>(gg.Kt, gg.ld, rw, gg.V[e.s & 1], pc.z, pc.w, sV, acc);
@@ -688,31 +712,34 @@ __global__ __launch_bounds__(64 * W) void k_prod_sk(EnsArgs e) {
 }
 
 // Stream-K product over the two-byte operator.  A piece whose geometry
-// field carries kLoPiece is the near-field lo part of one segment of its row
-// group: each 16-row block takes its lo tiles of that segment, or the zero
-// tiles where it has none.  Its partials are one more piece of the row
-// group, added by the epilogue after the hi pieces.
+// field carries kLoPiece streams the lo tiles of its row group over a range
+// of the group's lo hull (the row group lies in one 256-row group); its
+// partials are one more piece of the row group, added by the epilogue after
+// the hi pieces.
 constexpr int kLoPiece = 1 << 16;
 
 template <int RF, int MF, int W, int U, int D>
 __global__ __launch_bounds__(64 * W) void k_prod_sk_h(EnsArgs e) {
   __shared__ bf16_t sV[2 * 3 * 16 * MF * kSlabRowH];
+  __shared__ int srun[kMaxGeo];
   constexpr int NM = 16 * MF, TR = 16 * RF * W;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int k1 = e.wpc[blockIdx.x + 1];
+  running_flags(e, srun);
   for (int k = e.wpc[blockIdx.x]; k < k1; ++k) {
     const int4 pc = e.pieces[k];
     const EnsGeo& gg = e.geo[pc.x & (kLoPiece - 1)];
-    if (!geometry_running(e, gg)) continue;
+    if (!srun[pc.x & (kLoPiece - 1)]) continue;
     const int rw = pc.y + wave * 16 * RF;
     const int kc0 = pc.z / 2, kc1 = pc.w / 2;   // 32-column chunks
     const bf16_t* kab[RF];
     if (pc.x & kLoPiece) {
-      const int seg = pc.z / 16;
+      const int4 lh = gg.lo_rg[pc.y / 256];
+      const int nkh = kSegChunksH * (lh.y - lh.x);
 #pragma unroll
       for (int rf = 0; rf < RF; ++rf) {
-        const int q = gg.lo_idx[(size_t)(rw / 16 + rf) * gg.nseg + seg];
-        kab[rf] = q >= 0 ? gg.Kl + (size_t)q * kSegChunksH * kTileH : e.zero;
+        const int bl = rw / 16 + rf - 16 * (pc.y / 256);
+        kab[rf] = gg.Kl + ((size_t)lh.z + (size_t)bl * nkh + (kc0 - kSegChunksH * lh.x)) * kTileH;
       }
     } else {
       const long nk = gg.ld / 32;
@@ -758,15 +785,15 @@ __global__ void k_tile_h(const float* __restrict__ K, long ldk, int rows, int nr
       v4u{h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16)};
 }
 
-// the lo tiles bf16(K - hi) of the near-field (16-row block, segment)
-// groups grp[0 .. ngrp): group q's 8 tiles at Kl + (8 q + c) kTileH, lane
-// layout of k_tile_h
+// the lo tiles bf16(K - hi) of (16-row block, segment) pairs grp[0 ..
+// ngrp) = (block, segment, first tile): the pair's 8 tiles at Kl + (first
+// tile + c) kTileH, lane layout of k_tile_h
 __global__ void k_tile_lo(const float* __restrict__ K, long ldk, int rows,
-                          const int2* __restrict__ grp, int ngrp, bf16_t* __restrict__ Kl) {
+                          const int4* __restrict__ grp, int ngrp, bf16_t* __restrict__ Kl) {
   const long t = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   if (t >= (long)ngrp * kSegChunksH) return;
   const int l = threadIdx.x & 63, i = l & 15, j = l >> 4;
-  const int2 bs = grp[t / kSegChunksH];
+  const int4 bs = grp[t / kSegChunksH];
   const long row = 16L * bs.x + i, col = 256L * bs.y + 32 * (t % kSegChunksH) + 8 * j;
   uint32_t lo[8];
 #pragma unroll
@@ -778,7 +805,7 @@ __global__ void k_tile_lo(const float* __restrict__ K, long ldk, int rows,
     for (int r = 0; r < 4; ++r)
       lo[4 * q + r] = bf16_rn(v[r] - __uint_as_float(bf16_rn(v[r]) << 16));
   }
-  *reinterpret_cast<v4u*>(Kl + t * kTileH + 8 * l) = v4u{
+  *reinterpret_cast<v4u*>(Kl + ((size_t)bs.z + t % kSegChunksH) * kTileH + 8 * l) = v4u{
       lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16), lo[4] | (lo[5] << 16), lo[6] | (lo[7] << 16)};
 }
 
@@ -910,7 +937,8 @@ static const EnsCfg kEnsCfgs[] = {
     {4, 4, 1, 1024, 2, 1, 0}, {4, 2, 1, 1024, 2, 1, 0}, {2, 4, 1, 1024, 2, 1, 0},
     {4, 4, 1, 1024, 4, 1, 0}, {2, 8, 1, 1024, 2, 1, 0}, {4, 4, 1, 0, 4, 1, 1},
     {4, 4, 1, 0, 2, 1, 2},    {4, 4, 1, 0, 4, 1, 2},    {2, 8, 1, 0, 4, 1, 1},
-    {2, 4, 1, 0, 8, 1, 1},    {2, 4, 1, 0, 4, 1, 2}};
+    {2, 4, 1, 0, 8, 1, 1},    {2, 4, 1, 0, 4, 1, 2},    {2, 8, 1, 0, 8, 1, 1},
+    {2, 8, 2, 0, 4, 1, 1}};
 constexpr int kEnsNCfg = sizeof(kEnsCfgs) / sizeof(kEnsCfgs[0]);
 
 static int cfg_rows(int cfg) { return 16 * kEnsCfgs[cfg].rf * kEnsCfgs[cfg].w; }
@@ -962,7 +990,9 @@ static EnsProdFn prod_fn_mf(int cfg) {
     case 16: if constexpr (MF <= 2) return launch_prod_sk<4, MF, 4, 1, 4>; else return nullptr;
     case 17: return launch_prod_sk<2, MF, 8, 1, 4>;
     case 18: if constexpr (MF <= 2) return launch_prod_sk<2, MF, 4, 1, 8>; else return nullptr;
-    default: if constexpr (MF <= 2) return launch_prod_sk<2, MF, 4, 1, 4>; else return nullptr;
+    case 19: if constexpr (MF <= 2) return launch_prod_sk<2, MF, 4, 1, 4>; else return nullptr;
+    case 20: if constexpr (MF <= 2) return launch_prod_sk<2, MF, 8, 1, 8>; else return nullptr;
+    default: if constexpr (MF <= 2) return launch_prod_sk<2, MF, 8, 2, 4>; else return nullptr;
   }
 }
 
@@ -977,6 +1007,8 @@ static EnsProdFn prod_fn_h_mf(int cfg) {
     case 17: return launch_prod_sk_h<2, MF, 8, 1, 4>;
     case 18: return launch_prod_sk_h<2, MF, 4, 1, 8>;
     case 19: return launch_prod_sk_h<2, MF, 4, 1, 4>;
+    case 20: return launch_prod_sk_h<2, MF, 8, 1, 8>;
+    case 21: return launch_prod_sk_h<2, MF, 8, 2, 4>;
     default: return nullptr;
   }
 }
@@ -1041,9 +1073,8 @@ struct rsfg_ensemble {
   double nf_tol = 0.0;         // tolerance the lo groups were chosen with (0: not built)
   bf16_t* d_Kh = nullptr;
   bf16_t* d_Kl = nullptr;
-  int* d_loidx = nullptr;
-  bf16_t* d_zero = nullptr;
-  std::vector<std::vector<int>> lo_host;   // lo_idx per geometry (host copy for plans)
+  int4* d_lorg = nullptr;
+  std::vector<std::vector<int4>> lo_host;   // lo_rg per geometry (host copy for plans)
   double h_bytes = 0.0;        // operator bytes one two-byte product streams
   long long* d_halted = nullptr;
   hipStream_t stream = nullptr;
@@ -1056,10 +1087,9 @@ struct rsfg_ensemble {
 static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR, int nwg,
                          std::vector<int4>& tiles) {
   // with the two-byte operator a row group's items are its segments, then
-  // one lo piece per segment in which one of its 16-row blocks has lo tiles
+  // the segments of its lo hull
   struct RowGroup {
-    int g, r, nseg;
-    std::vector<int> lo;
+    int g, r, nseg, s0, s1;
   };
   const bool h = e->prec == RSFG_PREC_BF16NF;
   std::vector<RowGroup> rgs;
@@ -1068,19 +1098,16 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
     const EnsGeo& q = e->geo[g];
     rg0[g] = (int)rgs.size();
     for (int r = 0; r < q.rows; r += TR) {
-      RowGroup x{g, r, (int)(q.ld / 256), {}};
-      if (h)
-        for (int s = 0; s < q.nseg; ++s)
-          for (int b = r / 16; b < (r + TR) / 16; ++b)
-            if (e->lo_host[g][(size_t)b * q.nseg + s] >= 0) {
-              x.lo.push_back(s);
-              break;
-            }
+      RowGroup x{g, r, (int)(q.ld / 256), 0, 0};
+      if (h) {
+        x.s0 = e->lo_host[g][r / 256].x;
+        x.s1 = e->lo_host[g][r / 256].y;
+      }
       rgs.push_back(x);
     }
   }
   long long T = 0;
-  for (const auto& x : rgs) T += x.nseg + (long long)x.lo.size();
+  for (const auto& x : rgs) T += x.nseg + x.s1 - x.s0;
   std::vector<int4> pieces;
   std::vector<int> wpc(nwg + 1);
   std::vector<int2> rgtab(rgs.size(), make_int2(0, 0));
@@ -1092,16 +1119,17 @@ static int build_sk_plan(rsfg_ensemble* e, const std::vector<int>& order, int TR
     const long long end = (long long)(w + 1) * T / nwg;
     while (item < end) {
       const RowGroup& x = rgs[ri];
-      const int nit = x.nseg + (int)x.lo.size();
+      const int nit = x.nseg + x.s1 - x.s0;
       if (rgtab[ri].y == 0) rgtab[ri].x = (int)pieces.size();
       rgtab[ri].y++;
-      int take = 1;
+      int take;
       if (seg < x.nseg) {   // a run of hi segments
         take = (int)std::min<long long>(end - item, x.nseg - seg);
         pieces.push_back(make_int4(x.g, x.r, 16 * seg, 16 * (seg + take)));
-      } else {              // the lo piece of one segment
-        const int s = x.lo[seg - x.nseg];
-        pieces.push_back(make_int4(x.g | kLoPiece, x.r, 16 * s, 16 * (s + 1)));
+      } else {              // a run of the lo hull's segments
+        const int s = x.s0 + seg - x.nseg;
+        take = (int)std::min<long long>(end - item, x.s1 - s);
+        pieces.push_back(make_int4(x.g | kLoPiece, x.r, 16 * s, 16 * (s + take)));
       }
       item += take;
       seg += take;
@@ -1228,7 +1256,6 @@ static EnsArgs ens_args(rsfg_ensemble* e, int s) {
   a.wpc = sk ? e->d_wpc : nullptr;
   a.Psk = sk ? e->d_Psk : nullptr;
   a.sk_tr = sk ? cfg_rows(e->cfg) : 0;
-  a.zero = e->d_zero;
   return a;
 }
 
@@ -1291,31 +1318,29 @@ static void ens_attempt(rsfg_ensemble* e, int S) {
 
 // The two-byte operator of every geometry from its FP32 operator: hi tiles
 // of every entry; per row, the 256-column segments in decreasing energy
-// until the segments left out hold at most tol^2 of the row's energy, and
-// lo tiles for each (16-row block, segment) that one of its rows takes.
+// until the segments left out hold at most tol^2 of the row's energy; per
+// 256-row group, lo tiles of all its rows over the hull of the segments its
+// rows take (one dense run per group: streamed like the hi segments, with
+// lo for a few more entries than the rows need).
 static int ens_build_h(rsfg_ensemble* e, double tol) {
   e->nf_tol = 0.0;
   hipFree(e->d_Kh);
   hipFree(e->d_Kl);
-  hipFree(e->d_loidx);
+  hipFree(e->d_lorg);
   e->d_Kh = nullptr;
   e->d_Kl = nullptr;
-  e->d_loidx = nullptr;
-  e->lo_host.assign(e->ngeo, std::vector<int>());
-  size_t kh = 0, nidx = 0;
+  e->d_lorg = nullptr;
+  e->lo_host.assign(e->ngeo, std::vector<int4>());
+  size_t kh = 0, nrg = 0;
   for (auto& q : e->geo) {
     q.nseg = (int)(q.ld / 256);
     kh += (size_t)q.rows_pad * q.ld;
-    nidx += (size_t)(q.rows_pad / 16) * q.nseg;
-  }
-  if (!e->d_zero) {
-    RSFG_HIP(hipMalloc(&e->d_zero, sizeof(bf16_t) * kSegChunksH * kTileH));
-    RSFG_HIP(hipMemset(e->d_zero, 0, sizeof(bf16_t) * kSegChunksH * kTileH));
+    nrg += (size_t)(q.rows_pad / 256);
   }
   RSFG_HIP(hipMalloc(&e->d_Kh, sizeof(bf16_t) * kh));
-  RSFG_HIP(hipMalloc(&e->d_loidx, sizeof(int) * nidx));
-  std::vector<std::vector<int2>> grp(e->ngeo);
-  size_t ngrp = 0, ko = 0, io = 0;
+  RSFG_HIP(hipMalloc(&e->d_lorg, sizeof(int4) * nrg));
+  std::vector<std::vector<int4>> grp(e->ngeo);
+  size_t ntl = 0, ko = 0, io = 0;
   for (int g = 0; g < e->ngeo; ++g) {
     EnsGeo& q = e->geo[g];
     const int nrb = q.rows_pad / 16;
@@ -1338,8 +1363,11 @@ static int ens_build_h(rsfg_ensemble* e, double tol) {
     if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
     hipFree(dE);
     if (he != hipSuccess) return hip_fail(he, "rsfg_ensemble_set_operator", __FILE__, __LINE__);
-    std::vector<char> flag((size_t)nrb * q.nseg, 0);
-    std::vector<int> order(q.nseg);
+    // hull of the segments each 256-row group's rows take
+    const int ng = q.rows_pad / 256;
+    std::vector<int4>& rg = e->lo_host[g];
+    rg.assign(ng, make_int4(0, 0, 0, 0));
+    std::vector<int> lo(ng, q.nseg), hi(ng, -1), order(q.nseg);
     for (int r = 0; r < q.rows; ++r) {
       const double* er = E.data() + (size_t)r * q.nseg;
       double total = 0.0;
@@ -1348,47 +1376,48 @@ static int ens_build_h(rsfg_ensemble* e, double tol) {
       std::sort(order.begin(), order.end(), [&](int a, int b) { return er[a] > er[b]; });
       double left = total;
       for (int k = 0; k < q.nseg && left > tol * tol * total; ++k) {
-        flag[(size_t)(r / 16) * q.nseg + order[k]] = 1;
+        lo[r / 256] = std::min(lo[r / 256], order[k]);
+        hi[r / 256] = std::max(hi[r / 256], order[k]);
         left -= er[order[k]];
       }
     }
-    std::vector<int>& idx = e->lo_host[g];
-    idx.assign((size_t)nrb * q.nseg, -1);
-    for (int b = 0; b < nrb; ++b)
-      for (int s = 0; s < q.nseg; ++s)
-        if (flag[(size_t)b * q.nseg + s]) {
-          idx[(size_t)b * q.nseg + s] = (int)grp[g].size();
-          grp[g].push_back(make_int2(b, s));
-        }
-    q.lo_idx = e->d_loidx + io;
-    RSFG_HIP(hipMemcpy(const_cast<int*>(q.lo_idx), idx.data(), sizeof(int) * idx.size(),
+    for (int G = 0; G < ng; ++G) {
+      if (hi[G] < lo[G]) continue;
+      rg[G] = make_int4(lo[G], hi[G] + 1, (int)ntl, 0);
+      const int nh = hi[G] + 1 - lo[G];
+      for (int l = 0; l < 16; ++l)
+        for (int s = lo[G]; s <= hi[G]; ++s)
+          grp[g].push_back(make_int4(16 * G + l, s, (int)(ntl + ((size_t)l * nh + s - lo[G]) * kSegChunksH), 0));
+      ntl += (size_t)16 * nh * kSegChunksH;
+      RSFG_CHECK(ntl < (size_t)1 << 31, RSFG_EINVAL,
+                 "rsfg_ensemble_set_operator: too many near-field tiles");
+    }
+    q.lo_rg = e->d_lorg + io;
+    RSFG_HIP(hipMemcpy(const_cast<int4*>(q.lo_rg), rg.data(), sizeof(int4) * rg.size(),
                        hipMemcpyHostToDevice));
-    io += idx.size();
-    ngrp += grp[g].size();
+    io += rg.size();
   }
-  RSFG_HIP(hipMalloc(&e->d_Kl, sizeof(bf16_t) * std::max<size_t>(1, ngrp) * kSegChunksH * kTileH));
-  size_t go = 0;
+  RSFG_HIP(hipMalloc(&e->d_Kl, sizeof(bf16_t) * std::max<size_t>(1, ntl) * kTileH));
   double hb = 0.0;
   for (int g = 0; g < e->ngeo; ++g) {
     EnsGeo& q = e->geo[g];
-    q.Kl = e->d_Kl + go * kSegChunksH * kTileH;
+    q.Kl = e->d_Kl;   // hull tile offsets are global over the geometries
     hb += 2.0 * q.rows_pad * q.ld + 2.0 * grp[g].size() * kSegChunksH * kTileH;
     if (!grp[g].empty()) {
-      int2* dg = nullptr;
-      RSFG_HIP(hipMalloc(&dg, sizeof(int2) * grp[g].size()));
-      hipError_t he = hipMemcpy(dg, grp[g].data(), sizeof(int2) * grp[g].size(),
+      int4* dg = nullptr;
+      RSFG_HIP(hipMalloc(&dg, sizeof(int4) * grp[g].size()));
+      hipError_t he = hipMemcpy(dg, grp[g].data(), sizeof(int4) * grp[g].size(),
                                 hipMemcpyHostToDevice);
       const long nt = (long)grp[g].size() * kSegChunksH;
       if (he == hipSuccess) {
         hipLaunchKernelGGL(k_tile_lo, dim3((unsigned)((nt + 3) / 4)), dim3(256), 0, e->stream,
-                           q.K, q.ldk, q.rows, dg, (int)grp[g].size(), const_cast<bf16_t*>(q.Kl));
+                           q.K, q.ldk, q.rows, dg, (int)grp[g].size(), e->d_Kl);
         he = hipGetLastError();
       }
       if (he == hipSuccess) he = hipStreamSynchronize(e->stream);
       hipFree(dg);
       if (he != hipSuccess) return hip_fail(he, "rsfg_ensemble_set_operator", __FILE__, __LINE__);
     }
-    go += grp[g].size();
   }
   RSFG_HIP(hipStreamSynchronize(e->stream));
   e->h_bytes = hb;
@@ -1401,6 +1430,7 @@ extern "C" {
 int rsfg_ensemble_create(rsfg_handle* const* geoms, int ngeom, int members_per_geom,
                          rsfg_ensemble** out) {
   RSFG_CHECK(geoms && out && ngeom > 0, RSFG_EINVAL, "rsfg_ensemble_create: bad arguments");
+  RSFG_CHECK(ngeom <= kMaxGeo, RSFG_EINVAL, "rsfg_ensemble_create: at most 64 geometries");
   RSFG_CHECK(members_per_geom == 16 || members_per_geom == 32 || members_per_geom == 64,
              RSFG_EINVAL, "rsfg_ensemble_create: members per geometry must be 16, 32 or 64");
   *out = nullptr;
@@ -1833,8 +1863,7 @@ int rsfg_ensemble_destroy(rsfg_ensemble* e) {
   hipFree(e->d_Kt);
   hipFree(e->d_Kh);
   hipFree(e->d_Kl);
-  hipFree(e->d_loidx);
-  hipFree(e->d_zero);
+  hipFree(e->d_lorg);
   hipFree(e->d_halted);
   if (e->ev0) hipEventDestroy(e->ev0);
   if (e->ev1) hipEventDestroy(e->ev1);

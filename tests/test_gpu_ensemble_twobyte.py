@@ -87,7 +87,7 @@ def test_two_byte_product_against_the_fp32_operator(nm):
             assert "does not fit" in str(err) and sh[6] == 0, (cfg, sh)
             continue
         tried.append((cfg, worst(_first_step(ens, X0s, lib.PREC_BF16NF))))
-    assert len(tried) == 6, tried
+    assert len(tried) == 8, tried
     assert max(e for _, e in tried) < 2e-5, tried
     b_coarse = ens.set_operator(lib.PREC_BF16NF, nf_tol=0.5)
     assert b_coarse < bh
