@@ -622,15 +622,21 @@ def ensemble_bench(args):
     ev32 = [m.events() for m in ens.members]
 
     def onset_diff(evs_a, evs_b):
-        worst, nother = 0.0, 0
+        """worst onset difference over all events, members with another
+        event count, and the first-onset differences of every member
+        (median, 90th percentile, worst)"""
+        worst, nother, first = 0.0, 0, []
         for ea, eb in zip(evs_a, evs_b):
             oa = ea[ea[:, 1] > 0, 0] / SEC_PER_YEAR if len(ea) else np.zeros(0)
             ob = eb[eb[:, 1] > 0, 0] / SEC_PER_YEAR if len(eb) else np.zeros(0)
             k = min(len(oa), len(ob))
             if k:
                 worst = max(worst, float(np.abs(oa[:k] - ob[:k]).max()))
+                first.append(abs(float(oa[0] - ob[0])))
             nother += int(len(oa) != len(ob))
-        return worst, nother
+        f = np.array(first) if first else np.zeros(1)
+        return worst, nother, dict(median=float(np.median(f)), p90=float(np.percentile(f, 90)),
+                                   worst=float(f.max()))
 
     if two:
         # the same members on the two-byte operator
@@ -656,15 +662,15 @@ def ensemble_bench(args):
             walls_h.append(time.perf_counter() - t0)
         wall_h = float(np.mean(walls_h))
         evh = [m.events() for m in ens.members]
-        on_h32, n_h32 = onset_diff(evh, ev32)
+        on_h32, n_h32, first_h32 = onset_diff(evh, ev32)
         steps_h = [q["steps"] + q["rejects"] for q in st_h]
         log(f"two-byte ensemble: {M} members x {years:g} yr in {wall_h:.2f} s "
             f"({M * years / (wall_h / 3600.0):.3e} sim-yr/h); attempts per member min "
             f"{min(steps_h)} max {max(steps_h)}; onsets vs the FP32 ensemble: worst "
-            f"{on_h32:.2e} yr, {n_h32} members with another count")
+            f"{on_h32:.2e} yr, first onsets {first_h32}, {n_h32} members with another count")
     # the same members one after another through the single-fault solver
     nsing = M if args.singles is None else min(args.singles, M)
-    single_s, worst_on, n_on = 0.0, 0.0, 0
+    single_s, worst_on, n_on, ev_s32 = 0.0, 0.0, 0, []
     for i in range(nsing):
         gi, f, X0 = members[i]
         s = geoms[gi].member()
@@ -673,6 +679,7 @@ def ensemble_bench(args):
         s.solve(t_end, rtol=args.rtol, prec=lib.PREC_FP32, steps_per_launch=64)
         single_s += time.perf_counter() - t0
         e1, e2 = s.events(), ev32[i]
+        ev_s32.append(e1)
         o1 = e1[e1[:, 1] > 0, 0] / SEC_PER_YEAR if len(e1) else np.zeros(0)
         o2 = e2[e2[:, 1] > 0, 0] / SEC_PER_YEAR if len(e2) else np.zeros(0)
         k = min(len(o1), len(o2))
@@ -699,10 +706,13 @@ def ensemble_bench(args):
             s.close()
             if (i + 1) % nm == 0:
                 log(f"bf16nf singles: {i + 1} of {nsing} done, {single_h:.1f} s")
-        on_hs, n_hs = onset_diff(evh[:nsing], evs)
+        on_hs, n_hs, first_hs = onset_diff(evh[:nsing], evs)
+        on_ss, n_ss, first_ss = onset_diff(evs, ev_s32)
         seq_h = nsing * years / (single_h / 3600.0)
         log(f"bf16nf singles: {nsing} runs in {single_h:.2f} s ({seq_h:.3e} sim-yr/h); onsets vs "
-            f"the two-byte ensemble: worst {on_hs:.2e} yr, {n_hs} members with another count")
+            f"the two-byte ensemble: worst {on_hs:.2e} yr, first onsets {first_hs}, {n_hs} "
+            f"members with another count; vs the FP32 single runs: worst {on_ss:.2e} yr, first "
+            f"onsets {first_ss}, {n_ss} members with another count")
     # roofline of the batched product kernel at the tuned shape (HIP events
     # around repeated launches of a middle stage, on the ensemble's stream)
     launch_s = prod_ms * 1e-3
@@ -765,9 +775,14 @@ def ensemble_bench(args):
             speedup_vs_sequential_bf16nf=rate_h / seq_h,
             speedup_vs_sequential_fp32=rate_h / seq_rate,
             parity_vs_fp32_ensemble=dict(worst_onset_diff_yr=on_h32,
+                                         first_onset_diff_yr=first_h32,
                                          members_with_other_count=n_h32),
             parity_vs_single_bf16nf=dict(worst_onset_diff_yr=on_hs,
+                                         first_onset_diff_yr=first_hs,
                                          members_with_other_count=n_hs),
+            single_bf16nf_vs_single_fp32=dict(worst_onset_diff_yr=on_ss,
+                                              first_onset_diff_yr=first_ss,
+                                              members_with_other_count=n_ss),
             attempts_per_member=dict(min=min(steps_h), max=max(steps_h)),
             tuner=dict(choice=list(shapes[choice_h]), wall_s=tune_h,
                        ms={str(tuple(shapes[i])): ms_h[i] for i in range(len(ms_h)) if ms_h[i] > 0}),
