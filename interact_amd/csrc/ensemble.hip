@@ -748,7 +748,9 @@ __global__ __launch_bounds__(64 * W) void k_prod_sk_h(EnsArgs e) {
     }
     double acc[RF][MF][4];
     unit_product_slab_h<RF, MF, W, U, D>(kab, gg.V[e.s & 1], gg.ld, kc0, kc1, sV, acc);
-    double* P = e.Psk + (size_t)k * TR * NM;
+    // FP32 partials (the epilogue adds them in FP64): their rounding, 6e-8
+    // of a piece's sum, is far below the two-byte operator's 1e-5
+    float* P = reinterpret_cast<float*>(e.Psk) + (size_t)k * TR * NM;
 #pragma unroll
     for (int rf = 0; rf < RF; ++rf)
 #pragma unroll
@@ -756,7 +758,7 @@ __global__ __launch_bounds__(64 * W) void k_prod_sk_h(EnsArgs e) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = wave * 16 * RF + 16 * rf + 4 * (lane >> 4) + r;
-          P[(size_t)row * NM + 16 * mf + (lane & 15)] = acc[rf][mf][r];
+          P[(size_t)row * NM + 16 * mf + (lane & 15)] = (float)acc[rf][mf][r];
         }
   }
 }
@@ -833,8 +835,9 @@ __global__ void k_seg_energy(const float* __restrict__ K, long ldk, int rows, in
 // member's fields are read as 128-byte runs of 16 consecutive patches.  The
 // stressing rates of a row are its partials summed over the column blocks
 // in order; then the single-fault stage update, and each member's
-// reductions over the consecutive lanes holding its patches.
-template <int NC, int G, int MF>
+// reductions over the consecutive lanes holding its patches.  PT: the
+// partials' type (float for the two-byte product, which runs stream-K only).
+template <int NC, int G, int MF, typename PT = double>
 __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e) {
   constexpr int NPT = kEpiPatches;
   constexpr int NM = 16 * MF;
@@ -870,20 +873,22 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
     // added in order; the loads of a group of kSumGroup partials of every
     // row are issued together (the sum is a chain of dependent loads
     // otherwise)
-    const double* q[G];
+    const PT* q[G];
     size_t stride;
     int cnt;
-    if (e.sk_tr > 0) {
+    if (!std::is_same<PT, double>::value || e.sk_tr > 0) {
       stride = (size_t)e.sk_tr * NM;
       cnt = tile.w;
 #pragma unroll
       for (int k = 0; k < G; ++k)
-        q[k] = e.Psk + (size_t)tile.z * stride + (size_t)((p * G + k) % e.sk_tr) * NM + m;
+        q[k] = reinterpret_cast<const PT*>(e.Psk) + (size_t)tile.z * stride +
+               (size_t)((p * G + k) % e.sk_tr) * NM + m;
     } else {
       stride = cbs;
       cnt = ncb;
 #pragma unroll
-      for (int k = 0; k < G; ++k) q[k] = gg.P + (size_t)(p * G + k) * NM + m;
+      for (int k = 0; k < G; ++k)
+        q[k] = reinterpret_cast<const PT*>(gg.P) + (size_t)(p * G + k) * NM + m;
     }
     double td[G];
 #pragma unroll
@@ -961,9 +966,10 @@ static void launch_prod_sk_h(const EnsArgs& e, int nblocks, hipStream_t st) {
   hipLaunchKernelGGL((k_prod_sk_h<RF, MF, W, U, D>), dim3(nblocks), dim3(64 * W), 0, st, e);
 }
 
-template <int NC, int G, int MF>
+template <int NC, int G, int MF, typename PT = double>
 static void launch_epi_b(const EnsArgs& e, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_epi_b<NC, G, MF>), dim3(nblocks, MF), dim3(kEpiPatches * 16), 0, st, e);
+  hipLaunchKernelGGL((k_epi_b<NC, G, MF, PT>), dim3(nblocks, MF), dim3(kEpiPatches * 16), 0, st,
+                     e);
 }
 
 // shapes with four or more 16-row blocks per wave only with up to two
@@ -1027,7 +1033,16 @@ static EnsProdFn prod_fn(int nm, int cfg, bool h = false) {
   return nullptr;
 }
 
-static EnsEpiFn epi_fn(int ncomp, int G, int nm) {
+// the epilogue for the layout and member count; h: after the two-byte
+// product (FP32 partials; 16 or 32 members)
+static EnsEpiFn epi_fn(int ncomp, int G, int nm, bool h = false) {
+  if (h) {
+    if (ncomp == 2 && G == 2 && nm == 16) return launch_epi_b<2, 2, 1, float>;
+    if (ncomp == 2 && G == 2 && nm == 32) return launch_epi_b<2, 2, 2, float>;
+    if (ncomp == 1 && G == 1 && nm == 16) return launch_epi_b<1, 1, 1, float>;
+    if (ncomp == 1 && G == 1 && nm == 32) return launch_epi_b<1, 1, 2, float>;
+    return nullptr;
+  }
   if (ncomp == 2 && G == 2) {
     if (nm == 16) return launch_epi_b<2, 2, 1>;
     if (nm == 32) return launch_epi_b<2, 2, 2>;
@@ -1262,7 +1277,7 @@ static EnsArgs ens_args(rsfg_ensemble* e, int s) {
 // the stage product and epilogue of stage a.s for every member
 static void ens_stage(rsfg_ensemble* e, const EnsArgs& a) {
   prod_fn(e->nm, e->cfg, e->prec == RSFG_PREC_BF16NF)(a, e->nprod, e->stream);
-  epi_fn(e->ncomp, e->G, e->nm)(a, e->ntiles, e->stream);
+  epi_fn(e->ncomp, e->G, e->nm, e->prec == RSFG_PREC_BF16NF)(a, e->ntiles, e->stream);
 }
 
 static int max_patches(rsfg_ensemble* e) {
@@ -1767,7 +1782,7 @@ int rsfg_ensemble_bench(rsfg_ensemble* e, const rsfg_solver_opts* o, int nrep, d
   a.s = 1;
   ens_stage(e, a);
   EnsProdFn pf = prod_fn(e->nm, e->cfg, e->prec == RSFG_PREC_BF16NF);
-  EnsEpiFn ef = epi_fn(e->ncomp, e->G, e->nm);
+  EnsEpiFn ef = epi_fn(e->ncomp, e->G, e->nm, e->prec == RSFG_PREC_BF16NF);
   float t = 0.f;
   RSFG_HIP(hipEventRecord(e->ev0, e->stream));
   for (int r = 0; r < nrep; ++r) pf(a, e->nprod, e->stream);
