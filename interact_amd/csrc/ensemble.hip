@@ -19,7 +19,10 @@
 // load group, columns per block, depth of each wave's load ring, slip rates
 // shared through LDS or not) are chosen at run time by timing the
 // candidates on the ensemble's own stage (rsfg_ensemble_tune), or pinned
-// with rsfg_ensemble_select.
+// with rsfg_ensemble_select.  rsfg_ensemble_set_operator switches the
+// product to a two-byte operator built from the FP32 one (bf16 hi tiles on
+// v_mfma_f32_16x16x32_bf16, bf16 lo tiles over each row group's near-field
+// segments, the FP32 slip rates as three bf16 terms; stream-K shapes only).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
