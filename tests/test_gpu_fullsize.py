@@ -5,7 +5,8 @@ Ensemble (four planar outlines 40 x 20, 60 x 24, 80 x 32 and 100 x 40 km at
 1 km patches: 1600 to 8000 DOF, 32 seeded friction draws each, 128 members
 in one ensemble): every member runs, and one sampled member per geometry
 agrees with the same member run alone through the single-fault FP32
-solver.
+solver; on the two-byte operator every member keeps its events, with first
+onsets near the FP32 ensemble's.
 
 Adjoint (the 1 km, 8000-DOF fault, 64 seeded receivers): the device
 gradient of the log slip-rate misfit with respect to a, b and D_c of all
@@ -73,6 +74,25 @@ def test_ensemble_at_configured_size_matches_single_runs():
                                                           ens.members[i].monitor())
         assert rmsdev < (0.05 if len(o1) else 1e-4), (i, maxdev, rmsdev)
         single.close()
+    # the same members on the two-byte operator (bf16 hi + near-field lo):
+    # every member has the FP32 ensemble's event count, and the first onsets
+    # differ by a median below 1e-3 yr and a 90th percentile below 2e-2 yr
+    # (single members can move further: one of the 128 moves its first event
+    # by 2.5 yr over 100 yr, as the single-fault bf16nf solver moves it)
+    on32 = [accuracy.onsets(m.events()) for m in ens.members]
+    ens.set_operator(lib.PREC_BF16NF)
+    for m, (f, X0) in zip(ens.members, fields):
+        m.set_state(X0)
+    st = ens.solve(t_end, rtol=1e-8, steps_per_launch=64)
+    assert all(s["status"] == 0 and s["t"] >= t_end for s in st)
+    first = []
+    for i, m in enumerate(ens.members):
+        oh = accuracy.onsets(m.events())
+        assert len(oh) == len(on32[i]), (i, oh, on32[i])
+        if len(oh):
+            first.append(abs(oh[0] - on32[i][0]))
+    assert len(first) >= 32
+    assert np.median(first) < 1e-3 and np.percentile(first, 90) < 2e-2, sorted(first)[-5:]
     ens.close()
     for fs in geoms:
         fs.close()
