@@ -626,8 +626,6 @@ __device__ __forceinline__ void member_patch_stage(const StageArgs& a, const Mem
   for (int f = 0; f < D; ++f) r9[RED_XN2] += y[f] * y[f];
 }
 
-// true if any member of geometry gg is still running (every thread of
-// the workgroup gets the answer)
 // running flag of every geometry (some member not halted), once per
 // workgroup of a stream-K product (flags change only between launches)
 constexpr int kMaxGeo = 64;
@@ -642,6 +640,8 @@ __device__ __forceinline__ void running_flags(const EnsArgs& e, int* srun) {
   __syncthreads();
 }
 
+// true if any member of geometry gg is still running (every thread of
+// the workgroup gets the answer)
 __device__ __forceinline__ bool geometry_running(const EnsArgs& e, const EnsGeo& gg) {
   int run = 0;
   for (int m = threadIdx.x; m < gg.nm; m += blockDim.x) run |= !e.args[gg.m0 + m].ctl->halted;
