@@ -1207,7 +1207,11 @@ static int build_plan(rsfg_ensemble* e, int cfg, const std::vector<char>* run = 
     EnsGeo& q = e->geo[g];
     q.tile0 = (int)tiles.size();
     q.ntiles = (q.n + kEpiPatches - 1) / kEpiPatches;
-    for (int t = 0; t < q.ntiles; ++t) tiles.push_back(make_int4(g, t * kEpiPatches * e->G, 0, 0));
+    // the epilogue covers the geometries the plan covers (all of them but
+    // for a stream-K split over the running ones)
+    if (!SK || !run || (*run)[g])
+      for (int t = 0; t < q.ntiles; ++t)
+        tiles.push_back(make_int4(g, t * kEpiPatches * e->G, 0, 0));
     const int nk = (int)(q.ld / 16);
     if (SK) continue;
     for (int r = 0; r < q.rows; r += TR)
