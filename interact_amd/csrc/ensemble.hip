@@ -517,27 +517,39 @@ struct MemberOps {
   double P[PD], sn[NC], snn, x0[D], ws[D];
 };
 
+// The operands that do not depend on the member's controller fields
+// (coefficients, loading rates, state-only record), requested before the
+// controller snapshot is read, then the ones that do (state and stage
+// derivatives by the controller's slots).
 template <int NC>
-__device__ __forceinline__ void member_operands(const StageArgs& a, const MemberSnap& ms, int s,
-                                                int p, MemberOps<NC>& o) {
-  constexpr int D = Dim<NC>::v;
+__device__ __forceinline__ void member_operands_pre(const StageArgs& a, int s, int p,
+                                                    MemberOps<NC>& o) {
   constexpr int PD = PreDim<NC>::v;
-  const int n = a.n, S = a.tb.s;
-  const double* X = ms.xc ? a.Xb[1] : a.Xb[0];
-  const size_t sst = a.Stg[1] - a.Stg[0];
-  const bool mid = (s > 0) && (s < S - 1);
+  const int n = a.n;
   o.cf = load_coef(a.coef, n, p);
 #pragma unroll
   for (int k = 0; k < NC; ++k) o.sn[k] = a.sinc[NC * p + k];
   o.snn = a.sinc[NC * n + p];
-#pragma unroll
-  for (int f = 0; f < D; ++f) o.x0[f] = o.ws[f] = 0.0;
 #pragma unroll
   for (int k = 0; k < PD; ++k) o.P[k] = 0.0;
   if (s > 0) {
     const double* Ps = a.Pb[s & 1];
 #pragma unroll
     for (int k = 0; k < PD; ++k) o.P[k] = Ps[k * n + p];
+  }
+}
+
+template <int NC>
+__device__ __forceinline__ void member_operands(const StageArgs& a, const MemberSnap& ms, int s,
+                                                int p, MemberOps<NC>& o) {
+  constexpr int D = Dim<NC>::v;
+  const int n = a.n, S = a.tb.s;
+  const double* X = ms.xc ? a.Xb[1] : a.Xb[0];
+  const size_t sst = a.Stg[1] - a.Stg[0];
+  const bool mid = (s > 0) && (s < S - 1);
+#pragma unroll
+  for (int f = 0; f < D; ++f) o.x0[f] = o.ws[f] = 0.0;
+  if (s > 0) {
 #pragma unroll
     for (int f = 0; f < D; ++f) o.x0[f] = X[f * n + p];
     const int nj = mid ? s : S - 1;
@@ -858,45 +870,65 @@ __global__ __launch_bounds__(kEpiPatches * 16, kEpiWaves) void k_epi_b(EnsArgs e
     q.h = c->h;
     snap[m] = q;
   }
-  __syncthreads();
   const int S = e.args[gg.m0].tb.s;
   const bool reduce = (s == 0) || (s == S - 1);
   const int ncb = (int)((gg.ld / 16 + e.cb_chunks - 1) / e.cb_chunks);
   const size_t cbs = (size_t)gg.rows_pad * NM;
   const int ml = tid / NPT, m = mb + ml, pl = tid % NPT, p = r0 / G + pl;
+  const bool live = p < gg.n;
+  // the rows' partials over the pieces (stream-K: the tile lies in one
+  // row group, whose pieces are tile.z .. + tile.w) or the column blocks,
+  // added in order; the loads of a group of kSumGroup partials of every
+  // row are issued together (the sum is a chain of dependent loads
+  // otherwise)
+  const PT* q[G];
+  size_t stride;
+  int cnt;
+  if (!std::is_same<PT, double>::value || e.sk_tr > 0) {
+    stride = (size_t)e.sk_tr * NM;
+    cnt = tile.w;
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+      q[k] = reinterpret_cast<const PT*>(e.Psk) + (size_t)tile.z * stride +
+             (size_t)((p * G + k) % e.sk_tr) * NM + m;
+  } else {
+    stride = cbs;
+    cnt = ncb;
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+      q[k] = reinterpret_cast<const PT*>(gg.P) + (size_t)(p * G + k) * NM + m;
+  }
+  // what does not depend on the controller snapshot is requested before the
+  // barrier that publishes it: the member's coefficients, loading rates and
+  // state-only record, and the first group of partials
+  MemberOps<NC> ops;
+  double t0[G][kSumGroup];
+  const bool first = live && cnt >= kSumGroup;
+  if (live) member_operands_pre<NC>(e.args[gg.m0 + m], s, p, ops);
+  if (first) {
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+#pragma unroll
+      for (int u = 0; u < kSumGroup; ++u) t0[k][u] = q[k][(size_t)u * stride];
+  }
+  __syncthreads();
   double r9[kNRed];
   r9[RED_ERR] = 0.0; r9[RED_VMAX] = 0.0; r9[RED_VSUM] = 0.0; r9[RED_SLIP] = 0.0;
   r9[RED_MU] = 0.0; r9[RED_SMAX] = -1e30; r9[RED_SMIN] = 1e30; r9[RED_XN2] = 0.0;
   r9[RED_DX2] = 0.0;
-  if (p < gg.n && !snap[ml].halted) {
-    MemberOps<NC> ops;
+  if (live && !snap[ml].halted) {
     member_operands<NC>(e.args[gg.m0 + m], snap[ml], s, p, ops);
-    // the rows' partials over the pieces (stream-K: the tile lies in one
-    // row group, whose pieces are tile.z .. + tile.w) or the column blocks,
-    // added in order; the loads of a group of kSumGroup partials of every
-    // row are issued together (the sum is a chain of dependent loads
-    // otherwise)
-    const PT* q[G];
-    size_t stride;
-    int cnt;
-    if (!std::is_same<PT, double>::value || e.sk_tr > 0) {
-      stride = (size_t)e.sk_tr * NM;
-      cnt = tile.w;
-#pragma unroll
-      for (int k = 0; k < G; ++k)
-        q[k] = reinterpret_cast<const PT*>(e.Psk) + (size_t)tile.z * stride +
-               (size_t)((p * G + k) % e.sk_tr) * NM + m;
-    } else {
-      stride = cbs;
-      cnt = ncb;
-#pragma unroll
-      for (int k = 0; k < G; ++k)
-        q[k] = reinterpret_cast<const PT*>(gg.P) + (size_t)(p * G + k) * NM + m;
-    }
     double td[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) td[k] = 0.0;
     int c = 0;
+    if (first) {
+#pragma unroll
+      for (int k = 0; k < G; ++k)
+#pragma unroll
+        for (int u = 0; u < kSumGroup; ++u) td[k] += t0[k][u];
+      c = kSumGroup;
+    }
     for (; c + kSumGroup <= cnt; c += kSumGroup) {
       double t[G][kSumGroup];
 #pragma unroll
