@@ -71,7 +71,14 @@ MATVEC_TRAFFIC = {"c2": {"fp32": 260.1e6}, "c3": {"fp32": 4.0965e9}}
 # (profiles/k_prod_b_ensemble_c4.md)
 PRODUCT_TRAFFIC_C4 = {(32, 4, 1, 0, 8, 1, 1): 449.9e6, (32, 4, 1, 0, 4, 1, 2): 459.4e6,
                       (64, 4, 1, 1024, 2, 1, 0): 462.8e6, (32, 8, 1, 1024, 2, 1, 0): 462.5e6,
-                      (32, 8, 1, 0, 4, 1, 1): 464.8e6, (64, 4, 1, 1024, 4, 1, 0): 465.1e6}
+                      (32, 8, 1, 0, 4, 1, 1): 464.8e6, (64, 4, 1, 1024, 4, 1, 0): 465.1e6,
+                      (32, 8, 1, 0, 8, 1, 1): 468.0e6, (32, 8, 2, 0, 4, 1, 1): 473.7e6}
+# the same for the two-byte product (FP32 partials) by stream-K shape
+# (profiles/k_prod_sk_h_ensemble_c4.md)
+PRODUCT_TRAFFIC_C4_H = {(64, 4, 1, 0, 4, 1, 1): 277.0e6, (64, 4, 1, 0, 4, 1, 2): 277.0e6,
+                        (64, 4, 1, 0, 2, 1, 2): 278.5e6, (32, 8, 1, 0, 4, 1, 1): 271.8e6,
+                        (32, 4, 1, 0, 8, 1, 1): 265.2e6, (32, 4, 1, 0, 4, 1, 2): 268.4e6,
+                        (32, 8, 1, 0, 8, 1, 1): 275.1e6, (32, 8, 2, 0, 4, 1, 1): 281.0e6}
 # HBM bytes per launch of the backward sweep's K^T w at c5 (FP64 operator),
 # from the same two counter runs of `--config c5 --kernels-only`
 # (profiles/adjoint_planar1km_8000dof.md)
@@ -788,7 +795,8 @@ def ensemble_bench(args):
                        ms={str(tuple(shapes[i])): ms_h[i] for i in range(len(ms_h)) if ms_h[i] > 0}),
             roofline=dict(kernel=f"k_prod_sk_h (two-byte operator), shape {shapes[choice_h]}",
                           bound="hbm", achieved=hb / ls / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                          frac=hb / ls / 1e9 / HBM_PEAK_GBS, traffic=None,
+                          frac=hb / ls / 1e9 / HBM_PEAK_GBS,
+                          traffic=PRODUCT_TRAFFIC_C4_H.get(tuple(shapes[choice_h])),
                           bytes_per_launch=hb, lo_bytes=lo_bytes, streamed_bytes=bh,
                           launch_us=prod_h * 1e3, epilogue_us=epi_h * 1e3,
                           stage_us=ms_h[choice_h] * 1e3))
