@@ -73,7 +73,8 @@ PRODUCT_TRAFFIC_C4 = {(32, 4, 1, 0, 8, 1, 1): 449.9e6, (32, 4, 1, 0, 4, 1, 2): 4
                       (64, 4, 1, 1024, 2, 1, 0): 462.8e6, (32, 8, 1, 1024, 2, 1, 0): 462.5e6,
                       (32, 8, 1, 0, 4, 1, 1): 464.8e6, (64, 4, 1, 1024, 4, 1, 0): 465.1e6,
                       (32, 8, 1, 0, 8, 1, 1): 468.0e6, (32, 8, 2, 0, 4, 1, 1): 473.7e6}
-# the same for the two-byte product (FP32 partials) by stream-K shape
+# (the two deeper-ring shapes: profiles/k_prod_sk_h_ensemble_c4.md); the
+# same for the two-byte product (FP32 partials) by stream-K shape
 # (profiles/k_prod_sk_h_ensemble_c4.md)
 PRODUCT_TRAFFIC_C4_H = {(64, 4, 1, 0, 4, 1, 1): 277.0e6, (64, 4, 1, 0, 4, 1, 2): 277.0e6,
                         (64, 4, 1, 0, 2, 1, 2): 278.5e6, (32, 8, 1, 0, 4, 1, 1): 271.8e6,
